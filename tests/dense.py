@@ -1,0 +1,70 @@
+"""Independent dense Hamiltonians (numpy Kronecker products) for small clusters.
+
+Built from textbook spin operators and the couplings, NOT from the bond matrices the
+oracle and the product consume, so a mistake in either (dropped term, wrong sign,
+transposed operand, wrong site order) shows up as a mismatch.
+"""
+import math
+
+import numpy as np
+import scipy.sparse as sp_
+
+
+def site_ops(q):
+    """Standard (Sx, Sy, Sz) in the basis order the reference uses for that tree.
+
+    q=2 (S=1/2 tree): index 0 = up (Sz=+1/2).  q>2 (spin-S tree): digit i has m = i - S.
+    """
+    if q == 2:
+        sp = np.array([[0, 1], [0, 0]], dtype=complex)   # |0>=up <- |1>=down
+        sz = np.diag([0.5, -0.5]).astype(complex)
+    else:
+        S = 0.5 * (q - 1)
+        sp = np.zeros((q, q), dtype=complex)
+        for i in range(q - 1):
+            m = i - S
+            sp[i + 1, i] = math.sqrt((S - m) * (S + m + 1))
+        sz = np.diag([i - S for i in range(q)]).astype(complex)
+    sm = sp.conj().T
+    return 0.5 * (sp + sm), -0.5j * (sp - sm), sz
+
+
+def embed(op, site, n, q):
+    """Operator on ``site`` (0-based; basis index = sum d_k q^k) of an n-site system."""
+    out = sp_.csr_matrix(np.array([[1.0 + 0j]]))
+    for k in range(n - 1, -1, -1):
+        out = sp_.kron(out, sp_.csr_matrix(op) if k == site else sp_.identity(q, format="csr"),
+                       format="csr")
+    return out
+
+
+def kh_dense(cluster, J, Kx, Ky, Kz, q=2, h=0.0, hdir=(1, 1, 1), Jzz=None):
+    """H = sum_b [J S.S + K_gamma S^g S^g] + h sum_k S_k.n  on ``cluster`` (1-based bonds).
+
+    The reference's MODEL 20 writes the bond as diag(Jxx, Jyy, Jzz) with the Kitaev
+    axis coupling K1 = J + K: pass Kx/Ky/Kz as the extra coupling added on that axis.
+    """
+    n = cluster.n
+    sx, sy, sz = site_ops(q)
+    S = [sx, sy, sz]
+    ops = [[embed(a, k, n, q) for a in S] for k in range(n)]
+    D = q ** n
+    H = sp_.csr_matrix((D, D), dtype=complex)
+    for (a, b, t) in cluster.bonds:
+        i, j = a - 1, b - 1
+        K = (Kx, Ky, Kz)[t - 1]
+        for g in range(3):
+            c = J + (K if g == t - 1 else 0.0)
+            H += c * ops[i][g] @ ops[j][g]
+    if h:
+        nvec = np.asarray(hdir, dtype=float)
+        for k in range(n):
+            H += h * sum(nvec[g] * ops[k][g] for g in range(3))
+    return H.toarray() if D <= 4096 else H.tocsr()
+
+
+def model20_couplings(phi_deg):
+    a = math.pi / 180.0 * phi_deg
+    J = math.cos(a)
+    K1 = math.cos(a) + 2 * math.sin(a)
+    return J, K1 - J

@@ -1,0 +1,309 @@
+"""Pins of the CPU oracle (oracle/pned_oracle.c) before it is trusted as the checker.
+
+What pins each oracle function (the reference ships no tests or golden vectors,
+so every pin is a fixture extracted from its text or a mathematical fact):
+
+* lattice tables      -- tests/golden/ref_bond_tables.json (the reference's literal
+                         tables, extracted as text) + honeycomb invariants;
+* orc_hv (H_dot_v)    -- dense Kronecker-product H built from textbook spin operators
+                         (tests/dense.py), S=1/2 with/without [111] field and spin-1;
+                         Hermiticity; the eperbd shift identity;
+* orc_philox          -- Random123 published known-answer vectors;
+* gaussian start      -- moments of 2^20 samples; u1 in (0,1] never gives inf;
+* orc_bisec/vec12     -- scipy.linalg.eigh_tridiagonal;
+* orc_lanczos         -- dense eigvalsh ground energy; alpha/beta reproduce the
+                         tridiagonal of the Krylov basis built by numpy;
+* orc_inverse_iteration -- residual ||Hx - E0 x|| and overlap with the dense GS;
+* orc_tps             -- (L - H)^k psi by dense matrix powers; energies per step;
+* orc_sz_q / DSF CF   -- explicit sums; m=1 Lorentzian closed form; normalisation
+                         integral equals S(q).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import oracle
+from pned_amd import lattice, model
+from tests import dense
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_reference_tables_match_fixture():
+    d = json.load(open(os.path.join(GOLD, "ref_bond_tables.json")))
+    for n, v in d.items():
+        c = lattice.reference_cluster(int(n))
+        assert list(c.ipair) == v["ipair"]
+        assert list(c.ibond_type) == v["ibond_type"]
+
+
+@pytest.mark.parametrize("spec", [8, 12, 16, 18, 24, 32, "2x4", "3x3", "2x5", "3x4", "2x7"])
+def test_honeycomb_invariants(spec):
+    c = lattice.get_cluster(spec)
+    assert c.ibond == 3 * c.n // 2
+    per_site = {k: [] for k in range(1, c.n + 1)}
+    seen = set()
+    for a, b, t in c.bonds:
+        assert a != b
+        key = (min(a, b), max(a, b))
+        assert key not in seen
+        seen.add(key)
+        per_site[a].append(t)
+        per_site[b].append(t)
+    for k, ts in per_site.items():
+        assert sorted(ts) == [1, 2, 3], (k, ts)   # one x, one y, one z bond per site
+
+
+def test_torus44_is_reference_32():
+    c = lattice.torus(4, 4)
+    assert c.n == 32 and c.name == "ref32"
+    # x bonds join 2c-1,2c; the z bond of cell (i, j=0) wraps to j=3 (distance 25)
+    assert c.bonds[0] == (1, 2, 1) and c.bonds[2] == (1, 2 * (3 * 4 + 0) + 2, 3)
+
+
+def _oop(cluster, phi, q=2, h=0.0, tps=False):
+    op = model.build_operator(cluster, 20, phi, q=q, field=h, tps=tps)
+    return op, oracle.OracleOp(op)
+
+
+@pytest.mark.parametrize("spec,phi,h", [(8, 54.0, 0.0), (12, 126.0, 0.0), (12, 126.0, 0.1),
+                                         ("2x3", 77.0, 0.3)])
+def test_hv_vs_dense_s12(spec, phi, h):
+    c = lattice.get_cluster(spec)
+    op, oop = _oop(c, phi, h=h)
+    J, K = dense.model20_couplings(phi)
+    H = dense.kh_dense(c, J, K, K, K, q=2, h=h)
+    rng = np.random.default_rng(1)
+    v = rng.standard_normal(2 ** c.n) + 1j * rng.standard_normal(2 ** c.n)
+    v0 = rng.standard_normal(2 ** c.n) + 1j * rng.standard_normal(2 ** c.n)
+    out = v0.copy()
+    pr = oracle.hv(oop, v, out)
+    ref = v0 + H @ v
+    assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
+    assert abs(pr - np.vdot(v, H @ v)) <= 1e-12 * abs(np.vdot(v, H @ v))
+
+
+def test_hv_vs_dense_spin1():
+    c = lattice.reference_cluster(8)
+    op, oop = _oop(c, 108.0, q=3)
+    J, K = dense.model20_couplings(108.0)
+    H = dense.kh_dense(c, J, K, K, K, q=3)
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(3 ** 8) + 1j * rng.standard_normal(3 ** 8)
+    out = np.zeros_like(v)
+    oracle.hv(oop, v, out)
+    assert np.linalg.norm(out - H @ v) <= 1e-13 * np.linalg.norm(H @ v)
+
+
+def test_hv_spin1_field_and_shift():
+    c = lattice.reference_cluster(8)
+    op, oop = _oop(c, 30.0, q=3, h=0.2)
+    J, K = dense.model20_couplings(30.0)
+    H = dense.kh_dense(c, J, K, K, K, q=3, h=0.2)
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(3 ** 8) + 1j * rng.standard_normal(3 ** 8)
+    out = np.zeros_like(v)
+    ep = 0.37
+    oracle.hv(oop, v, out, eperbd=ep)
+    # shift identity: eperbd is subtracted on every bond diagonal -> -ibond*eperbd
+    ref = H @ v - c.ibond * ep * v
+    assert np.linalg.norm(out - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_hermiticity_and_linearity_n16():
+    c = lattice.reference_cluster(16)
+    op, oop = _oop(c, 200.0, h=0.15)
+    rng = np.random.default_rng(4)
+    d = 2 ** 16
+    x = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    y = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    hx, hy = np.zeros(d, complex), np.zeros(d, complex)
+    oracle.hv(oop, x, hx)
+    oracle.hv(oop, y, hy)
+    a = np.vdot(y, hx)
+    b = np.conj(np.vdot(x, hy))
+    assert abs(a - b) <= 1e-12 * abs(a)
+    hz = np.zeros(d, complex)
+    oracle.hv(oop, 2.0 * x - 3j * y, hz)
+    assert np.linalg.norm(hz - (2.0 * hx - 3j * hy)) <= 1e-12 * np.linalg.norm(hz)
+
+
+def test_kitaev_flux_commutes_n16():
+    """Pure Kitaev (MODEL 20, phi=90 -> J=0): plaquette W_p = prod sigma^{gamma_out} commutes."""
+    c = lattice.reference_cluster(16)
+    op, oop = _oop(c, 90.0)
+    # find one hexagon: a 6-cycle in the bond graph
+    adj = {k: [] for k in range(c.n)}
+    for a, b, t in c.bonds:
+        adj[a - 1].append((b - 1, t))
+        adj[b - 1].append((a - 1, t))
+    hexagon = None
+
+    def dfs(path):
+        nonlocal hexagon
+        if hexagon:
+            return
+        if len(path) == 6:
+            if any(nb == path[0] for nb, _ in adj[path[-1]]):
+                hexagon = list(path)
+            return
+        for nb, _ in adj[path[-1]]:
+            if nb not in path:
+                dfs(path + [nb])
+    dfs([0])
+    assert hexagon
+    # outgoing bond type of each hexagon site = the one not used inside the loop
+    d = 2 ** c.n
+    idx = np.arange(d)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+
+    def W(x):
+        y = x.copy()
+        for k, s in enumerate(hexagon):
+            inner = {t for nb, t in adj[s] if nb in (hexagon[k - 1], hexagon[(k + 1) % 6])}
+            g = ({1, 2, 3} - inner).pop()
+            bit = (idx >> s) & 1
+            if g == 3:     # sigma^z: +1 on up (bit 0)
+                y = y * (1 - 2 * bit)
+            elif g == 1:   # sigma^x: flip
+                y = y[idx ^ (1 << s)]
+            else:          # sigma^y |b> : flip with i(1-2b') phase
+                y = y[idx ^ (1 << s)] * (1j * (1 - 2 * bit)) * (-1)
+        return y
+    hv1 = np.zeros(d, complex)
+    oracle.hv(oop, W(v), hv1)
+    hv2 = np.zeros(d, complex)
+    oracle.hv(oop, v, hv2)
+    assert np.linalg.norm(hv1 - W(hv2)) <= 1e-12 * np.linalg.norm(hv1)
+
+
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32_10
+    assert list(oracle.philox([0, 0, 0, 0], [0, 0])) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert list(oracle.philox([0xffffffff] * 4, [0xffffffff] * 2)) == \
+        [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert list(oracle.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
+                              [0xa4093822, 0x299f31d0])) == [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_gaussian_moments():
+    v = oracle.gaussian(1 << 20, 20261202, normalize=False)
+    assert np.all(np.isfinite(v))
+    for part in (v.real, v.imag):
+        assert abs(part.mean()) < 5e-3
+        assert abs(part.var() - 1.0) < 5e-3
+    assert abs(np.mean(v.real * v.imag)) < 5e-3
+    w = oracle.gaussian(1 << 12, 7)
+    assert abs(np.linalg.norm(w) - 1.0) < 1e-14
+    assert not np.allclose(w, oracle.gaussian(1 << 12, 8))
+
+
+def test_bisec_vec12_vs_scipy():
+    rng = np.random.default_rng(6)
+    m = 60
+    a = rng.standard_normal(m) * 3
+    b = rng.random(m) + 0.1
+    E = oracle.bisec(a, b, 8)
+    ev, evec = sla.eigh_tridiagonal(a, b[:-1])
+    assert np.allclose(E, ev[:8], atol=1e-12)
+    V = oracle.vec12(a, b, E, 3)
+    for k in range(3):
+        assert abs(abs(V[k] @ evec[:, k]) - 1.0) < 1e-10
+
+
+def test_lanczos_ground_energy_n12():
+    c = lattice.reference_cluster(12)
+    op, oop = _oop(c, 126.0, h=0.1)
+    J, K = dense.model20_couplings(126.0)
+    H = dense.kh_dense(c, J, K, K, K, h=0.1)
+    e_exact = np.linalg.eigvalsh(H)
+    v = oracle.gaussian(2 ** 12, 20261202)
+    itr, alpha, beta, ene = oracle.lanczos(oop, v, max_itr=1024, nvec=8)
+    assert itr > 0
+    assert abs(ene[0] - e_exact[0]) < 1e-10 * abs(e_exact[0])
+    # first steps: alpha/beta are the tridiagonal of the Krylov basis (numpy Gram-Schmidt)
+    q0 = v.copy()
+    w = H @ q0
+    a0 = np.vdot(q0, w).real
+    r = w - a0 * q0
+    assert abs(alpha[0] - a0) < 1e-12 * abs(a0)
+    assert abs(beta[0] - np.linalg.norm(r)) < 1e-12 * np.linalg.norm(r)
+    q1 = r / np.linalg.norm(r)
+    a1 = np.vdot(q1, H @ q1).real
+    assert abs(alpha[1] - a1) < 1e-12 * abs(a1)
+
+
+def test_inverse_iteration_n12():
+    c = lattice.reference_cluster(12)
+    op, oop = _oop(c, 54.0)
+    J, K = dense.model20_couplings(54.0)
+    H = dense.kh_dense(c, J, K, K, K)
+    e, vecs = np.linalg.eigh(H)
+    b = oracle.gaussian(2 ** 12, 11)
+    it, x = oracle.inverse_iteration(oop, e[0], b, max_inv=50)
+    assert it > 0
+    assert np.linalg.norm(H @ x - e[0] * x) < 1e-8
+    assert abs(abs(np.vdot(vecs[:, 0], x)) - 1.0) < 1e-10
+
+
+def test_tps_vs_dense_powers_n8():
+    c = lattice.reference_cluster(8)
+    op_tps = model.build_operator(c, 20, 90.0, tps=True)
+    oop = oracle.OracleOp(op_tps)
+    J, K = dense.model20_couplings(90.0)
+    H = dense.kh_dense(c, J, K, K, K)
+    L = op_tps.shift_L
+    A = L * np.eye(2 ** 8) - H
+    psi = oracle.gaussian(2 ** 8, 3)
+    steps = 12
+    sv = c.site_vec
+    sq = c.sq_momenta()
+    alpha, beta, coef = oracle.tps(oop, sv, sq, L, steps, psi)
+    v = psi.copy()
+    for k in range(steps):
+        e = np.vdot(v, H @ v).real
+        assert abs(alpha[k] - e) < 1e-11 * max(1.0, abs(e)), k
+        w = A @ v
+        assert abs(beta[k] - np.linalg.norm(w)) < 1e-11 * np.linalg.norm(w)
+        v = w / np.linalg.norm(w)
+    assert coef.shape == (steps, 7 + len(sq))
+    # coef(:,4) = l_max - alpha' = <H>, coef(:,2) = l_max
+    assert np.allclose(coef[:, 3], alpha, atol=1e-11)
+    assert np.allclose(coef[:, 1], L)
+
+
+def test_sz_q_and_cf_closed_forms():
+    c = lattice.reference_cluster(8)
+    x = oracle.gaussian(2 ** 8, 5)
+    qv = c.q_list[4]
+    v = oracle.sz_q(8, 2, c.site_vec, qv, x)
+    idx = np.arange(2 ** 8)
+    ref = np.zeros_like(x)
+    for j in range(8):
+        ph = np.exp(1j * (qv @ c.site_vec[j]))
+        ref += (((idx >> j) & 1) - 0.5) * ph * x
+    assert np.allclose(v, ref, atol=1e-15)
+    # continued fraction: the modified Lentz recursion must equal the backward
+    # evaluation of f = t + a1/(b1 + a2/(b2 + ...)), a_m = -sbeta_m^2, b_m = w+E-alpha_m+i eta
+    rng = np.random.default_rng(9)
+    depth = 6
+    salpha = rng.standard_normal(depth) + 1.0
+    sbeta = rng.random(depth + 1) + 0.2
+    eta, E = 0.05, -1.3
+    w, dsf, conv = oracle.renbun(salpha, sbeta, E, eta, 2000, 2.5e-3, depth + 1)
+    tiny = 1e-10 + 1e-10j
+    tail = np.zeros_like(w, dtype=complex)
+    for m in range(depth - 1, -1, -1):
+        b = w + E - salpha[m] + 1j * eta
+        tail = -sbeta[m] ** 2 / (b + tail)
+    ref = (tiny + tail).imag
+    # early exit at |C D - 1|^2 < 5e-29 truncates the tail only below that level
+    assert np.allclose(dsf, ref, rtol=1e-9, atol=1e-13)
+    nd = oracle.normalize_dsf(dsf, 2.5e-3, 0.3)
+    integral = np.sum(0.5 * 2.5e-3 * (nd[1:] + nd[:-1]))
+    assert abs(integral - 0.3) < 1e-12
