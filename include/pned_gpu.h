@@ -1,0 +1,211 @@
+/*
+ * pned_gpu.h -- C ABI of the MI355X H.v + Krylov library (libpned_gpu.so).
+ *
+ * One context = one operator (cluster + bond matrices) on one GPU, with its own HIP
+ * stream.  Vectors live in device memory and are referred to by opaque handles; the
+ * caller only sees host scalars.  Every call returns 0 on success or a negative
+ * PNED_E_* code; pned_last_error() gives the message (the reference prints a
+ * message and stops instead, e.g. #(E07) at S_one_half/Hybrid_TPS_Lanczos.f90:126-130;
+ * the thin Fortran layer restores that behaviour).
+ *
+ * Array conventions (identical to the reference's data_mem tables, S_one_half/
+ * Hybrid_TPS_params.f90:313-329):
+ *   ipair       int32[2*ibond], 1-based site numbers, reference bond order
+ *   ibond_type  int32[ibond], 1..9
+ *   xlocal_ham  double[2*qq*qq*9]  complex (re,im) in Fortran order of
+ *               xlocal_Ham(0:qq-1, 0:qq-1, 9), qq = q_spin^2
+ *   llocal_ham  int32[qq*qq*9]      Fortran order of llocal_Ham (nonzero = skipped)
+ * The H.v applies v0(out) += xlocal_Ham(in,out,type) * v1(in) over every bond, the
+ * formula of H_dot_v (S_one_half/Hybrid_TPS_parallel.f90:415-440), with the bond's
+ * local index q*digit(max site) + digit(min site).
+ * Complex vectors are (re,im) pairs: complex128 = 2 doubles, complex64 = 2 floats.
+ * A vector with nvec > 1 holds nvec columns interleaved state-major: [dim][nvec].
+ */
+#ifndef PNED_GPU_H
+#define PNED_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pned_ctx pned_ctx;
+typedef struct pned_vec pned_vec;
+
+enum {
+    PNED_OK = 0,
+    PNED_E_ARG = -1,        /* bad argument */
+    PNED_E_NOMEM = -2,      /* device allocation failed */
+    PNED_E_HIP = -3,        /* HIP runtime error */
+    PNED_E_UNSUPPORTED = -4,/* lattice/model outside what the kernels handle */
+    PNED_E_BREAKDOWN = -5,  /* Lanczos beta below 0.5e-30 (reference #(E07)) */
+    PNED_E_NOCONV = -6      /* iteration limit reached without convergence */
+};
+
+enum { PNED_C128 = 0, PNED_C64 = 1 };
+
+const char *pned_last_error(void);
+int pned_version(void);
+/* number of visible HIP devices (0 when no GPU is usable) */
+int pned_device_count(void);
+
+/* ---- context: replaces the data_mem tables + set_parallel_bits + make_cpu_list2 ----
+ * set_xlocal_Ham / set_bond_info stay with the caller; their outputs come in here.
+ * (S_one_half/Hybrid_TPS_main.f90:57-65; S_arbitrary/Hybrid_TPS9_main.f90)            */
+int pned_ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair,
+                    const int32_t *ibond_type, const double *xlocal_ham,
+                    const int32_t *llocal_ham, int device, pned_ctx **out);
+int pned_ctx_destroy(pned_ctx *ctx);
+/* basis dimension q^N (ldim at np=1, S_one_half/Hybrid_TPS_MPIlib.f90:74-84) */
+int64_t pned_ctx_dim(const pned_ctx *ctx);
+/* replace the bond matrices in place (same sparsity structure not required),
+ * e.g. change_to_TPS_hamiltonian (S_one_half/Hybrid_TPS_params.f90:1600-1623) */
+int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t *llocal_ham);
+/* tile/pass tuning knobs of the S=1/2 H.v (0 = autotuned default) */
+int pned_ctx_tune(pned_ctx *ctx, int tile_bits, int low_bits, int threads);
+/* autotuner (no reference counterpart): times one fresh H.v v1 -> v0 for every
+ * (tile bits, forced low bits, threads per block) candidate that fits LDS and keeps the
+ * fastest for this context; v0 is overwritten.  Outputs may be NULL. */
+int pned_ctx_autotune(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, int reps, int *best_tile_bits,
+                      int *best_low_bits, int *best_threads, double *best_ms);
+/* diagnostic: number of tile passes one H.v makes, and terms per pass (max 64) */
+int pned_ctx_plan_info(const pned_ctx *ctx, int *npasses, int *terms_per_pass, int max);
+/* diagnostic, needs no GPU: the S=1/2 term decomposition and tile passes the planner
+ * builds for these tables.  Outputs (caller-sized):
+ *   pass_info[npass*4]   = {K, first_term, nterm, has_diag}
+ *   pass_bits[npass*40]  = tile bit positions (memory index), K used
+ *   terms[nterm*6]       = {tmask, selmask, c0r, c0i, c1r, c1i} (as doubles)
+ *   term_gmask[nterm]    = flipped state bits
+ *   diag[ngroup*5 + 2]   = {kind, shift, mask, wr, wi}..., c0r, c0i
+ * Returns the number of passes in *npass, terms in *nterm, groups in *ngroup. */
+int pned_s12_plan_export(int n_sites, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                         const double *xlocal_ham, const int32_t *llocal_ham, int colbits,
+                         int tile_bits, int low_bits, int *npass, int *nterm, int *ngroup,
+                         int32_t *pass_info, int32_t *pass_bits, double *terms,
+                         uint32_t *term_gmask, double *diag, int max_pass, int max_term);
+/* synchronize the context stream */
+int pned_ctx_sync(pned_ctx *ctx);
+/* the hipStream_t the context launches on (for event timing by the caller) */
+void *pned_ctx_stream(pned_ctx *ctx);
+
+/* ---- vectors: replace the working_area arrays wwk/rr1/xvec (Hybrid_TPS_params.f90:272-311) */
+int pned_vec_create(pned_ctx *ctx, int nvec, int dtype, pned_vec **out);
+int pned_vec_destroy(pned_vec *v);
+int pned_vec_info(const pned_vec *v, int64_t *dim, int *nvec, int *dtype);
+/* copy count states (all columns) starting at state 'offset' to/from host memory */
+int pned_vec_upload(pned_ctx *ctx, pned_vec *v, const void *host, int64_t offset, int64_t count);
+int pned_vec_download(pned_ctx *ctx, const pned_vec *v, void *host, int64_t offset, int64_t count);
+int pned_vec_zero(pned_ctx *ctx, pned_vec *v);
+int pned_vec_copy(pned_ctx *ctx, const pned_vec *src, pned_vec *dst);
+/* device pointer of the data (for zero-copy use by the caller's own kernels) */
+void *pned_vec_data(pned_vec *v);
+
+/* ---- parallelized_diag (S_one_half/Hybrid_TPS_parallel.f90:6-8) ---------------------- */
+
+/* parallelized_set_initial_vec_p (Hybrid_TPS_parallel.f90:17-102), replaced: i.i.d. N(0,1)
+ * real/imag parts from Philox-4x32-10 (key = seed, counter = (index, column)), each column
+ * normalised in FP64. */
+int pned_set_initial_vec(pned_ctx *ctx, pned_vec *v, uint64_t seed);
+
+/* p_mltply_v3 (Hybrid_TPS_parallel.f90:190-263): v0 += (H - ibond*eperbd) v1 and
+ * prdct[c] = <v1_c|(H - ibond*eperbd)|v1_c> per column (2 doubles each).
+ * eperbd is one value for all columns (or an nvec array, see pned_hv_multi). */
+int pned_p_mltply_v3(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, double *prdct);
+/* same, with v0 treated as zero on input (fresh output: 2 instead of 3 vector streams) */
+int pned_hv_fresh(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, double *prdct);
+
+/* calc_vec_inprdct (Hybrid_TPS_parallel.f90:104-126): out[c] = sum conj(x_c) y_c, FP64 */
+int pned_calc_vec_inprdct(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out);
+/* calc_beta (Hybrid_TPS_parallel.f90:161-188): beta2[c] = ||v0_c - alpha_c v1_c||^2 */
+int pned_calc_beta(pned_ctx *ctx, const double *alpha, const pned_vec *v1, const pned_vec *v0,
+                   double *beta2);
+/* calc_alpha_beta (Hybrid_TPS_parallel.f90:145-159): p_mltply_v3 then calc_beta */
+int pned_calc_alpha_beta(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double *alpha, double *beta);
+/* calc_alpha_beta2 (Hybrid_TPS_parallel.f90:128-142): p_mltply_v3 then ||v0|| */
+int pned_calc_alpha_beta2(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double *alpha, double *beta);
+
+/* BLAS-1 helpers used by the Krylov loops (per column scalars, complex = 2 doubles):
+ * y = a*x + b*y */
+int pned_axpby(pned_ctx *ctx, const double *a, const pned_vec *x, const double *b, pned_vec *y);
+int pned_scale(pned_ctx *ctx, pned_vec *x, const double *a);
+
+/* ---- Krylov drivers ---------------------------------------------------------------------- */
+
+/* Lanczos three-term update of lnc1z / dsflnc1z (Hybrid_TPS_Lanczos.f90:111-116):
+ *   t = v1; v1 = (v0 - alpha v1)/beta; v0 = -beta t      (per column)                    */
+int pned_lanczos_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double *alpha,
+                        const double *beta);
+
+/* parallelized_lnc1 + parallelized_lnc1z (Hybrid_TPS_Lanczos.f90:7-163) on column 0.
+ * v1 holds the start vector (restart semantics: the caller fills it, e.g. with
+ * pned_set_initial_vec), v0 is work.  alpha/beta need max_itr entries, ene nvec.
+ * fixed_steps > 0 runs exactly that many steps (no convergence exit).  *itr gets the
+ * step count.  Convergence: 8th (nvec-th) Ritz value changes by < 1e-13 relative,
+ * checked every 5 steps after 4*nvec (Hybrid_TPS_Lanczos.f90:133-150).                    */
+int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, int fixed_steps,
+              double *alpha, double *beta, double *ene, int *itr);
+
+/* bisec (Hybrid_TPS_basic.f90:25-107) and vec12 (112-213); host FP64.  coef is ndim x nvec
+ * column-major. */
+int pned_bisec(const double *alpha, const double *beta, int ndim, double *ene, int ne, double eps);
+int pned_vec12(const double *alpha, const double *beta, int ndim, const double *ene, int nvec,
+               double *coef);
+
+/* Ritz vector: x = sum_k coef[k] v_k regenerated by re-running the Lanczos recursion
+ * from the start vector in v1 (3 resident vectors: v1, v0, x).  Uses the alpha/beta of the
+ * first run.  */
+int pned_ritz_vector(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *x, int nsteps,
+                     const double *alpha, const double *beta, const double *coef);
+
+/* parallelized_inv1z + parallelized_cg1 (Hybrid_TPS_cg.f90:18-197): inverse iteration for
+ * the eigenvector of eigenvalue eig.  b holds the start vector; x the result (normalised).
+ * work must hold 3 vectors' handles (p, r, y).  reference_saved_loop = 1 reproduces the
+ * SAVEd 'loop' flag of cg1 (Hybrid_TPS_cg.f90:118).  *iters = inverse iterations taken. */
+int pned_inv1(pned_ctx *ctx, double eig, pned_vec *x, pned_vec *b, pned_vec *p, pned_vec *r,
+              pned_vec *y, int max_inv, int max_cg, int reference_saved_loop, int *iters);
+
+/* COCG for the complex-shifted system (H - e0 + i*eta) x = b (no reference counterpart:
+ * the reference's CG shift is real, Hybrid_TPS_cg.f90:135).  Stops at ||r|| < tol ||b||. */
+int pned_cocg(pned_ctx *ctx, double e0, double eta, const pned_vec *b, pned_vec *x, pned_vec *r,
+              pned_vec *p, pned_vec *y, int max_it, double tol, int *iters, double *resid);
+
+/* check1 (Hybrid_TPS_check.f90:8-54): expect = <x|H|x>, ratios (Hx)_j/x_j at
+ * j = min(ldim/3,13) : ldim : max(1,ldim/20) (1-based), up to max_ratios; *nratios set. */
+int pned_check1(pned_ctx *ctx, const pned_vec *x, pned_vec *work, double *expect, double *ratios,
+                int max_ratios, int *nratios);
+
+/* residual ||(H - e) x|| / ||x|| using 'work' as scratch vector */
+int pned_residual(pned_ctx *ctx, const pned_vec *x, double e, pned_vec *work, double *res);
+
+/* ---- mTPQ (Hybrid_TPS_TPS.f90:45-409) -------------------------------------------------- */
+/* Observables of eval_phys_val: site positions (n x 2), the static-structure-factor
+ * momenta (nsq x 2).  The operator of the context must already be the shifted one
+ * (l - h_b, change_to_TPS_hamiltonian) and lmax = ibond*l_b. */
+int pned_tps_set_observables(pned_ctx *ctx, const double *site_vec, int nsq, const double *sq_q);
+/* Runs nsteps steps of parallelized_TPSz on every column of v1 (the normalised start
+ * vectors); v0, vg, vt are work vectors with the same shape.  alpha/beta are
+ * [nsteps][nvec], coef is [nsteps][nvec][7+nsq] (row of eval_phys_val per column). */
+int pned_tps(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, pned_vec *vt, double lmax,
+             int nsteps, double *alpha, double *beta, double *coef);
+
+/* ---- DSF (Hybrid_TPS_DSF.f90) ------------------------------------------------------------ */
+/* intalv_zz (Hybrid_TPS_DSF.f90:18-87): v = normalised sum_j w(d_j) e^{i q.r_j} x with
+ * w = (d - 1/2) for S=1/2 (bit value d) or (d - S) for spin S; cnorm = ||.||^2 / N. */
+int pned_intalv_zz(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double q[2],
+                   const double *site_vec, double *cnorm);
+/* parallelized_dsflnc1z (Hybrid_TPS_DSF.f90:195-261): s_maxitr-1 Lanczos steps from the
+ * normalised excitation in v1; salpha[s_maxitr], sbeta[s_maxitr+1] (sbeta[0] = cnorm). */
+int pned_dsflnc1z(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double cnorm, int s_maxitr,
+                  double *salpha, double *sbeta);
+/* exec_renbun (Hybrid_TPS_DSF.f90:263-334) on the GPU, one thread per omega, followed by
+ * nomalization_DSF (337-351) when sq > 0.  w/dsf/conv have i_range entries.              */
+int pned_renbun(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,
+                const double *salpha, const double *sbeta, double sq, double *w, double *dsf,
+                int *conv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PNED_GPU_H */

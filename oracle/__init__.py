@@ -214,3 +214,44 @@ def normalize_dsf(dsf, split, sq):
     d = np.ascontiguousarray(dsf, dtype=np.float64).copy()
     lib().orc_normalize_dsf(len(d), split, float(sq), d)
     return d
+
+
+def cpu_rows_sample(op, rows: int, seed: int = 20261202):
+    """Time the oracle's reference-structured H.v on output rows [0, rows) of ``op``.
+
+    Used for the CPU baseline at sizes whose full vectors do not fit in host memory
+    (N = 32: 64 GiB per complex128 vector).  v1 and v0 are anonymous NORESERVE
+    mappings of the full dimension; only the blocks the row slice touches are filled
+    (the partner blocks i ^ mask of every flip term), so the memory actually used is a
+    few GB.  Returns (seconds, rows, threads).
+    """
+    import mmap
+    import time
+
+    oop = OracleOp(op)
+    d = oop.dim
+    nbytes = d * 16
+    flags = mmap.MAP_PRIVATE | mmap.MAP_ANONYMOUS | getattr(mmap, "MAP_NORESERVE", 0x4000)
+    m1 = mmap.mmap(-1, nbytes, flags=flags, prot=mmap.PROT_READ | mmap.PROT_WRITE)
+    m0 = mmap.mmap(-1, nbytes, flags=flags, prot=mmap.PROT_READ | mmap.PROT_WRITE)
+    v1 = np.frombuffer(m1, dtype=np.complex128)
+    v0 = np.frombuffer(m0, dtype=np.complex128)
+    # blocks touched by rows [0, rows): every partner index i ^ m stays in the block
+    # (m & ~(B-1)) of size B = next power of two >= rows
+    B = 1 << max(0, (rows - 1).bit_length())
+    masks = set()
+    pairs = oop.site_pair.reshape(-1, 2)
+    for a, b in pairs:
+        for m in (1 << int(a), 1 << int(b), (1 << int(a)) | (1 << int(b)), 0):
+            masks.add(m & ~(B - 1))
+    for base in sorted(masks):
+        lib().orc_fill_range(seed, base, base + B, v1.ctypes.data)
+    pr = np.zeros(1, dtype=np.complex128)
+    t = time.perf_counter()
+    lib().orc_hv_rows(*oop.args(), 0.0, v1, v0, pr, 0, rows)
+    dt = time.perf_counter() - t
+    threads = lib().orc_num_threads()
+    del v1, v0
+    m1.close()
+    m0.close()
+    return dt, rows, threads

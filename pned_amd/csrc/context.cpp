@@ -1,0 +1,731 @@
+// context.cpp -- contexts, vectors, the S=1/2 term planner and the hot-path ABI entries.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <set>
+#include <tuple>
+
+#include "internal.h"
+
+namespace pned {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+void reset_spin_tables(pned_ctx *ctx);
+
+// ------------------------------------------------------------------------------------
+// S=1/2 decomposition.  For every bond (sites a < b, local index t = 2*s_b + s_a) the
+// reference applies v0(t) += h(in, t) v1(in).  Each flip pattern f = in ^ t (1: flip a,
+// 2: flip b, 3: both) gives a coefficient function g_f(t) of the OUTPUT local state,
+// which is expanded in the Walsh basis {1, sigma_a, sigma_b, sigma_a sigma_b}
+// (sigma = 1 - 2 s).  Every component becomes a term whose selector is the parity of the
+// output-state bits it depends on.  The diagonal (f = 0) is expanded the same way into a
+// constant, per-site and per-bond-parity popcount groups.
+// ------------------------------------------------------------------------------------
+namespace {
+struct Cx {
+    double r = 0, i = 0;
+};
+inline Cx operator+(Cx a, Cx b) { return {a.r + b.r, a.i + b.i}; }
+inline Cx operator-(Cx a, Cx b) { return {a.r - b.r, a.i - b.i}; }
+inline Cx operator*(double s, Cx a) { return {s * a.r, s * a.i}; }
+inline bool nz(Cx a) { return a.r * a.r + a.i * a.i > 0.5e-28; }
+
+void walsh(const Cx g[4], Cx &e0, Cx &e1, Cx &e2, Cx &e12)
+{
+    // g index t = 2*s2 + s1 ; sigma = +1 for s = 0
+    e0 = 0.25 * (g[0] + g[1] + g[2] + g[3]);
+    e1 = 0.25 * (g[0] - g[1] + g[2] - g[3]);
+    e2 = 0.25 * (g[0] + g[1] - g[2] - g[3]);
+    e12 = 0.25 * (g[0] - g[1] - g[2] + g[3]);
+}
+}  // namespace
+
+int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype, const double *xl,
+                  const int32_t *ll, S12Plan &plan)
+{
+    plan = S12Plan();
+    plan.n = n;
+    std::map<std::pair<uint32_t, uint32_t>, Term> terms;
+    std::vector<Cx> site_w(n);
+    std::map<std::pair<int, int>, Cx> pair_w;   // (a, b) -> weight of parity(a,b)
+    Cx c0;
+    auto X = [&](int t, int in, int out) -> Cx {
+        const size_t f = (size_t)in + 4 * (size_t)out + 16 * (size_t)t;
+        if (ll[f]) return Cx();
+        return Cx{xl[2 * f], xl[2 * f + 1]};
+    };
+    auto add_term = [&](uint32_t gmask, uint32_t selmask, Cx cs0, Cx cs1) {
+        auto key = std::make_pair(gmask, selmask);
+        Term &T = terms[key];
+        T.gmask = gmask;
+        T.selmask = selmask;
+        T.c0r += cs0.r;
+        T.c0i += cs0.i;
+        T.c1r += cs1.r;
+        T.c1i += cs1.i;
+    };
+    for (int b = 0; b < ibond; ++b) {
+        int s_a = ipair[2 * b] - 1, s_b = ipair[2 * b + 1] - 1;
+        if (s_a < 0 || s_b < 0 || s_a >= n || s_b >= n || s_a == s_b)
+            return fail(PNED_E_ARG, "#(E03) Incorrect data in ipair at bond " + std::to_string(b + 1));
+        if (s_a > s_b) std::swap(s_a, s_b);
+        const int t = btype[b] - 1;
+        if (t < 0 || t > 8) return fail(PNED_E_ARG, "bond type outside 1..9");
+        // diagonal
+        for (int l = 0; l < 4; ++l)
+            if (ll[(size_t)l + 4 * l + 16 * t])
+                return fail(PNED_E_UNSUPPORTED, "a masked bond diagonal (llocal_Ham(k,k)) is not supported");
+        Cx g[4];
+        for (int l = 0; l < 4; ++l) g[l] = X(t, l, l);
+        Cx e0, e1, e2, e12;
+        walsh(g, e0, e1, e2, e12);
+        // d = e0 + e1 sa + e2 sb + e12 sa sb, sigma = 1 - 2s, sigma_a sigma_b = 1 - 2 par
+        c0 = c0 + e0 + e1 + e2 + e12;
+        site_w[s_a] = site_w[s_a] - 2.0 * e1;
+        site_w[s_b] = site_w[s_b] - 2.0 * e2;
+        pair_w[{s_a, s_b}] = pair_w[{s_a, s_b}] - 2.0 * e12;
+        // flips
+        const uint32_t ba = 1u << s_a, bb = 1u << s_b;
+        for (int f = 1; f <= 3; ++f) {
+            for (int l = 0; l < 4; ++l) g[l] = X(t, l ^ f, l);
+            walsh(g, e0, e1, e2, e12);
+            const uint32_t gm = (f == 1) ? ba : (f == 2) ? bb : (ba | bb);
+            Cx base = e0;
+            bool used = false;
+            if (nz(e12)) { add_term(gm, ba | bb, base + e12, base - e12); base = Cx(); used = true; }
+            if (nz(e1)) { add_term(gm, ba, base + e1, base - e1); base = Cx(); used = true; }
+            if (nz(e2)) { add_term(gm, bb, base + e2, base - e2); base = Cx(); used = true; }
+            if (!used && nz(base)) add_term(gm, 0u, base, base);
+        }
+    }
+    for (auto &kv : terms) {
+        const Term &T = kv.second;
+        if (nz(Cx{T.c0r, T.c0i}) || nz(Cx{T.c1r, T.c1i})) plan.terms.push_back(T);
+    }
+    // diagonal groups: sites with equal weight; bond parities grouped by (distance, weight)
+    plan.c0r = c0.r;
+    plan.c0i = c0.i;
+    std::map<std::pair<double, double>, uint32_t> sg;
+    for (int k = 0; k < n; ++k)
+        if (nz(site_w[k])) sg[{site_w[k].r, site_w[k].i}] |= 1u << k;
+    for (auto &kv : sg) plan.diag.push_back(DiagGroup{0, 0, kv.second, kv.first.first, kv.first.second});
+    std::map<std::tuple<int, double, double>, uint32_t> pg;
+    for (auto &kv : pair_w)
+        if (nz(kv.second))
+            pg[std::make_tuple(kv.first.second - kv.first.first, kv.second.r, kv.second.i)] |= 1u << kv.first.first;
+    for (auto &kv : pg)
+        plan.diag.push_back(DiagGroup{1, std::get<0>(kv.first), kv.second, std::get<1>(kv.first), std::get<2>(kv.first)});
+    if (plan.diag.size() > 48) return fail(PNED_E_UNSUPPORTED, "diagonal has more than 48 coefficient groups");
+    return PNED_OK;
+}
+
+// Tile-pass cover.  Memory bits: [colbits column bits | n state bits].  Every term must
+// lie inside one pass's tile bits; each pass costs one full stream of v1 and v0, so the
+// planner minimises the number of passes.  Randomised greedy with restarts (fixed seed,
+// so the plan is reproducible): a pass starts from the forced low bits plus the bits of one
+// uncovered term and grows by the bit that completes the most uncovered terms (ties:
+// terms touching it, then a random key).
+namespace {
+struct Lcg {
+    uint64_t s;
+    uint32_t next() { s = s * 6364136223846793005ull + 1442695040888963407ull; return (uint32_t)(s >> 33); }
+};
+}  // namespace
+
+static std::vector<uint64_t> cover_once(const std::vector<uint64_t> &mm, int colbits, int nb, int K, int L,
+                                        Lcg &rng, bool randomised)
+{
+    std::vector<char> done(mm.size(), 0);
+    size_t remaining = mm.size();
+    std::vector<uint64_t> tiles;
+    const uint64_t low = (L >= 64) ? ~0ull : ((1ull << L) - 1);
+    bool first = true;
+    while (first || remaining > 0) {
+        uint64_t T = low;
+        int cnt = L;
+        if (remaining > 0 && randomised) {
+            // seed with one uncovered term
+            size_t pick = rng.next() % remaining, k = 0;
+            for (size_t i = 0; i < mm.size(); ++i)
+                if (!done[i] && k++ == pick) {
+                    const uint64_t add = mm[i] & ~T;
+                    if (__builtin_popcountll(add) + cnt <= K) {
+                        T |= add;
+                        cnt += __builtin_popcountll(add);
+                    }
+                    break;
+                }
+        }
+        while (cnt < K) {
+            int best = -1;
+            uint64_t best_score = 0;
+            for (int bit = colbits; bit < nb; ++bit) {
+                if (T >> bit & 1) continue;
+                const uint64_t T2 = T | (1ull << bit);
+                uint64_t newly = 0, touch = 0;
+                for (size_t i = 0; i < mm.size(); ++i) {
+                    if (done[i] || !(mm[i] >> bit & 1)) continue;
+                    ++touch;
+                    if ((mm[i] & ~T2) == 0) ++newly;
+                }
+                const uint64_t key = randomised ? (rng.next() & 0xFFFFF) : (uint64_t)(nb - bit);
+                const uint64_t score = (newly << 44) | (touch << 24) | key;
+                if (best < 0 || score > best_score) { best_score = score; best = bit; }
+            }
+            if (best < 0) break;
+            T |= 1ull << best;
+            ++cnt;
+        }
+        size_t got = 0;
+        for (size_t i = 0; i < mm.size(); ++i)
+            if (!done[i] && (mm[i] & ~T) == 0) { done[i] = 1; --remaining; ++got; }
+        if (got == 0 && !first) return {};   // stuck
+        tiles.push_back(T);
+        first = false;
+        if (tiles.size() > 64) return {};
+    }
+    return tiles;
+}
+
+int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
+{
+    plan.passes.clear();
+    plan.devterms.clear();
+    plan.colbits = colbits;
+    const int nb = colbits + plan.n;
+    const int K = std::min(tile_bits, nb);
+    const int L = std::min(std::max(low_bits, colbits), K);
+    std::vector<uint64_t> mmask(plan.terms.size());
+    for (size_t i = 0; i < plan.terms.size(); ++i) mmask[i] = (uint64_t)plan.terms[i].gmask << colbits;
+    Lcg rng{0x9E3779B97F4A7C15ull};
+    std::vector<uint64_t> best = cover_once(mmask, colbits, nb, K, L, rng, false);
+    const int restarts = (nb > K) ? 400 : 0;
+    for (int r = 0; r < restarts; ++r) {
+        std::vector<uint64_t> t = cover_once(mmask, colbits, nb, K, L, rng, true);
+        if (!t.empty() && (best.empty() || t.size() < best.size())) best = t;
+    }
+    if (best.empty()) return fail(PNED_E_UNSUPPORTED, "tile planner made no progress");
+    std::vector<char> done(plan.terms.size(), 0);
+    for (size_t p = 0; p < best.size(); ++p) {
+        const uint64_t T = best[p];
+        PassPlan pp;
+        pp.K = __builtin_popcountll(T);
+        for (int bit = 0; bit < nb; ++bit) {
+            if (T >> bit & 1) pp.tb.push_back(bit);
+            else pp.ob.push_back(bit);
+        }
+        pp.diag = (p == 0);
+        pp.first_term = (int)plan.devterms.size();
+        for (size_t i = 0; i < plan.terms.size(); ++i) {
+            if (done[i] || (mmask[i] & ~T) != 0) continue;
+            done[i] = 1;
+            const Term &t = plan.terms[i];
+            DevTerm d{};
+            uint32_t tm = 0;
+            for (int k = 0; k < pp.K; ++k)
+                if (mmask[i] >> pp.tb[k] & 1) tm |= 1u << k;
+            d.tmask = tm;
+            d.selmask = t.selmask;
+            d.is_real = (t.c0i == 0.0 && t.c1i == 0.0) ? 1u : 0u;
+            d.c0r = t.c0r;
+            d.c0i = t.c0i;
+            d.c1r = t.c1r;
+            d.c1i = t.c1i;
+            plan.devterms.push_back(d);
+        }
+        pp.nterm = (int)plan.devterms.size() - pp.first_term;
+        plan.passes.push_back(pp);
+    }
+    return PNED_OK;
+}
+
+int ensure_partial(pned_ctx *ctx, size_t nd)
+{
+    if (nd <= ctx->partial_cap) return PNED_OK;
+    if (ctx->d_partial) PNED_HIP(hipFree(ctx->d_partial));
+    ctx->d_partial = nullptr;
+    size_t cap = std::max(nd, (size_t)1 << 16);
+    PNED_HIP(hipMalloc((void **)&ctx->d_partial, sizeof(double) * cap));
+    ctx->partial_cap = cap;
+    return PNED_OK;
+}
+
+int ensure_result(pned_ctx *ctx, size_t nd)
+{
+    if (nd <= ctx->result_cap) return PNED_OK;
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_result) PNED_HIP(hipFree(ctx->d_result));
+    if (ctx->h_result) PNED_HIP(hipHostFree(ctx->h_result));
+    ctx->d_result = nullptr;
+    ctx->h_result = nullptr;
+    size_t cap = std::max(nd, (size_t)8192);
+    PNED_HIP(hipMalloc((void **)&ctx->d_result, sizeof(double) * cap));
+    PNED_HIP(hipHostMalloc((void **)&ctx->h_result, sizeof(double) * cap, hipHostMallocDefault));
+    ctx->result_cap = cap;
+    return PNED_OK;
+}
+
+// small argument arrays for kernels: uploaded in stream order into a dedicated buffer
+// (each kernel that reads them is enqueued before the next upload overwrites them)
+int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr)
+{
+    if ((size_t)n > ctx->scalars_cap) {
+        PNED_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->d_scalars) PNED_HIP(hipFree(ctx->d_scalars));
+        ctx->d_scalars = nullptr;
+        const size_t cap = std::max<size_t>((size_t)n, 8192);
+        PNED_HIP(hipMalloc((void **)&ctx->d_scalars, sizeof(double) * cap));
+        ctx->scalars_cap = cap;
+    }
+    PNED_HIP(hipMemcpyAsync(ctx->d_scalars, h, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    *dptr = ctx->d_scalars;
+    return PNED_OK;
+}
+
+int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
+{
+    if (ctx->plan_colbits == colbits && ctx->plan_dtype == dtype) return PNED_OK;
+    // the whole tile sits in LDS (160 KB on gfx950): 2^13 complex128 or 2^14 complex64
+    const int kmax = (dtype == PNED_C128) ? 13 : 14;
+    int K = ctx->tune_tile_bits;
+    if (K <= 0) K = kmax;
+    if (K > kmax) K = kmax;
+    int L = ctx->tune_low_bits > 0 ? ctx->tune_low_bits : 3;
+    PNED_TRY(s12_make_passes(ctx->s12, colbits, K, L));
+    S12Plan &pl = ctx->s12;
+    const size_t nt = std::max<size_t>(pl.devterms.size(), 1);
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_terms) PNED_HIP(hipFree(ctx->d_terms));
+    ctx->d_terms = nullptr;
+    PNED_HIP(hipMalloc((void **)&ctx->d_terms, sizeof(DevTerm) * nt));
+    ctx->d_terms_cap = nt;
+    if (!pl.devterms.empty())
+        PNED_HIP(hipMemcpy(ctx->d_terms, pl.devterms.data(), sizeof(DevTerm) * pl.devterms.size(),
+                           hipMemcpyHostToDevice));
+    DiagDev &dh = ctx->diag_host;
+    std::memset(&dh, 0, sizeof(dh));
+    dh.ngroups = (int)pl.diag.size();
+    for (int g = 0; g < dh.ngroups; ++g) {
+        dh.kind[g] = pl.diag[g].kind;
+        dh.shift[g] = pl.diag[g].shift;
+        dh.mask[g] = pl.diag[g].mask;
+        dh.wr[g] = pl.diag[g].wr;
+        dh.wi[g] = pl.diag[g].wi;
+    }
+    dh.c0r = pl.c0r;
+    dh.c0i = pl.c0i;
+    if (!ctx->d_diag) PNED_HIP(hipMalloc((void **)&ctx->d_diag, sizeof(DiagDev)));
+    PNED_HIP(hipMemcpy(ctx->d_diag, &dh, sizeof(DiagDev), hipMemcpyHostToDevice));
+    ctx->plan_colbits = colbits;
+    ctx->plan_dtype = dtype;
+    return PNED_OK;
+}
+
+static int hv_dispatch(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh, double *prdct)
+{
+    if (!ctx || !v1 || !v0) return fail(PNED_E_ARG, "null argument");
+    if (v1->ctx != ctx || v0->ctx != ctx) return fail(PNED_E_ARG, "vector belongs to another context");
+    if (v1->dim != v0->dim || v1->nvec != v0->nvec || v1->dtype != v0->dtype)
+        return fail(PNED_E_ARG, "v1 and v0 differ in shape or type");
+    if (v1 == v0) return fail(PNED_E_ARG, "v1 and v0 must be different vectors");
+    if (ctx->q == 2) return launch_hv_s12(ctx, v1, v0, eperbd, fresh, prdct);
+    return launch_hv_spin(ctx, v1, v0, eperbd, fresh, prdct);
+}
+
+}  // namespace pned
+
+using namespace pned;
+
+extern "C" {
+
+const char *pned_last_error(void) { return g_err.c_str(); }
+int pned_version(void) { return 1; }
+
+int pned_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int pned_ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                    const double *xlocal_ham, const int32_t *llocal_ham, int device, pned_ctx **out)
+{
+    if (!out || !ipair || !ibond_type || !xlocal_ham || !llocal_ham) return fail(PNED_E_ARG, "null argument");
+    *out = nullptr;
+    if (n_sites < 2 || q_spin < 2 || ibond < 1) return fail(PNED_E_ARG, "bad sizes");
+    if (q_spin == 2 && n_sites > 32) return fail(PNED_E_UNSUPPORTED, "S=1/2: at most 32 sites on one GPU");
+    double dimd = std::pow((double)q_spin, (double)n_sites);
+    if (dimd > 9.3e18) return fail(PNED_E_UNSUPPORTED, "dimension overflows 64 bits");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(PNED_E_HIP, "no HIP device available (libpned_gpu needs an MI355X)");
+    if (device < 0 || device >= ndev) return fail(PNED_E_ARG, "device index out of range");
+    PNED_HIP(hipSetDevice(device));
+    // datack (Hybrid_TPS_basic.f90:8-21)
+    for (int k = 0; k < 2 * ibond; ++k)
+        if (ipair[k] <= 0 || ipair[k] > n_sites)
+            return fail(PNED_E_ARG, "#(E03) Incorrect data in ipair at " + std::to_string(k + 1));
+    auto *ctx = new pned_ctx();
+    ctx->device = device;
+    ctx->n = n_sites;
+    ctx->q = q_spin;
+    ctx->ibond = ibond;
+    int64_t d = 1;
+    for (int k = 0; k < n_sites; ++k) d *= q_spin;
+    ctx->dim = d;
+    ctx->ipair.assign(ipair, ipair + 2 * ibond);
+    ctx->btype.assign(ibond_type, ibond_type + ibond);
+    const size_t qq = (size_t)q_spin * q_spin;
+    ctx->xl.assign(xlocal_ham, xlocal_ham + 2 * qq * qq * 9);
+    ctx->ll.assign(llocal_ham, llocal_ham + qq * qq * 9);
+    ctx->site_pair.resize(2 * ibond);
+    for (int b = 0; b < ibond; ++b) {
+        const int a = ipair[2 * b] - 1, c = ipair[2 * b + 1] - 1;
+        ctx->site_pair[2 * b] = std::min(a, c);
+        ctx->site_pair[2 * b + 1] = std::max(a, c);
+    }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return fail(PNED_E_HIP, "hipStreamCreate failed");
+    }
+    if (q_spin == 2) {
+        int rc = s12_decompose(n_sites, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, ctx->s12);
+        if (rc != PNED_OK) {
+            pned_ctx_destroy(ctx);
+            return rc;
+        }
+    }
+    *out = ctx;
+    return PNED_OK;
+}
+
+int pned_ctx_destroy(pned_ctx *ctx)
+{
+    if (!ctx) return PNED_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_terms) (void)hipFree(ctx->d_terms);
+    if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+    if (ctx->d_partial) (void)hipFree(ctx->d_partial);
+    if (ctx->d_result) (void)hipFree(ctx->d_result);
+    if (ctx->h_result) (void)hipHostFree(ctx->h_result);
+    if (ctx->d_scalars) (void)hipFree(ctx->d_scalars);
+    if (ctx->spin.pairs) (void)hipFree(ctx->spin.pairs);
+    if (ctx->spin.btype) (void)hipFree(ctx->spin.btype);
+    reset_spin_tables(ctx);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return PNED_OK;
+}
+
+int64_t pned_ctx_dim(const pned_ctx *ctx) { return ctx ? ctx->dim : -1; }
+
+int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t *llocal_ham)
+{
+    if (!ctx || !xlocal_ham || !llocal_ham) return fail(PNED_E_ARG, "null argument");
+    const size_t qq = (size_t)ctx->q * ctx->q;
+    ctx->xl.assign(xlocal_ham, xlocal_ham + 2 * qq * qq * 9);
+    ctx->ll.assign(llocal_ham, llocal_ham + qq * qq * 9);
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->q == 2) {
+        PNED_TRY(s12_decompose(ctx->n, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), xlocal_ham, llocal_ham,
+                               ctx->s12));
+        ctx->plan_colbits = -1;
+        ctx->plan_dtype = -1;
+    } else {
+        reset_spin_tables(ctx);
+    }
+    return PNED_OK;
+}
+
+int pned_ctx_tune(pned_ctx *ctx, int tile_bits, int low_bits, int threads)
+{
+    if (!ctx) return fail(PNED_E_ARG, "null context");
+    if (tile_bits < 0 || tile_bits > 14 || low_bits < 0 || low_bits > 8 || (threads != 0 && threads != 256 && threads != 512))
+        return fail(PNED_E_ARG, "tile_bits in 0..14, low_bits in 0..8, threads 0, 256 or 512");
+    ctx->tune_tile_bits = tile_bits;
+    ctx->tune_low_bits = low_bits;
+    ctx->tune_threads = threads;
+    ctx->plan_colbits = -1;
+    return PNED_OK;
+}
+
+// Autotuner: times one fresh H.v v1 -> v0 for each candidate (tile bits, forced low bits,
+// threads per block) that fits LDS, keeps the fastest (v0 is overwritten).
+int pned_ctx_autotune(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, int reps, int *best_tile_bits,
+                      int *best_low_bits, int *best_threads, double *best_ms)
+{
+    if (!ctx || !v1 || !v0) return fail(PNED_E_ARG, "null argument");
+    if (ctx->q != 2) {
+        if (best_ms) *best_ms = 0.0;
+        return PNED_OK;   // spin-S kernel has no tile knobs
+    }
+    if (reps < 1) reps = 2;
+    const int kmax = (v1->dtype == PNED_C128) ? 13 : 14;
+    hipEvent_t e0, e1;
+    PNED_HIP(hipEventCreate(&e0));
+    PNED_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    int bk = 0, bl = 0, bt = 0;
+    for (int K = kmax - 2; K <= kmax; ++K)
+        for (int L = 3; L <= 4; ++L)
+            for (int T : {256, 512}) {
+                ctx->tune_tile_bits = K;
+                ctx->tune_low_bits = L;
+                ctx->tune_threads = T;
+                ctx->plan_colbits = -1;
+                int rc = launch_hv_s12(ctx, v1, v0, 0.0, true, nullptr);   // warm-up, builds the plan
+                if (rc != PNED_OK) continue;
+                PNED_HIP(hipEventRecord(e0, ctx->stream));
+                for (int k = 0; k < reps; ++k) PNED_TRY(launch_hv_s12(ctx, v1, v0, 0.0, true, nullptr));
+                PNED_HIP(hipEventRecord(e1, ctx->stream));
+                PNED_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                PNED_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms / reps < best) { best = ms / reps; bk = K; bl = L; bt = T; }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->tune_tile_bits = bk;
+    ctx->tune_low_bits = bl;
+    ctx->tune_threads = bt;
+    ctx->plan_colbits = -1;
+    if (best_tile_bits) *best_tile_bits = bk;
+    if (best_low_bits) *best_low_bits = bl;
+    if (best_threads) *best_threads = bt;
+    if (best_ms) *best_ms = best;
+    return PNED_OK;
+}
+
+int pned_ctx_plan_info(const pned_ctx *ctx, int *npasses, int *terms_per_pass, int max)
+{
+    if (!ctx || !npasses) return fail(PNED_E_ARG, "null argument");
+    if (ctx->q != 2) {
+        *npasses = 1;
+        if (terms_per_pass && max > 0) terms_per_pass[0] = ctx->ibond;
+        return PNED_OK;
+    }
+    *npasses = (int)ctx->s12.passes.size();
+    for (int p = 0; p < *npasses && p < max && terms_per_pass; ++p) terms_per_pass[p] = ctx->s12.passes[p].nterm;
+    return PNED_OK;
+}
+
+int pned_s12_plan_export(int n_sites, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                         const double *xlocal_ham, const int32_t *llocal_ham, int colbits, int tile_bits,
+                         int low_bits, int *npass, int *nterm, int *ngroup, int32_t *pass_info,
+                         int32_t *pass_bits, double *terms, uint32_t *term_gmask, double *diag, int max_pass,
+                         int max_term)
+{
+    if (!ipair || !ibond_type || !xlocal_ham || !llocal_ham || !npass || !nterm || !ngroup)
+        return fail(PNED_E_ARG, "null argument");
+    S12Plan pl;
+    PNED_TRY(s12_decompose(n_sites, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, pl));
+    PNED_TRY(s12_make_passes(pl, colbits, tile_bits, low_bits));
+    *npass = (int)pl.passes.size();
+    *nterm = (int)pl.devterms.size();
+    *ngroup = (int)pl.diag.size();
+    if (*npass > max_pass || *nterm > max_term) return fail(PNED_E_ARG, "output arrays too small");
+    // term gmask in devterm order: recover from tmask and the pass's tile bits
+    for (int p = 0; p < *npass; ++p) {
+        const PassPlan &pp = pl.passes[p];
+        pass_info[4 * p] = pp.K;
+        pass_info[4 * p + 1] = pp.first_term;
+        pass_info[4 * p + 2] = pp.nterm;
+        pass_info[4 * p + 3] = pp.diag ? 1 : 0;
+        for (int k = 0; k < 40; ++k) pass_bits[40 * p + k] = k < pp.K ? pp.tb[k] : -1;
+        for (int t = pp.first_term; t < pp.first_term + pp.nterm; ++t) {
+            const DevTerm &d = pl.devterms[t];
+            uint64_t m = 0;
+            for (int k = 0; k < pp.K; ++k)
+                if (d.tmask >> k & 1) m |= 1ull << pp.tb[k];
+            term_gmask[t] = (uint32_t)(m >> colbits);
+            terms[6 * t] = d.tmask;
+            terms[6 * t + 1] = d.selmask;
+            terms[6 * t + 2] = d.c0r;
+            terms[6 * t + 3] = d.c0i;
+            terms[6 * t + 4] = d.c1r;
+            terms[6 * t + 5] = d.c1i;
+        }
+    }
+    for (int g = 0; g < *ngroup; ++g) {
+        diag[5 * g] = pl.diag[g].kind;
+        diag[5 * g + 1] = pl.diag[g].shift;
+        diag[5 * g + 2] = pl.diag[g].mask;
+        diag[5 * g + 3] = pl.diag[g].wr;
+        diag[5 * g + 4] = pl.diag[g].wi;
+    }
+    diag[5 * *ngroup] = pl.c0r;
+    diag[5 * *ngroup + 1] = pl.c0i;
+    return PNED_OK;
+}
+
+int pned_ctx_sync(pned_ctx *ctx)
+{
+    if (!ctx) return fail(PNED_E_ARG, "null context");
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    return PNED_OK;
+}
+
+void *pned_ctx_stream(pned_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int pned_vec_create(pned_ctx *ctx, int nvec, int dtype, pned_vec **out)
+{
+    if (!ctx || !out) return fail(PNED_E_ARG, "null argument");
+    *out = nullptr;
+    if (nvec < 1 || (dtype != PNED_C128 && dtype != PNED_C64)) return fail(PNED_E_ARG, "bad nvec or dtype");
+    auto *v = new pned_vec();
+    v->ctx = ctx;
+    v->dim = ctx->dim;
+    v->nvec = nvec;
+    v->dtype = dtype;
+    v->bytes = (size_t)ctx->dim * nvec * elem_bytes(dtype);
+    PNED_HIP(hipSetDevice(ctx->device));
+    if (hipMalloc(&v->data, v->bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        delete v;
+        return fail(PNED_E_NOMEM, "device allocation of " + std::to_string(v->bytes) + " bytes failed");
+    }
+    *out = v;
+    return PNED_OK;
+}
+
+int pned_vec_destroy(pned_vec *v)
+{
+    if (!v) return PNED_OK;
+    if (v->ctx && v->ctx->stream) (void)hipStreamSynchronize(v->ctx->stream);
+    if (v->data) (void)hipFree(v->data);
+    delete v;
+    return PNED_OK;
+}
+
+int pned_vec_info(const pned_vec *v, int64_t *dim, int *nvec, int *dtype)
+{
+    if (!v) return fail(PNED_E_ARG, "null vector");
+    if (dim) *dim = v->dim;
+    if (nvec) *nvec = v->nvec;
+    if (dtype) *dtype = v->dtype;
+    return PNED_OK;
+}
+
+void *pned_vec_data(pned_vec *v) { return v ? v->data : nullptr; }
+
+int pned_vec_upload(pned_ctx *ctx, pned_vec *v, const void *host, int64_t offset, int64_t count)
+{
+    if (!ctx || !v || !host) return fail(PNED_E_ARG, "null argument");
+    if (offset < 0 || count < 0 || offset + count > v->dim) return fail(PNED_E_ARG, "range outside the vector");
+    const size_t eb = elem_bytes(v->dtype) * v->nvec;
+    PNED_HIP(hipMemcpyAsync((char *)v->data + eb * offset, host, eb * count, hipMemcpyHostToDevice, ctx->stream));
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    return PNED_OK;
+}
+
+int pned_vec_download(pned_ctx *ctx, const pned_vec *v, void *host, int64_t offset, int64_t count)
+{
+    if (!ctx || !v || !host) return fail(PNED_E_ARG, "null argument");
+    if (offset < 0 || count < 0 || offset + count > v->dim) return fail(PNED_E_ARG, "range outside the vector");
+    const size_t eb = elem_bytes(v->dtype) * v->nvec;
+    PNED_HIP(hipMemcpyAsync(host, (const char *)v->data + eb * offset, eb * count, hipMemcpyDeviceToHost, ctx->stream));
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    return PNED_OK;
+}
+
+int pned_vec_zero(pned_ctx *ctx, pned_vec *v)
+{
+    if (!ctx || !v) return fail(PNED_E_ARG, "null argument");
+    PNED_HIP(hipMemsetAsync(v->data, 0, v->bytes, ctx->stream));
+    return PNED_OK;
+}
+
+int pned_vec_copy(pned_ctx *ctx, const pned_vec *src, pned_vec *dst)
+{
+    if (!ctx || !src || !dst) return fail(PNED_E_ARG, "null argument");
+    if (src->bytes != dst->bytes || src->dtype != dst->dtype) return fail(PNED_E_ARG, "shape mismatch");
+    PNED_HIP(hipMemcpyAsync(dst->data, src->data, src->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return PNED_OK;
+}
+
+int pned_set_initial_vec(pned_ctx *ctx, pned_vec *v, uint64_t seed)
+{
+    if (!ctx || !v) return fail(PNED_E_ARG, "null argument");
+    return gaussian_fill(ctx, v, seed);
+}
+
+int pned_p_mltply_v3(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, double *prdct)
+{
+    return hv_dispatch(ctx, v1, v0, eperbd, false, prdct);
+}
+
+int pned_hv_fresh(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, double *prdct)
+{
+    return hv_dispatch(ctx, v1, v0, eperbd, true, prdct);
+}
+
+int pned_calc_vec_inprdct(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out)
+{
+    if (!ctx || !x || !y || !out) return fail(PNED_E_ARG, "null argument");
+    return dot(ctx, x, y, out);
+}
+
+int pned_calc_beta(pned_ctx *ctx, const double *alpha, const pned_vec *v1, const pned_vec *v0, double *b2)
+{
+    if (!ctx || !alpha || !v1 || !v0 || !b2) return fail(PNED_E_ARG, "null argument");
+    return beta2(ctx, alpha, v1, v0, b2);
+}
+
+int pned_calc_alpha_beta(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double *alpha, double *beta)
+{
+    if (!ctx || !v1 || !v0 || !alpha || !beta) return fail(PNED_E_ARG, "null argument");
+    const int nc = v1->nvec;
+    std::vector<double> pr(2 * nc), b2(nc);
+    PNED_TRY(hv_dispatch(ctx, v1, v0, 0.0, false, pr.data()));
+    PNED_TRY(beta2(ctx, pr.data(), v1, v0, b2.data()));
+    for (int c = 0; c < nc; ++c) {
+        alpha[c] = pr[2 * c];
+        beta[c] = std::sqrt(b2[c]);
+    }
+    return PNED_OK;
+}
+
+int pned_calc_alpha_beta2(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double *alpha, double *beta)
+{
+    if (!ctx || !v1 || !v0 || !alpha || !beta) return fail(PNED_E_ARG, "null argument");
+    const int nc = v1->nvec;
+    std::vector<double> pr(2 * nc), nn(2 * nc);
+    PNED_TRY(hv_dispatch(ctx, v1, v0, 0.0, false, pr.data()));
+    PNED_TRY(dot(ctx, v0, v0, nn.data()));
+    for (int c = 0; c < nc; ++c) {
+        alpha[c] = pr[2 * c];
+        beta[c] = std::sqrt(nn[2 * c]);
+    }
+    return PNED_OK;
+}
+
+int pned_axpby(pned_ctx *ctx, const double *a, const pned_vec *x, const double *b, pned_vec *y)
+{
+    if (!ctx || !a || !x || !b || !y) return fail(PNED_E_ARG, "null argument");
+    return axpby(ctx, a, x, b, y);
+}
+
+int pned_scale(pned_ctx *ctx, pned_vec *x, const double *a)
+{
+    if (!ctx || !x || !a) return fail(PNED_E_ARG, "null argument");
+    return scale(ctx, x, a);
+}
+
+int pned_lanczos_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double *alpha, const double *beta)
+{
+    if (!ctx || !v1 || !v0 || !alpha || !beta) return fail(PNED_E_ARG, "null argument");
+    return lanczos_update(ctx, v1, v0, alpha, beta);
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+// internal.h -- shared declarations of libpned_gpu (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pned_gpu.h"
+
+namespace pned {
+
+// ------------------------------------------------------------------------------------
+// error plumbing: every ABI entry returns a status; the message is kept per thread
+// ------------------------------------------------------------------------------------
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define PNED_HIP(call)                                                                    \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::pned::fail(PNED_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define PNED_TRY(expr)                                                                    \
+    do {                                                                                  \
+        int rc_ = (expr);                                                                 \
+        if (rc_ != PNED_OK) return rc_;                                                   \
+    } while (0)
+
+// ------------------------------------------------------------------------------------
+// S=1/2 H.v plan: the operator decomposed into a real-or-complex diagonal and flip terms,
+// grouped into tile passes.  Memory index m = (state << colbits) | column.
+// ------------------------------------------------------------------------------------
+struct Term {            // one off-diagonal term: v0(i) += c(sel) * v1(i ^ gmask)
+    uint32_t gmask;      // flipped state bits
+    uint32_t selmask;    // coefficient selector = parity of (state & selmask)
+    double c0r, c0i;     // coefficient when selector = 0
+    double c1r, c1i;     // coefficient when selector = 1
+};
+
+struct DiagGroup {       // D(i) += w * popcount(f(i))
+    int kind;            // 0: f = i & mask ; 1: f = (i ^ (i >> shift)) & mask
+    int shift;
+    uint32_t mask;
+    double wr, wi;
+};
+
+// device-side term of one pass (tile-local partner mask precomputed)
+struct DevTerm {
+    uint32_t tmask;      // partner offset inside the tile (tile-local bit mask)
+    uint32_t selmask;
+    uint32_t is_real;    // coefficients purely real (fast path)
+    uint32_t pad;
+    double c0r, c0i, c1r, c1i;
+};
+
+struct PassPlan {
+    int K;                   // tile bits (over memory-index bits)
+    std::vector<int> tb;     // tile bit positions (memory index), ascending
+    std::vector<int> ob;     // outer bit positions (memory index), ascending
+    int first_term, nterm;   // range into the concatenated DevTerm table
+    bool diag;               // this pass adds the diagonal (and the v0 input)
+};
+
+struct S12Plan {
+    int n = 0, colbits = 0;
+    std::vector<Term> terms;
+    std::vector<DiagGroup> diag;
+    double c0r = 0, c0i = 0;     // constant part of the diagonal
+    std::vector<PassPlan> passes;
+    std::vector<DevTerm> devterms;
+};
+
+// build the S=1/2 decomposition of the reference tables (no passes yet)
+int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype,
+                  const double *xl_fortran, const int32_t *ll_fortran, S12Plan &plan);
+// group terms into tile passes for a given memory layout
+int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits);
+
+struct DiagDev {             // flattened diagonal for the kernel (constant per context)
+    int ngroups;
+    int kind[48];
+    int shift[48];
+    uint32_t mask[48];
+    double wr[48], wi[48];
+    double c0r, c0i;
+};
+
+// ------------------------------------------------------------------------------------
+// context and vectors
+// ------------------------------------------------------------------------------------
+struct GpuTables {           // spin-S tables in device memory
+    int32_t *pairs = nullptr;    // 2*ibond (0-based min,max)
+    int32_t *btype = nullptr;    // ibond (0-based)
+    double *xl = nullptr;        // [9][qq][qq] complex, [t][in][out]
+    int32_t *mk = nullptr;       // [9][qq][qq]
+};
+
+}  // namespace pned
+
+struct pned_vec {
+    pned_ctx *ctx = nullptr;
+    int64_t dim = 0;
+    int nvec = 1;
+    int dtype = PNED_C128;
+    void *data = nullptr;        // device
+    size_t bytes = 0;
+};
+
+struct pned_ctx {
+    int device = 0;
+    int n = 0, q = 2, ibond = 0;
+    int64_t dim = 0;
+    hipStream_t stream = nullptr;
+    std::vector<int32_t> ipair, btype;
+    std::vector<double> xl;      // Fortran order copy
+    std::vector<int32_t> ll;
+    std::vector<int32_t> site_pair;   // 0-based (min,max) per bond
+    // S=1/2
+    pned::S12Plan s12;
+    pned::DiagDev diag_host;
+    // per-(colbits, dtype) pass layouts are rebuilt on demand
+    int tune_tile_bits = 0, tune_low_bits = 0, tune_threads = 0;
+    int plan_colbits = -1, plan_dtype = -1;
+    pned::DevTerm *d_terms = nullptr;
+    size_t d_terms_cap = 0;
+    pned::DiagDev *d_diag = nullptr;
+    // spin-S
+    pned::GpuTables spin;
+    // scratch for reductions: per-block partials and final results (pinned host mirror)
+    double *d_partial = nullptr;
+    size_t partial_cap = 0;          // in doubles
+    double *d_result = nullptr;
+    double *h_result = nullptr;      // pinned
+    size_t result_cap = 0;           // in doubles
+    double *d_scalars = nullptr;     // small per-call scalar/argument uploads
+    size_t scalars_cap = 0;          // in doubles
+    // mTPQ observables
+    std::vector<double> site_vec;
+    std::vector<double> sq_q;
+    int nsq = 0;
+};
+
+namespace pned {
+// scratch management
+int ensure_partial(pned_ctx *ctx, size_t ndoubles);
+int ensure_result(pned_ctx *ctx, size_t ndoubles);
+int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr);
+int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
+
+// kernels (launchers), all asynchronous on ctx->stream unless stated
+int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                  double *prdct_host);
+int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                   double *prdct_host);
+
+// reductions return host values (synchronizing)
+int dot(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out /*2*nvec*/);
+int beta2(pned_ctx *ctx, const double *alpha /*2*nvec complex*/, const pned_vec *v1,
+          const pned_vec *v0, double *out /*nvec*/);
+int axpby(pned_ctx *ctx, const double *a, const pned_vec *x, const double *b, pned_vec *y);
+int scale(pned_ctx *ctx, pned_vec *x, const double *a);
+int lanczos_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double *alpha,
+                   const double *beta);
+int tps_shift_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, const double *xn);
+int gaussian_fill(pned_ctx *ctx, pned_vec *v, uint64_t seed);
+int sz_q_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *q, const double *site_vec);
+int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, double *out);
+int renbun_gpu(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,
+               const double *salpha, const double *sbeta, double *w, double *dsf, int *conv);
+
+inline size_t elem_bytes(int dtype) { return dtype == PNED_C128 ? 16 : 8; }
+}  // namespace pned

@@ -1,0 +1,667 @@
+// vec_ops.hip -- BLAS-1, start vectors, excitations, observables, continued fractions.
+//
+// All reductions are two-stage and order-fixed (fixed grid, fixed per-thread strides,
+// fixed tree), so a result depends only on the vector contents: the same column gives
+// the same bits alone or batched with others.  Sums are FP64 for both storage types.
+#include <cmath>
+#include <cstring>
+
+#include "internal.h"
+
+namespace pned {
+
+namespace {
+
+constexpr int RED_TPB = 256;
+constexpr int RED_GRID = 1024;
+
+template <typename Real> struct V2;
+template <> struct V2<double> { using T = double2; };
+template <> struct V2<float> { using T = float2; };
+
+__device__ __forceinline__ void ld(const double2 *p, size_t i, double &re, double &im)
+{
+    const double2 v = p[i];
+    re = v.x;
+    im = v.y;
+}
+__device__ __forceinline__ void ld(const float2 *p, size_t i, double &re, double &im)
+{
+    const float2 v = p[i];
+    re = v.x;
+    im = v.y;
+}
+__device__ __forceinline__ void st(double2 *p, size_t i, double re, double im) { p[i] = make_double2(re, im); }
+__device__ __forceinline__ void st(float2 *p, size_t i, double re, double im)
+{
+    p[i] = make_float2((float)re, (float)im);
+}
+
+// per-column tree: after it, red[0..NV*ncol) holds the block sums of thread groups with
+// equal (tid % ncol).  NV values per thread.
+template <int NV>
+__device__ void block_reduce_cols(double *red, double (&v)[NV], int ncol, double *out_block)
+{
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[NV * tid + k] = v[k];
+    __syncthreads();
+    for (int s = RED_TPB / 2; s >= ncol && s > 0; s >>= 1) {
+        if (tid < s) {
+#pragma unroll
+            for (int k = 0; k < NV; ++k) red[NV * tid + k] += red[NV * (tid + s) + k];
+        }
+        __syncthreads();
+    }
+    if (tid < ncol) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out_block[NV * tid + k] = red[NV * tid + k];
+    }
+}
+
+// final stage: out[c] = sum over blocks of partial[block][c]; one block per output,
+// fixed strides and tree (deterministic)
+__global__ __launch_bounds__(256) void reduce_rows(const double *__restrict__ partial, int nrows, int width,
+                                                   double *__restrict__ out)
+{
+    __shared__ double red[256];
+    const int c = blockIdx.x, tid = threadIdx.x;
+    double s = 0.0;
+    for (int b = tid; b < nrows; b += 256) s += partial[(size_t)b * width + c];
+    red[tid] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (tid < k) red[tid] += red[tid + k];
+        __syncthreads();
+    }
+    if (tid == 0) out[c] = red[0];
+}
+
+// dot: sum conj(x) y per column -> 2 values per column
+template <typename Real>
+__global__ __launch_bounds__(RED_TPB) void k_dot(const typename V2<Real>::T *__restrict__ x,
+                                                 const typename V2<Real>::T *__restrict__ y,
+                                                 size_t nelem, int ncol, double *__restrict__ partial)
+{
+    __shared__ double red[2 * RED_TPB];
+    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    double acc[2] = {0.0, 0.0};
+    for (size_t e = (size_t)blockIdx.x * RED_TPB + threadIdx.x; e < nelem; e += stride) {
+        double xr, xi, yr, yi;
+        ld(x, e, xr, xi);
+        ld(y, e, yr, yi);
+        acc[0] = fma(xr, yr, fma(xi, yi, acc[0]));
+        acc[1] = fma(xr, yi, fma(-xi, yr, acc[1]));
+    }
+    block_reduce_cols<2>(red, acc, ncol, partial + (size_t)blockIdx.x * 2 * ncol);
+}
+
+// beta2: sum |v0 - alpha_c v1|^2 per column (alpha complex per column, in constant args)
+template <typename Real>
+__global__ __launch_bounds__(RED_TPB) void k_beta2(const typename V2<Real>::T *__restrict__ v1,
+                                                   const typename V2<Real>::T *__restrict__ v0,
+                                                   const double *__restrict__ alpha, size_t nelem,
+                                                   int ncol, double *__restrict__ partial)
+{
+    __shared__ double red[RED_TPB];
+    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    const size_t e0 = (size_t)blockIdx.x * RED_TPB + threadIdx.x;
+    const int col = (int)(e0 % (size_t)ncol);
+    const double ar = alpha[2 * col], ai = alpha[2 * col + 1];
+    double acc[1] = {0.0};
+    for (size_t e = e0; e < nelem; e += stride) {
+        double xr, xi, yr, yi;
+        ld(v1, e, xr, xi);
+        ld(v0, e, yr, yi);
+        const double wr = yr - (ar * xr - ai * xi);
+        const double wi = yi - (ar * xi + ai * xr);
+        acc[0] = fma(wr, wr, fma(wi, wi, acc[0]));
+    }
+    block_reduce_cols<1>(red, acc, ncol, partial + (size_t)blockIdx.x * ncol);
+}
+
+// y = a_c x + b_c y  (complex scalars per column)
+template <typename Real>
+__global__ void k_axpby(const typename V2<Real>::T *__restrict__ x, typename V2<Real>::T *__restrict__ y,
+                        const double *__restrict__ a, const double *__restrict__ b, size_t nelem, int ncol)
+{
+    const size_t stride = (size_t)blockDim.x * gridDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += stride) {
+        const int c = (int)(e % (size_t)ncol);
+        double xr, xi, yr, yi;
+        ld(x, e, xr, xi);
+        ld(y, e, yr, yi);
+        const double ar = a[2 * c], ai = a[2 * c + 1], br = b[2 * c], bi = b[2 * c + 1];
+        st(y, e, ar * xr - ai * xi + br * yr - bi * yi, ar * xi + ai * xr + br * yi + bi * yr);
+    }
+}
+
+template <typename Real>
+__global__ void k_scale(typename V2<Real>::T *__restrict__ x, const double *__restrict__ a, size_t nelem,
+                        int ncol)
+{
+    const size_t stride = (size_t)blockDim.x * gridDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += stride) {
+        const int c = (int)(e % (size_t)ncol);
+        double xr, xi;
+        ld(x, e, xr, xi);
+        const double ar = a[2 * c], ai = a[2 * c + 1];
+        st(x, e, ar * xr - ai * xi, ar * xi + ai * xr);
+    }
+}
+
+// Lanczos three-term update (Hybrid_TPS_Lanczos.f90:111-116), alpha real per column
+template <typename Real>
+__global__ void k_lanczos_update(typename V2<Real>::T *__restrict__ v1, typename V2<Real>::T *__restrict__ v0,
+                                 const double *__restrict__ ab, size_t nelem, int ncol)
+{
+    const size_t stride = (size_t)blockDim.x * gridDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += stride) {
+        const int c = (int)(e % (size_t)ncol);
+        const double al = ab[2 * c], be = ab[2 * c + 1];
+        double xr, xi, yr, yi;
+        ld(v1, e, xr, xi);
+        ld(v0, e, yr, yi);
+        st(v0, e, -be * xr, -be * xi);
+        st(v1, e, (yr - al * xr) / be, (yi - al * xi) / be);
+    }
+}
+
+// mTPQ update (Hybrid_TPS_TPS.f90:380-386): vg = v1; v1 = xnorm*v0; v0 = 0
+template <typename Real>
+__global__ void k_tps_update(typename V2<Real>::T *__restrict__ v1, typename V2<Real>::T *__restrict__ v0,
+                             typename V2<Real>::T *__restrict__ vg, const double *__restrict__ xn,
+                             size_t nelem, int ncol)
+{
+    const size_t stride = (size_t)blockDim.x * gridDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += stride) {
+        const int c = (int)(e % (size_t)ncol);
+        double xr, xi, yr, yi;
+        ld(v1, e, xr, xi);
+        ld(v0, e, yr, yi);
+        st(vg, e, xr, xi);
+        st(v1, e, xn[c] * yr, xn[c] * yi);
+        st(v0, e, 0.0, 0.0);
+    }
+}
+
+// ---- Philox-4x32-10 (Salmon et al., SC'11) + Box-Muller --------------------------------
+__device__ __forceinline__ void philox(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
+                                       uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(RED_TPB) void k_gauss(typename V2<Real>::T *__restrict__ v, uint64_t seed,
+                                                   size_t nelem, int ncol, double *__restrict__ partial)
+{
+    __shared__ double red[RED_TPB];
+    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    double acc[1] = {0.0};
+    for (size_t e = (size_t)blockIdx.x * RED_TPB + threadIdx.x; e < nelem; e += stride) {
+        const uint64_t i = e / (size_t)ncol;
+        const uint32_t col = (uint32_t)(e % (size_t)ncol);
+        uint32_t c0 = (uint32_t)i, c1 = (uint32_t)(i >> 32), c2 = col, c3 = 0u;
+        philox(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const uint64_t a = ((uint64_t)c0 << 21) ^ (c1 >> 11);
+        const uint64_t b = ((uint64_t)c2 << 21) ^ (c3 >> 11);
+        const double u1 = ((double)(a & ((1ull << 53) - 1)) + 1.0) * 0x1.0p-53;
+        const double u2 = (double)(b & ((1ull << 53) - 1)) * 0x1.0p-53;
+        const double r = sqrt(-2.0 * log(u1));
+        const double th = 6.283185307179586476925286766559 * u2;
+        double s, c;
+        sincos(th, &s, &c);
+        const double re = r * c, im = r * s;
+        st(v, e, re, im);
+        double qr, qi;
+        ld(v, e, qr, qi);   // norm of the stored (possibly rounded) value
+        acc[0] = fma(qr, qr, fma(qi, qi, acc[0]));
+    }
+    block_reduce_cols<1>(red, acc, ncol, partial + (size_t)blockIdx.x * ncol);
+}
+
+// ---- excitation S^z(q) x (intalv_zz, Hybrid_TPS_DSF.f90:18-87) ---------------------------
+struct SzArgs {
+    int n, q;
+    double phr[32], phi[32];  // e^{i q.r_j}
+    double wlow;              // weight offset: 0.5 (S=1/2) or S
+};
+
+template <typename Real>
+__global__ void k_sz_q(const typename V2<Real>::T *__restrict__ x, typename V2<Real>::T *__restrict__ v,
+                       SzArgs a, size_t dim)
+{
+    const size_t stride = (size_t)blockDim.x * gridDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        double cr = 0.0, ci = 0.0;
+        uint64_t s = i;
+        for (int j = 0; j < a.n; ++j) {
+            const int d = (int)(s % (uint64_t)a.q);
+            s /= (uint64_t)a.q;
+            const double w = (double)d - a.wlow;
+            cr = fma(w, a.phr[j], cr);
+            ci = fma(w, a.phi[j], ci);
+        }
+        double xr, xi;
+        ld(x, i, xr, xi);
+        st(v, i, cr * xr - ci * xi, cr * xi + ci * xr);
+    }
+}
+
+// ---- mTPQ observables of eval_phys_val (Hybrid_TPS_TPS.f90:83-322), fused ------------------
+// Per column: out[0..1] = <vg|v1>, out[2..3] = <v1|Wzz|v1>, out[4..5] = <vg|Wzz|v1>,
+// then for each requested structure-factor entry k: out[6+2k..] = <a_k|W_k|v1>.
+constexpr int OBS_MAX_SQ = 32;
+struct ObsArgs {
+    int n;
+    uint32_t zz_mask1[64], zz_mask2[64];  // z-type bond bit masks (off-by-one bits kept)
+    int nzz;
+    int nsq;                              // number of structure-factor entries computed
+    int sq_mode[OBS_MAX_SQ];              // 0: zz weight, 1: pp weight
+    int sq_left[OBS_MAX_SQ];              // 0: <v1|, 1: <vg|
+    int sq_q[OBS_MAX_SQ];                 // momentum slot
+    double phr[8][32], phi[8][32];        // e^{i q.r_j} per momentum slot
+};
+
+template <typename Real, int NOUT>
+__global__ __launch_bounds__(RED_TPB) void k_tps_obs(const typename V2<Real>::T *__restrict__ vg,
+                                                     const typename V2<Real>::T *__restrict__ v1,
+                                                     const ObsArgs *__restrict__ ap, size_t nelem,
+                                                     int ncol, double *__restrict__ partial)
+{
+    extern __shared__ double red[];
+    const ObsArgs &a = *ap;
+    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    double acc[NOUT];
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) acc[k] = 0.0;
+    const int nout = 6 + 2 * a.nsq;
+    for (size_t e = (size_t)blockIdx.x * RED_TPB + threadIdx.x; e < nelem; e += stride) {
+        const uint32_t s = (uint32_t)(e / (size_t)ncol);
+        double gr, gi, xr, xi;
+        ld(vg, e, gr, gi);
+        ld(v1, e, xr, xi);
+        acc[0] = fma(gr, xr, fma(gi, xi, acc[0]));
+        acc[1] = fma(gr, xi, fma(-gi, xr, acc[1]));
+        double wzz = 0.0;
+        for (int b = 0; b < a.nzz; ++b) {
+            const double s3 = (s & a.zz_mask1[b]) ? 0.5 : -0.5;
+            const double s4 = (s & a.zz_mask2[b]) ? 0.5 : -0.5;
+            wzz = fma(s3, s4, wzz);
+        }
+        const double n1 = xr * xr + xi * xi;
+        acc[2] = fma(wzz, n1, acc[2]);
+        acc[4] = fma(wzz, gr * xr + gi * xi, acc[4]);
+        acc[5] = fma(wzz, gr * xi - gi * xr, acc[5]);
+        const int s0 = (int)(s & 1u);
+        for (int k = 0; k < a.nsq && 6 + 2 * k + 1 < NOUT; ++k) {
+            const int qs = a.sq_q[k];
+            double cr = 0.0, ci = 0.0;
+            for (int j = 0; j < a.n; ++j) {
+                const int sj = (int)((s >> j) & 1u);
+                double w;
+                if (a.sq_mode[k] == 0) w = ((double)sj - 0.5) * ((double)s0 - 0.5);
+                else w = (sj != s0) ? 0.25 : 0.0;
+                cr = fma(w, a.phr[qs][j], cr);
+                ci = fma(w, a.phi[qs][j], ci);
+            }
+            // W x
+            const double wr = cr * xr - ci * xi, wi = cr * xi + ci * xr;
+            double lr = xr, li = xi;
+            if (a.sq_left[k]) { lr = gr; li = gi; }
+            acc[6 + 2 * k] = fma(lr, wr, fma(li, wi, acc[6 + 2 * k]));
+            acc[6 + 2 * k + 1] = fma(lr, wi, fma(-li, wr, acc[6 + 2 * k + 1]));
+        }
+    }
+    (void)nout;
+    // per-column block reduction of NOUT values
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) red[NOUT * tid + k] = acc[k];
+    __syncthreads();
+    for (int s2 = RED_TPB / 2; s2 >= ncol && s2 > 0; s2 >>= 1) {
+        if (tid < s2) {
+#pragma unroll
+            for (int k = 0; k < NOUT; ++k) red[NOUT * tid + k] += red[NOUT * (tid + s2) + k];
+        }
+        __syncthreads();
+    }
+    if (tid < ncol) {
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k)
+            partial[(size_t)blockIdx.x * NOUT * ncol + NOUT * tid + k] = red[NOUT * tid + k];
+    }
+}
+
+// ---- continued fraction (exec_renbun, Hybrid_TPS_DSF.f90:263-334): one thread per omega ----
+__global__ void k_renbun(int i_range, int s_maxitr, double split, double ene, double eps,
+                         const double *__restrict__ salpha, const double *__restrict__ sbeta,
+                         double *__restrict__ w, double *__restrict__ dsf, int *__restrict__ conv)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= i_range) return;
+    const double omega = (double)j * split;
+    w[j] = omega;
+    int i_conv = 0;
+    double xr = 1.0e-10, xi = 1.0e-10, cr = xr, ci = xi, dr = 0.0, di = 0.0;
+    for (int m1 = 1; m1 <= s_maxitr - 1; ++m1) {
+        const double f1 = omega + ene - salpha[m1 - 1];
+        const double f2 = -1.0 * (sbeta[m1 - 1] * sbeta[m1 - 1]);
+        const double xr1 = xr, xi1 = xi;
+        dr = dr * f2 + f1;
+        di = di * f2 + eps;
+        double d = dr * dr + di * di;
+        if (d == 0.0) { dr = 0.5e-28; di = 0.5e-28; }
+        double c = cr * cr + ci * ci;
+        cr = cr * f2 / c + f1;
+        ci = -ci * f2 / c + eps;
+        c = cr * cr + ci * ci;
+        if (c == 0.0) { cr = 0.5e-28; ci = 0.5e-28; }
+        d = dr * dr + di * di;
+        dr = dr / d;
+        di = -di / d;
+        const double dlr = cr * dr - ci * di, dli = ci * dr + cr * di;
+        xr = xr1 * dlr - xi1 * dli;
+        xi = xr1 * dli + xi1 * dlr;
+        const double dl = (dlr - 1.0) * (dlr - 1.0) + dli * dli;
+        if (dl < 0.5e-28) { i_conv = m1; break; }
+    }
+    conv[j] = i_conv;
+    dsf[j] = xi;
+}
+
+inline int grid_for(size_t n, int tpb, int cap = 8192)
+{
+    size_t g = (n + tpb - 1) / tpb;
+    if (g > (size_t)cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+int check_pow2_cols(int ncol)
+{
+    if (ncol < 1 || (ncol & (ncol - 1)) || ncol > RED_TPB)
+        return fail(PNED_E_UNSUPPORTED, "column count must be a power of two <= 256");
+    return PNED_OK;
+}
+
+// run the final stage and copy 'width' doubles to host (synchronizing)
+int finish_reduce(pned_ctx *ctx, int nrows, int width, double *out)
+{
+    PNED_TRY(ensure_result(ctx, (size_t)width));
+    hipLaunchKernelGGL(reduce_rows, dim3(width), dim3(256), 0, ctx->stream, ctx->d_partial, nrows, width,
+                       ctx->d_result);
+    PNED_HIP(hipGetLastError());
+    PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double) * width, hipMemcpyDeviceToHost,
+                            ctx->stream));
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < width; ++c) out[c] = ctx->h_result[c];
+    return PNED_OK;
+}
+
+int same_shape(const pned_vec *a, const pned_vec *b)
+{
+    if (!a || !b || a->dim != b->dim || a->nvec != b->nvec || a->dtype != b->dtype)
+        return fail(PNED_E_ARG, "vector shapes or types differ");
+    return PNED_OK;
+}
+
+}  // namespace
+
+int dot(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out)
+{
+    PNED_TRY(same_shape(x, y));
+    PNED_TRY(check_pow2_cols(x->nvec));
+    const size_t nelem = (size_t)x->dim * x->nvec;
+    const int grid = RED_GRID;
+    PNED_TRY(ensure_partial(ctx, (size_t)grid * 2 * x->nvec));
+    if (x->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_dot<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const double2 *)x->data,
+                           (const double2 *)y->data, nelem, x->nvec, ctx->d_partial);
+    else
+        hipLaunchKernelGGL(k_dot<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const float2 *)x->data,
+                           (const float2 *)y->data, nelem, x->nvec, ctx->d_partial);
+    PNED_HIP(hipGetLastError());
+    return finish_reduce(ctx, grid, 2 * x->nvec, out);
+}
+
+int beta2(pned_ctx *ctx, const double *alpha, const pned_vec *v1, const pned_vec *v0, double *out)
+{
+    PNED_TRY(same_shape(v1, v0));
+    PNED_TRY(check_pow2_cols(v1->nvec));
+    const size_t nelem = (size_t)v1->dim * v1->nvec;
+    const int grid = RED_GRID;
+    PNED_TRY(ensure_partial(ctx, (size_t)grid * v1->nvec));
+    double *da;
+    PNED_TRY(upload_scalars(ctx, alpha, 2 * v1->nvec, &da));
+    if (v1->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_beta2<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const double2 *)v1->data,
+                           (const double2 *)v0->data, (const double *)da, nelem, v1->nvec, ctx->d_partial);
+    else
+        hipLaunchKernelGGL(k_beta2<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const float2 *)v1->data,
+                           (const float2 *)v0->data, (const double *)da, nelem, v1->nvec, ctx->d_partial);
+    PNED_HIP(hipGetLastError());
+    return finish_reduce(ctx, grid, v1->nvec, out);
+}
+
+int axpby(pned_ctx *ctx, const double *a, const pned_vec *x, const double *b, pned_vec *y)
+{
+    PNED_TRY(same_shape(x, y));
+    const size_t nelem = (size_t)x->dim * x->nvec;
+    std::vector<double> ab(4 * x->nvec);
+    for (int c = 0; c < 2 * x->nvec; ++c) { ab[c] = a[c]; ab[2 * x->nvec + c] = b[c]; }
+    double *d;
+    PNED_TRY(upload_scalars(ctx, ab.data(), 4 * x->nvec, &d));
+    const int grid = grid_for(nelem, 256);
+    if (x->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_axpby<double>, dim3(grid), dim3(256), 0, ctx->stream, (const double2 *)x->data,
+                           (double2 *)y->data, (const double *)d, (const double *)(d + 2 * x->nvec), nelem, x->nvec);
+    else
+        hipLaunchKernelGGL(k_axpby<float>, dim3(grid), dim3(256), 0, ctx->stream, (const float2 *)x->data,
+                           (float2 *)y->data, (const double *)d, (const double *)(d + 2 * x->nvec), nelem, x->nvec);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
+int scale(pned_ctx *ctx, pned_vec *x, const double *a)
+{
+    const size_t nelem = (size_t)x->dim * x->nvec;
+    double *d;
+    PNED_TRY(upload_scalars(ctx, a, 2 * x->nvec, &d));
+    const int grid = grid_for(nelem, 256);
+    if (x->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_scale<double>, dim3(grid), dim3(256), 0, ctx->stream, (double2 *)x->data,
+                           (const double *)d, nelem, x->nvec);
+    else
+        hipLaunchKernelGGL(k_scale<float>, dim3(grid), dim3(256), 0, ctx->stream, (float2 *)x->data,
+                           (const double *)d, nelem, x->nvec);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
+int lanczos_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double *alpha, const double *beta)
+{
+    PNED_TRY(same_shape(v1, v0));
+    const size_t nelem = (size_t)v1->dim * v1->nvec;
+    std::vector<double> ab(2 * v1->nvec);
+    for (int c = 0; c < v1->nvec; ++c) { ab[2 * c] = alpha[c]; ab[2 * c + 1] = beta[c]; }
+    double *d;
+    PNED_TRY(upload_scalars(ctx, ab.data(), 2 * v1->nvec, &d));
+    const int grid = grid_for(nelem, 256);
+    if (v1->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_lanczos_update<double>, dim3(grid), dim3(256), 0, ctx->stream, (double2 *)v1->data,
+                           (double2 *)v0->data, (const double *)d, nelem, v1->nvec);
+    else
+        hipLaunchKernelGGL(k_lanczos_update<float>, dim3(grid), dim3(256), 0, ctx->stream, (float2 *)v1->data,
+                           (float2 *)v0->data, (const double *)d, nelem, v1->nvec);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
+int tps_shift_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, const double *xn)
+{
+    PNED_TRY(same_shape(v1, v0));
+    PNED_TRY(same_shape(v1, vg));
+    const size_t nelem = (size_t)v1->dim * v1->nvec;
+    double *d;
+    PNED_TRY(upload_scalars(ctx, xn, v1->nvec, &d));
+    const int grid = grid_for(nelem, 256);
+    if (v1->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_tps_update<double>, dim3(grid), dim3(256), 0, ctx->stream, (double2 *)v1->data,
+                           (double2 *)v0->data, (double2 *)vg->data, (const double *)d, nelem, v1->nvec);
+    else
+        hipLaunchKernelGGL(k_tps_update<float>, dim3(grid), dim3(256), 0, ctx->stream, (float2 *)v1->data,
+                           (float2 *)v0->data, (float2 *)vg->data, (const double *)d, nelem, v1->nvec);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
+int gaussian_fill(pned_ctx *ctx, pned_vec *v, uint64_t seed)
+{
+    PNED_TRY(check_pow2_cols(v->nvec));
+    const size_t nelem = (size_t)v->dim * v->nvec;
+    const int grid = RED_GRID;
+    PNED_TRY(ensure_partial(ctx, (size_t)grid * v->nvec));
+    if (v->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_gauss<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (double2 *)v->data, seed,
+                           nelem, v->nvec, ctx->d_partial);
+    else
+        hipLaunchKernelGGL(k_gauss<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (float2 *)v->data, seed,
+                           nelem, v->nvec, ctx->d_partial);
+    PNED_HIP(hipGetLastError());
+    std::vector<double> nrm(v->nvec);
+    PNED_TRY(finish_reduce(ctx, grid, v->nvec, nrm.data()));
+    std::vector<double> sc(2 * v->nvec);
+    for (int c = 0; c < v->nvec; ++c) { sc[2 * c] = 1.0 / std::sqrt(nrm[c]); sc[2 * c + 1] = 0.0; }
+    return scale(ctx, v, sc.data());
+}
+
+int sz_q_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, const double *site_vec)
+{
+    PNED_TRY(same_shape(x, v));
+    if (x->nvec != 1) return fail(PNED_E_UNSUPPORTED, "S^z(q) excitation takes one column");
+    if (ctx->n > 32) return fail(PNED_E_UNSUPPORTED, "at most 32 sites");
+    SzArgs a{};
+    a.n = ctx->n;
+    a.q = ctx->q;
+    a.wlow = 0.5 * (ctx->q - 1);
+    for (int j = 0; j < ctx->n; ++j) {
+        const double qn = qv[0] * site_vec[2 * j] + qv[1] * site_vec[2 * j + 1];
+        a.phr[j] = std::cos(qn);
+        a.phi[j] = std::sin(qn);
+    }
+    const int grid = grid_for((size_t)x->dim, 256);
+    if (x->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_sz_q<double>, dim3(grid), dim3(256), 0, ctx->stream, (const double2 *)x->data,
+                           (double2 *)v->data, a, (size_t)x->dim);
+    else
+        hipLaunchKernelGGL(k_sz_q<float>, dim3(grid), dim3(256), 0, ctx->stream, (const float2 *)x->data,
+                           (float2 *)v->data, a, (size_t)x->dim);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
+// out layout per column: [vg.v1 (2)] [v1 Wzz v1 (2)] [vg Wzz v1 (2)] [sq entries (2 each)]
+int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, double *out)
+{
+    PNED_TRY(same_shape(vg, v1));
+    PNED_TRY(check_pow2_cols(v1->nvec));
+    if (ctx->q != 2) return fail(PNED_E_UNSUPPORTED, "mTPQ observables are defined for S=1/2");
+    ObsArgs a{};
+    a.n = ctx->n;
+    a.nzz = 0;
+    for (int b = 0; b < ctx->ibond; ++b) {
+        if (ctx->btype[b] != 3) continue;
+        // H/TPS.f90:290-316 shifts by the 1-based site number: bit 'ipair'; past the last
+        // site it reads the rank id (0 at one process) -> mask 0 -> bit value 0
+        const int p1 = ctx->ipair[2 * b], p2 = ctx->ipair[2 * b + 1];
+        a.zz_mask1[a.nzz] = p1 < ctx->n ? (1u << p1) : 0u;
+        a.zz_mask2[a.nzz] = p2 < ctx->n ? (1u << p2) : 0u;
+        if (++a.nzz >= 64) break;
+    }
+    // entries stored by the reference: sq(1..nsq), sq(4k+1..4k+4) = (zz,v1),(zz,vg),(pp,v1),(pp,vg)
+    const int nsq = ctx->nsq;
+    a.nsq = nsq;
+    if (nsq > OBS_MAX_SQ) return fail(PNED_E_UNSUPPORTED, "too many structure-factor momenta");
+    int nslots = 0;
+    for (int k = 0; k < nsq; ++k) {
+        const int qi = k / 4, m = k % 4;
+        a.sq_mode[k] = m / 2;
+        a.sq_left[k] = m % 2;
+        a.sq_q[k] = qi;
+        if (qi + 1 > nslots) nslots = qi + 1;
+    }
+    if (nslots > 8) return fail(PNED_E_UNSUPPORTED, "too many momentum slots");
+    for (int s = 0; s < nslots; ++s)
+        for (int j = 0; j < ctx->n; ++j) {
+            const double qn = ctx->sq_q[2 * s] * ctx->site_vec[2 * j] + ctx->sq_q[2 * s + 1] * ctx->site_vec[2 * j + 1];
+            a.phr[s][j] = std::cos(qn);
+            a.phi[s][j] = std::sin(qn);
+        }
+    constexpr int NOUT = 6 + 2 * OBS_MAX_SQ;   // fixed instance; unused slots stay zero
+    const int width = (6 + 2 * nsq);
+    const size_t nelem = (size_t)v1->dim * v1->nvec;
+    const int grid = RED_GRID / 4;
+    PNED_TRY(ensure_partial(ctx, (size_t)grid * NOUT * v1->nvec));
+    // arguments too large for the kernel parameter block: copy to device scratch
+    double *dargs;
+    const int nd = (int)((sizeof(ObsArgs) + sizeof(double) - 1) / sizeof(double));
+    std::vector<double> hbuf(nd);
+    std::memcpy(hbuf.data(), &a, sizeof(ObsArgs));
+    PNED_TRY(upload_scalars(ctx, hbuf.data(), nd, &dargs));
+    const size_t lds = sizeof(double) * NOUT * RED_TPB;
+    if (v1->dtype == PNED_C128) {
+        auto k = k_tps_obs<double, NOUT>;
+        PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k, dim3(grid), dim3(RED_TPB), lds, ctx->stream, (const double2 *)vg->data,
+                           (const double2 *)v1->data, (const ObsArgs *)dargs, nelem, v1->nvec, ctx->d_partial);
+    } else {
+        auto k = k_tps_obs<float, NOUT>;
+        PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k, dim3(grid), dim3(RED_TPB), lds, ctx->stream, (const float2 *)vg->data,
+                           (const float2 *)v1->data, (const ObsArgs *)dargs, nelem, v1->nvec, ctx->d_partial);
+    }
+    PNED_HIP(hipGetLastError());
+    std::vector<double> all((size_t)NOUT * v1->nvec);
+    PNED_TRY(finish_reduce(ctx, grid, NOUT * v1->nvec, all.data()));
+    for (int c = 0; c < v1->nvec; ++c)
+        for (int k = 0; k < width; ++k) out[(size_t)c * width + k] = all[(size_t)c * NOUT + k];
+    return PNED_OK;
+}
+
+int renbun_gpu(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,
+               const double *salpha, const double *sbeta, double *w, double *dsf, int *conv)
+{
+    const size_t na = (size_t)s_maxitr, nb = (size_t)s_maxitr + 1;
+    const size_t need = na + nb + 2 * (size_t)i_range + (size_t)i_range;
+    double *dbuf = nullptr;
+    PNED_HIP(hipMallocAsync((void **)&dbuf, sizeof(double) * need, ctx->stream));
+    double *da = dbuf, *db = dbuf + na, *dw = db + nb, *dd = dw + i_range;
+    int *dc = reinterpret_cast<int *>(dd + i_range);
+    PNED_HIP(hipMemcpyAsync(da, salpha, sizeof(double) * na, hipMemcpyHostToDevice, ctx->stream));
+    PNED_HIP(hipMemcpyAsync(db, sbeta, sizeof(double) * nb, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_renbun, dim3((i_range + 63) / 64), dim3(64), 0, ctx->stream, i_range, s_maxitr, split,
+                       ene, eta, (const double *)da, (const double *)db, dw, dd, dc);
+    PNED_HIP(hipGetLastError());
+    PNED_HIP(hipMemcpyAsync(w, dw, sizeof(double) * i_range, hipMemcpyDeviceToHost, ctx->stream));
+    PNED_HIP(hipMemcpyAsync(dsf, dd, sizeof(double) * i_range, hipMemcpyDeviceToHost, ctx->stream));
+    PNED_HIP(hipMemcpyAsync(conv, dc, sizeof(int) * i_range, hipMemcpyDeviceToHost, ctx->stream));
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    PNED_HIP(hipFreeAsync(dbuf, ctx->stream));
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    return PNED_OK;
+}
+
+}  // namespace pned

@@ -1,0 +1,381 @@
+"""Python front of libpned_gpu.so (the C ABI in include/pned_gpu.h).
+
+Method names follow the reference modules they replace: ``p_mltply_v3``,
+``calc_vec_inprdct``, ``calc_alpha_beta`` ... (S_one_half/Hybrid_TPS_parallel.f90:6-8),
+``lnc1`` (Hybrid_TPS_Lanczos.f90:7-163), ``inv1`` (Hybrid_TPS_cg.f90:7-100), ``tps``
+(Hybrid_TPS_TPS.f90:45-409), ``intalv_zz``/``dsflnc1z``/``renbun`` (Hybrid_TPS_DSF.f90).
+Errors raise :class:`PnedError` with the library's message (the reference prints and
+stops).  There is no CPU fallback: without the built library or a GPU every call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpned_gpu.so")
+
+C128, C64 = 0, 1
+E_OK, E_ARG, E_NOMEM, E_HIP, E_UNSUPPORTED, E_BREAKDOWN, E_NOCONV = 0, -1, -2, -3, -4, -5, -6
+
+
+class PnedError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pned error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load libpned_gpu.so (built in-tree by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PnedError(E_HIP, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P, i32, i64, f64, u64 = C.c_void_p, C.c_int, C.c_int64, C.c_double, C.c_uint64
+    L.pned_last_error.restype = C.c_char_p
+    L.pned_device_count.restype = i32
+    L.pned_ctx_create.argtypes = [i32, i32, i32, _ip, _ip, _dp, _ip, i32, C.POINTER(P)]
+    L.pned_ctx_destroy.argtypes = [P]
+    L.pned_ctx_dim.argtypes = [P]
+    L.pned_ctx_dim.restype = i64
+    L.pned_ctx_set_matrices.argtypes = [P, _dp, _ip]
+    L.pned_ctx_tune.argtypes = [P, i32, i32, i32]
+    L.pned_ctx_plan_info.argtypes = [P, C.POINTER(i32), _ip, i32]
+    L.pned_ctx_autotune.argtypes = [P, P, P, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                    C.POINTER(f64)]
+    L.pned_ctx_sync.argtypes = [P]
+    L.pned_ctx_stream.argtypes = [P]
+    L.pned_ctx_stream.restype = P
+    L.pned_vec_create.argtypes = [P, i32, i32, C.POINTER(P)]
+    L.pned_vec_destroy.argtypes = [P]
+    L.pned_vec_upload.argtypes = [P, P, P, i64, i64]
+    L.pned_vec_download.argtypes = [P, P, P, i64, i64]
+    L.pned_vec_zero.argtypes = [P, P]
+    L.pned_vec_copy.argtypes = [P, P, P]
+    L.pned_vec_data.argtypes = [P]
+    L.pned_vec_data.restype = P
+    L.pned_set_initial_vec.argtypes = [P, P, u64]
+    L.pned_p_mltply_v3.argtypes = [P, P, P, f64, _dp]
+    L.pned_hv_fresh.argtypes = [P, P, P, f64, _dp]
+    L.pned_calc_vec_inprdct.argtypes = [P, P, P, _dp]
+    L.pned_calc_beta.argtypes = [P, _dp, P, P, _dp]
+    L.pned_calc_alpha_beta.argtypes = [P, P, P, _dp, _dp]
+    L.pned_calc_alpha_beta2.argtypes = [P, P, P, _dp, _dp]
+    L.pned_axpby.argtypes = [P, _dp, P, _dp, P]
+    L.pned_scale.argtypes = [P, P, _dp]
+    L.pned_lanczos_update.argtypes = [P, P, P, _dp, _dp]
+    L.pned_lnc1.argtypes = [P, P, P, i32, i32, i32, _dp, _dp, _dp, C.POINTER(i32)]
+    L.pned_bisec.argtypes = [_dp, _dp, i32, _dp, i32, f64]
+    L.pned_vec12.argtypes = [_dp, _dp, i32, _dp, i32, _dp]
+    L.pned_ritz_vector.argtypes = [P, P, P, P, i32, _dp, _dp, _dp]
+    L.pned_inv1.argtypes = [P, f64, P, P, P, P, P, i32, i32, i32, C.POINTER(i32)]
+    L.pned_cocg.argtypes = [P, f64, f64, P, P, P, P, P, i32, f64, C.POINTER(i32), C.POINTER(f64)]
+    L.pned_check1.argtypes = [P, P, P, _dp, _dp, i32, C.POINTER(i32)]
+    L.pned_residual.argtypes = [P, P, f64, P, C.POINTER(f64)]
+    L.pned_tps_set_observables.argtypes = [P, _dp, i32, _dp]
+    L.pned_tps.argtypes = [P, P, P, P, P, f64, i32, _dp, _dp, _dp]
+    L.pned_intalv_zz.argtypes = [P, P, P, _dp, _dp, C.POINTER(f64)]
+    L.pned_dsflnc1z.argtypes = [P, P, P, f64, i32, _dp, _dp]
+    L.pned_renbun.argtypes = [P, i32, i32, f64, f64, f64, _dp, _dp, f64, _dp, _dp, _ip]
+    L.pned_s12_plan_export.argtypes = [i32, i32, _ip, _ip, _dp, _ip, i32, i32, i32, C.POINTER(i32),
+                                       C.POINTER(i32), C.POINTER(i32), _ip, _ip, _dp,
+                                       np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), _dp, i32, i32]
+    _lib = L
+    return L
+
+
+def check(rc: int):
+    if rc != 0:
+        raise PnedError(rc, lib().pned_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    return int(lib().pned_device_count())
+
+
+class Vector:
+    """A device vector of ``dim`` states x ``nvec`` interleaved columns."""
+
+    def __init__(self, ctx: "Context", nvec: int = 1, dtype: int = C128):
+        self.ctx = ctx
+        self.nvec = nvec
+        self.dtype = dtype
+        h = C.c_void_p()
+        check(lib().pned_vec_create(ctx.h, nvec, dtype, C.byref(h)))
+        self.h = h
+
+    @property
+    def np_dtype(self):
+        return np.complex128 if self.dtype == C128 else np.complex64
+
+    def upload(self, host: np.ndarray, offset: int = 0):
+        a = np.ascontiguousarray(host, dtype=self.np_dtype)
+        count = a.size // self.nvec
+        check(lib().pned_vec_upload(self.ctx.h, self.h, a.ctypes.data, offset, count))
+
+    def download(self, offset: int = 0, count: Optional[int] = None) -> np.ndarray:
+        if count is None:
+            count = self.ctx.dim - offset
+        out = np.empty(count * self.nvec, dtype=self.np_dtype)
+        check(lib().pned_vec_download(self.ctx.h, self.h, out.ctypes.data, offset, count))
+        return out.reshape(count, self.nvec) if self.nvec > 1 else out
+
+    def zero(self):
+        check(lib().pned_vec_zero(self.ctx.h, self.h))
+
+    def copy_from(self, other: "Vector"):
+        check(lib().pned_vec_copy(self.ctx.h, other.h, self.h))
+
+    def data_ptr(self) -> int:
+        return lib().pned_vec_data(self.h)
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().pned_vec_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One operator on one GPU: the reference's data_mem tables handed to the device."""
+
+    def __init__(self, op, device: int = 0):
+        self.op = op
+        self.n, self.q, self.ibond = op.n, op.q, op.ibond
+        h = C.c_void_p()
+        check(lib().pned_ctx_create(op.n, op.q, op.ibond, np.ascontiguousarray(op.ipair, dtype=np.int32),
+                                    np.ascontiguousarray(op.ibond_type, dtype=np.int32),
+                                    np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
+                                    op.fortran_llocal(), device, C.byref(h)))
+        self.h = h
+        self.dim = int(lib().pned_ctx_dim(h))
+
+    # -- management -------------------------------------------------------------------
+    def vector(self, nvec: int = 1, dtype: int = C128) -> Vector:
+        return Vector(self, nvec, dtype)
+
+    def set_matrices(self, op):
+        self.op = op
+        check(lib().pned_ctx_set_matrices(self.h, np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
+                                          op.fortran_llocal()))
+
+    def tune(self, tile_bits=0, low_bits=0, threads=0):
+        check(lib().pned_ctx_tune(self.h, tile_bits, low_bits, threads))
+
+    def autotune(self, v1: "Vector", v0: "Vector", reps: int = 2):
+        """Pick the fastest tile plan on these vectors (v0 is overwritten)."""
+        k, l, t, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(lib().pned_ctx_autotune(self.h, v1.h, v0.h, reps, C.byref(k), C.byref(l), C.byref(t), C.byref(ms)))
+        return {"tile_bits": k.value, "low_bits": l.value, "threads": t.value, "ms": ms.value}
+
+    def plan_info(self):
+        n = C.c_int()
+        t = np.zeros(64, dtype=np.int32)
+        check(lib().pned_ctx_plan_info(self.h, C.byref(n), t, 64))
+        return n.value, list(t[: min(n.value, 64)])
+
+    def sync(self):
+        check(lib().pned_ctx_sync(self.h))
+
+    def stream(self) -> int:
+        return lib().pned_ctx_stream(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pned_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- parallelized_diag ------------------------------------------------------------
+    def set_initial_vec(self, v: Vector, seed: int):
+        check(lib().pned_set_initial_vec(self.h, v.h, seed))
+
+    def p_mltply_v3(self, v1: Vector, v0: Vector, eperbd: float = 0.0):
+        pr = np.zeros(2 * v1.nvec)
+        check(lib().pned_p_mltply_v3(self.h, v1.h, v0.h, eperbd, pr))
+        z = pr[0::2] + 1j * pr[1::2]
+        return z[0] if v1.nvec == 1 else z
+
+    def hv_fresh(self, v1: Vector, v0: Vector, eperbd: float = 0.0):
+        pr = np.zeros(2 * v1.nvec)
+        check(lib().pned_hv_fresh(self.h, v1.h, v0.h, eperbd, pr))
+        z = pr[0::2] + 1j * pr[1::2]
+        return z[0] if v1.nvec == 1 else z
+
+    def calc_vec_inprdct(self, x: Vector, y: Vector):
+        out = np.zeros(2 * x.nvec)
+        check(lib().pned_calc_vec_inprdct(self.h, x.h, y.h, out))
+        z = out[0::2] + 1j * out[1::2]
+        return z[0] if x.nvec == 1 else z
+
+    def calc_alpha_beta(self, v1: Vector, v0: Vector):
+        a = np.zeros(v1.nvec)
+        b = np.zeros(v1.nvec)
+        check(lib().pned_calc_alpha_beta(self.h, v1.h, v0.h, a, b))
+        return (a[0], b[0]) if v1.nvec == 1 else (a, b)
+
+    def calc_alpha_beta2(self, v1: Vector, v0: Vector):
+        a = np.zeros(v1.nvec)
+        b = np.zeros(v1.nvec)
+        check(lib().pned_calc_alpha_beta2(self.h, v1.h, v0.h, a, b))
+        return (a[0], b[0]) if v1.nvec == 1 else (a, b)
+
+    def axpby(self, a, x: Vector, b, y: Vector):
+        a = np.broadcast_to(np.asarray(a, dtype=complex), (x.nvec,))
+        b = np.broadcast_to(np.asarray(b, dtype=complex), (x.nvec,))
+        check(lib().pned_axpby(self.h, np.ascontiguousarray(a.view(np.float64)), x.h,
+                               np.ascontiguousarray(b.view(np.float64)), y.h))
+
+    def scale(self, x: Vector, a):
+        a = np.broadcast_to(np.asarray(a, dtype=complex), (x.nvec,))
+        check(lib().pned_scale(self.h, x.h, np.ascontiguousarray(a.view(np.float64))))
+
+    def lanczos_update(self, v1: Vector, v0: Vector, alpha, beta):
+        check(lib().pned_lanczos_update(self.h, v1.h, v0.h, np.atleast_1d(np.asarray(alpha, float)),
+                                        np.atleast_1d(np.asarray(beta, float))))
+
+    # -- Krylov -----------------------------------------------------------------------
+    def lnc1(self, v1: Vector, v0: Vector, max_itr=1024, nvec=8, fixed_steps=0, allow_noconv=False):
+        steps = fixed_steps if fixed_steps > 0 else max_itr
+        alpha = np.zeros(steps + 1)
+        beta = np.zeros(steps + 1)
+        ene = np.zeros(max(nvec, 1))
+        itr = C.c_int()
+        rc = lib().pned_lnc1(self.h, v1.h, v0.h, max_itr, nvec, fixed_steps, alpha, beta, ene, C.byref(itr))
+        if rc == E_NOCONV and allow_noconv:
+            rc = 0
+        check(rc)
+        return itr.value, alpha[: itr.value], beta[: itr.value], ene
+
+    def ritz_vector(self, v1: Vector, v0: Vector, x: Vector, alpha, beta, coef):
+        n = len(coef)
+        check(lib().pned_ritz_vector(self.h, v1.h, v0.h, x.h, n, np.ascontiguousarray(alpha, dtype=float),
+                                     np.ascontiguousarray(beta, dtype=float), np.ascontiguousarray(coef, dtype=float)))
+
+    def inv1(self, eig: float, x: Vector, b: Vector, p: Vector, r: Vector, y: Vector, max_inv=5000, max_cg=500,
+             reference_saved_loop=True):
+        it = C.c_int()
+        check(lib().pned_inv1(self.h, eig, x.h, b.h, p.h, r.h, y.h, max_inv, max_cg,
+                              1 if reference_saved_loop else 0, C.byref(it)))
+        return it.value
+
+    def cocg(self, e0: float, eta: float, b: Vector, x: Vector, r: Vector, p: Vector, y: Vector,
+             max_it=5000, tol=1e-10):
+        it = C.c_int()
+        res = C.c_double()
+        check(lib().pned_cocg(self.h, e0, eta, b.h, x.h, r.h, p.h, y.h, max_it, tol, C.byref(it), C.byref(res)))
+        return it.value, res.value
+
+    def check1(self, x: Vector, work: Vector, max_ratios=32):
+        ex = np.zeros(2)
+        ra = np.zeros(2 * max_ratios)
+        k = C.c_int()
+        check(lib().pned_check1(self.h, x.h, work.h, ex, ra, max_ratios, C.byref(k)))
+        return complex(ex[0], ex[1]), (ra[0::2] + 1j * ra[1::2])[: k.value]
+
+    def residual(self, x: Vector, e: float, work: Vector) -> float:
+        r = C.c_double()
+        check(lib().pned_residual(self.h, x.h, e, work.h, C.byref(r)))
+        return r.value
+
+    def tps_set_observables(self, site_vec, sq_q):
+        sq = np.ascontiguousarray(np.asarray(sq_q, dtype=float).reshape(-1))
+        check(lib().pned_tps_set_observables(self.h, np.ascontiguousarray(np.asarray(site_vec, float).reshape(-1)),
+                                             len(sq) // 2, sq if len(sq) else np.zeros(2)))
+        self._nsq = len(sq) // 2
+
+    def tps(self, v1: Vector, v0: Vector, vg: Vector, vt: Vector, lmax: float, nsteps: int):
+        nc = v1.nvec
+        ncoef = 7 + getattr(self, "_nsq", 0)
+        alpha = np.zeros(nsteps * nc)
+        beta = np.zeros(nsteps * nc)
+        coef = np.zeros(nsteps * nc * ncoef)
+        check(lib().pned_tps(self.h, v1.h, v0.h, vg.h, vt.h, lmax, nsteps, alpha, beta, coef))
+        return alpha.reshape(nsteps, nc), beta.reshape(nsteps, nc), coef.reshape(nsteps, nc, ncoef)
+
+    def intalv_zz(self, x: Vector, v: Vector, q, site_vec) -> float:
+        cn = C.c_double()
+        check(lib().pned_intalv_zz(self.h, x.h, v.h, np.asarray(q, dtype=float),
+                                   np.ascontiguousarray(np.asarray(site_vec, float).reshape(-1)), C.byref(cn)))
+        return cn.value
+
+    def dsflnc1z(self, v1: Vector, v0: Vector, cnorm: float, s_maxitr: int):
+        sa = np.zeros(s_maxitr)
+        sb = np.zeros(s_maxitr + 1)
+        check(lib().pned_dsflnc1z(self.h, v1.h, v0.h, cnorm, s_maxitr, sa, sb))
+        return sa, sb
+
+    def renbun(self, salpha, sbeta, ene, eta, i_range=2000, split=2.5e-3, s_maxitr=None, sq=0.0):
+        if s_maxitr is None:
+            s_maxitr = len(salpha)
+        sa = np.zeros(max(s_maxitr, len(salpha)))
+        sa[: len(salpha)] = salpha
+        sb = np.zeros(len(sa) + 1)
+        sb[: len(sbeta)] = sbeta
+        w = np.zeros(i_range)
+        dsf = np.zeros(i_range)
+        conv = np.zeros(i_range, dtype=np.int32)
+        check(lib().pned_renbun(self.h, i_range, s_maxitr, split, ene, eta, sa, sb, sq, w, dsf, conv))
+        return w, dsf, conv
+
+
+def s12_plan(op, colbits=0, tile_bits=12, low_bits=4):
+    """The planner's decomposition of an S=1/2 operator (no GPU needed)."""
+    mp, mt = 64, 4096
+    npass, nterm, ngroup = C.c_int(), C.c_int(), C.c_int()
+    pinfo = np.zeros(4 * mp, dtype=np.int32)
+    pbits = np.zeros(40 * mp, dtype=np.int32)
+    terms = np.zeros(6 * mt)
+    gmask = np.zeros(mt, dtype=np.uint32)
+    diag = np.zeros(5 * 64 + 2)
+    check(lib().pned_s12_plan_export(op.n, op.ibond, np.ascontiguousarray(op.ipair, dtype=np.int32),
+                                     np.ascontiguousarray(op.ibond_type, dtype=np.int32),
+                                     np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
+                                     op.fortran_llocal(), colbits, tile_bits, low_bits, C.byref(npass),
+                                     C.byref(nterm), C.byref(ngroup), pinfo, pbits, terms, gmask, diag, mp, mt))
+    P, T, G = npass.value, nterm.value, ngroup.value
+    passes = []
+    for p in range(P):
+        K, ft, nt, hd = pinfo[4 * p: 4 * p + 4]
+        passes.append({"K": int(K), "first": int(ft), "n": int(nt), "diag": bool(hd),
+                       "tb": [int(b) for b in pbits[40 * p: 40 * p + K]]})
+    tt = terms[: 6 * T].reshape(T, 6)
+    return {"passes": passes, "tmask": tt[:, 0].astype(np.int64), "selmask": tt[:, 1].astype(np.int64),
+            "c0": tt[:, 2] + 1j * tt[:, 3], "c1": tt[:, 4] + 1j * tt[:, 5], "gmask": gmask[:T].astype(np.int64),
+            "diag": diag[: 5 * G].reshape(G, 5), "c0diag": diag[5 * G] + 1j * diag[5 * G + 1]}
+
+
+def bisec(alpha, beta, ne, eps=1e-25):
+    a = np.ascontiguousarray(alpha, dtype=float)
+    b = np.ascontiguousarray(beta, dtype=float)
+    E = np.zeros(max(ne, 1))
+    check(lib().pned_bisec(a, b, len(a), E, ne, eps))
+    return E[:ne]
+
+
+def vec12(alpha, beta, E, nvec):
+    a = np.ascontiguousarray(alpha, dtype=float)
+    b = np.ascontiguousarray(beta, dtype=float)
+    v = np.zeros(len(a) * nvec)
+    check(lib().pned_vec12(a, b, len(a), np.ascontiguousarray(E, dtype=float), nvec, v))
+    return v.reshape(nvec, len(a))

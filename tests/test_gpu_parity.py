@@ -1,0 +1,233 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the same inputs.
+
+Tolerances (stated by the hot-path contract): H.v <= 1e-12 relative (normwise) for
+complex128, energies <= 1e-10 relative; complex64 storage is checked against its own
+rounding bound (~1e-6 relative).  Every case prints its name first so a failing GPU
+step is identifiable from the log.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from pned_amd import gpu, lattice, model
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if gpu.device_count() < 1:
+        pytest.skip("no GPU")
+
+
+def _rand(d, seed, nvec=1):
+    r = np.random.default_rng(seed)
+    z = r.standard_normal(d * nvec) + 1j * r.standard_normal(d * nvec)
+    return z if nvec == 1 else z.reshape(d, nvec)
+
+
+HV_CASES = [
+    (8, 54.0, 0.0, 2), (12, 126.0, 0.0, 2), (12, 126.0, 0.1, 2), (16, 200.0, 0.15, 2),
+    ("2x4", 252.0, 0.3, 2), (18, 33.0, 0.0, 2), (24, 54.0, 0.0, 2), ("3x4", 126.0, 0.1, 2),
+    (8, 108.0, 0.0, 3), (8, 30.0, 0.2, 3), (12, 108.0, 0.0, 3),
+]
+
+
+@pytest.mark.parametrize("spec,phi,h,q", HV_CASES)
+def test_hv_parity(spec, phi, h, q):
+    _say("hv", spec, phi, h, q)
+    cl = lattice.get_cluster(spec)
+    op = model.build_operator(cl, 20, phi, q=q, field=h)
+    ctx = gpu.Context(op)
+    d = ctx.dim
+    x = _rand(d, 1)
+    y = _rand(d, 2)
+    v1, v0 = ctx.vector(), ctx.vector()
+    v1.upload(x)
+    v0.upload(y)
+    ep = 0.07
+    pr = ctx.p_mltply_v3(v1, v0, ep)
+    got = v0.download()
+    want = y.copy()
+    pr_ref = oracle.hv(oracle.OracleOp(op), x, want, eperbd=ep)
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+    # <x|H|x> of an unnormalised vector cancels heavily: bound by |x| |Hx|
+    hx = want - y
+    scale = np.linalg.norm(x) * np.linalg.norm(hx)
+    assert abs(pr - pr_ref) <= 1e-13 * scale
+    # fresh-output variant
+    pr2 = ctx.hv_fresh(v1, v0, ep)
+    want2 = np.zeros_like(x)
+    oracle.hv(oracle.OracleOp(op), x, want2, eperbd=ep)
+    got2 = v0.download()
+    assert np.linalg.norm(got2 - want2) <= 1e-12 * np.linalg.norm(want2)
+    assert abs(pr2 - pr_ref) <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("tile_bits", [8, 10, 13])
+def test_hv_tile_shapes(tile_bits):
+    _say("hv tile", tile_bits)
+    op = model.build_operator(lattice.reference_cluster(16), 20, 77.0, field=0.2)
+    ctx = gpu.Context(op)
+    ctx.tune(tile_bits=tile_bits, low_bits=3)
+    x = _rand(ctx.dim, 3)
+    v1, v0 = ctx.vector(), ctx.vector()
+    v1.upload(x)
+    ctx.hv_fresh(v1, v0)
+    want = np.zeros_like(x)
+    oracle.hv(oracle.OracleOp(op), x, want)
+    assert np.linalg.norm(v0.download() - want) <= 1e-12 * np.linalg.norm(want)
+
+
+@pytest.mark.parametrize("nvec,dtype", [(4, gpu.C128), (8, gpu.C64), (1, gpu.C64)])
+def test_hv_batched_and_c64(nvec, dtype):
+    _say("hv batched", nvec, dtype)
+    op = model.build_operator(lattice.reference_cluster(16), 20, 126.0, field=0.1)
+    ctx = gpu.Context(op)
+    X = _rand(ctx.dim, 5, nvec)
+    if nvec == 1:
+        X = X.reshape(-1, 1)
+    v1, v0 = ctx.vector(nvec, dtype), ctx.vector(nvec, dtype)
+    v1.upload(X)
+    pr = np.atleast_1d(ctx.hv_fresh(v1, v0))
+    got = v0.download().reshape(ctx.dim, nvec)
+    tol = 1e-12 if dtype == gpu.C128 else 2e-6
+    xr = X.astype(np.complex64).astype(np.complex128) if dtype == gpu.C64 else X
+    for c in range(nvec):
+        want = np.zeros(ctx.dim, complex)
+        p_ref = oracle.hv(oracle.OracleOp(op), np.ascontiguousarray(xr[:, c]), want)
+        assert np.linalg.norm(got[:, c] - want) <= tol * np.linalg.norm(want), c
+        scale = np.linalg.norm(xr[:, c]) * np.linalg.norm(want)
+        assert abs(pr[c] - p_ref) <= 0.1 * tol * scale
+
+
+def test_blas1_and_gaussian():
+    _say("blas1")
+    op = model.build_operator(lattice.reference_cluster(12), 20, 54.0)
+    ctx = gpu.Context(op)
+    d = ctx.dim
+    v = ctx.vector()
+    ctx.set_initial_vec(v, 20261202)
+    ref = oracle.gaussian(d, 20261202)
+    assert np.linalg.norm(v.download() - ref) <= 1e-14
+    x, y = _rand(d, 7), _rand(d, 8)
+    a, b = ctx.vector(), ctx.vector()
+    a.upload(x)
+    b.upload(y)
+    assert abs(ctx.calc_vec_inprdct(a, b) - np.vdot(x, y)) <= 1e-13 * abs(np.vdot(x, y))
+    ctx.axpby(0.3 - 0.2j, a, 1.5 + 0.1j, b)
+    assert np.allclose(b.download(), (0.3 - 0.2j) * x + (1.5 + 0.1j) * y, rtol=1e-14, atol=1e-13)
+    b.upload(y)
+    ctx.lanczos_update(a, b, 0.7, 1.9)
+    assert np.allclose(a.download(), (y - 0.7 * x) / 1.9, atol=1e-14)
+    assert np.allclose(b.download(), -1.9 * x, atol=1e-14)
+
+
+def test_gaussian_batched_columns_independent():
+    _say("gaussian batched")
+    op = model.build_operator(lattice.reference_cluster(12), 20, 54.0)
+    ctx = gpu.Context(op)
+    v = ctx.vector(4, gpu.C128)
+    ctx.set_initial_vec(v, 99)
+    X = v.download()
+    for c in range(4):
+        ref = oracle.gaussian(ctx.dim, 99, col=c)
+        assert np.linalg.norm(X[:, c] - ref) <= 1e-14
+
+
+def test_lanczos_parity_n16():
+    _say("lanczos n16")
+    op = model.build_operator(lattice.reference_cluster(16), 20, 126.0, field=0.1)
+    ctx = gpu.Context(op)
+    v1, v0 = ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 20261202)
+    x = v1.download()
+    itr, alpha, beta, ene = ctx.lnc1(v1, v0, max_itr=1024, nvec=8)
+    i2, a2, b2, e2 = oracle.lanczos(oracle.OracleOp(op), x, max_itr=1024, nvec=8)
+    assert abs(ene[0] - e2[0]) <= 1e-10 * abs(e2[0])
+    assert np.allclose(alpha[:20], a2[:20], rtol=1e-12, atol=0)
+    assert np.allclose(beta[:20], b2[:20], rtol=1e-12, atol=0)
+
+
+def test_inverse_iteration_and_check1_n12():
+    _say("inv1 n12")
+    op = model.build_operator(lattice.reference_cluster(12), 20, 54.0)
+    ctx = gpu.Context(op)
+    v1, v0 = ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 5)
+    itr, alpha, beta, ene = ctx.lnc1(v1, v0, nvec=8)
+    x, b, p, r, y = (ctx.vector() for _ in range(5))
+    ctx.set_initial_vec(b, 6)
+    b0 = b.download()
+    it = ctx.inv1(ene[0], x, b, p, r, y, max_inv=50)
+    assert it > 0
+    # same start vector and eigenvalue through the oracle's inv1z/cg1 restatement
+    it_ref, x_ref = oracle.inverse_iteration(oracle.OracleOp(op), ene[0], b0, max_inv=50)
+    assert it == it_ref
+    assert abs(abs(np.vdot(x_ref, x.download())) - 1.0) < 1e-10
+    # the reference's stopping rule |<x|b>|-1 < 1e-12 leaves a residual ~1e-7
+    res = ctx.residual(x, ene[0], y)
+    assert res < 1e-6
+    ex, ratios = ctx.check1(x, y)
+    assert abs(ex.real - ene[0]) < 1e-10 * abs(ene[0])
+    # check1 logs (Hx)_j/x_j at j = min(ldim/3,13) : ldim : max(1,ldim/20), 1-based
+    xv = x.download()
+    hx = np.zeros_like(xv)
+    oracle.hv(oracle.OracleOp(op), xv, hx)
+    d = ctx.dim
+    js = list(range(min(d // 3, 13), d + 1, max(1, d // 20)))
+    assert len(ratios) == len(js)
+    assert np.allclose(ratios, hx[np.array(js) - 1] / xv[np.array(js) - 1], rtol=1e-10)
+
+
+def test_tps_parity_n12_batched():
+    _say("tps n12")
+    cl = lattice.reference_cluster(12)
+    op = model.build_operator(cl, 20, 90.0, tps=True)
+    ctx = gpu.Context(op)
+    ctx.tps_set_observables(cl.site_vec, cl.sq_momenta())
+    nvec = 4
+    v1, v0, vg, vt = (ctx.vector(nvec, gpu.C128) for _ in range(4))
+    ctx.set_initial_vec(v1, 11)
+    X = v1.download()
+    steps = 15
+    al, be, co = ctx.tps(v1, v0, vg, vt, op.shift_L, steps)
+    for c in range(nvec):
+        a_ref, b_ref, c_ref = oracle.tps(oracle.OracleOp(op), cl.site_vec, cl.sq_momenta(), op.shift_L, steps,
+                                         np.ascontiguousarray(X[:, c]))
+        assert np.allclose(al[:, c], a_ref, rtol=1e-11)
+        assert np.allclose(be[:, c], b_ref, rtol=1e-11)
+        assert np.allclose(co[:, c, :], c_ref, rtol=1e-9, atol=1e-12)
+
+
+def test_dsf_parity_n12():
+    _say("dsf n12")
+    cl = lattice.reference_cluster(12)
+    op = model.build_operator(cl, 20, 54.0)
+    ctx = gpu.Context(op)
+    v1, v0 = ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 3)
+    x = v1.download()
+    q = cl.q_list[3]
+    w = ctx.vector()
+    cn = ctx.intalv_zz(v1, w, q, cl.site_vec)
+    vx = oracle.sz_q(cl.n, 2, cl.site_vec, q, x)
+    cn_ref = np.vdot(vx, vx).real
+    assert abs(cn - cn_ref / cl.n) <= 1e-13 * cn_ref
+    assert np.linalg.norm(w.download() - vx / np.sqrt(cn_ref)) <= 1e-13
+    sa, sb = ctx.dsflnc1z(w, v0, cn, 40)
+    sa_ref, sb_ref = oracle.dsf_lanczos(oracle.OracleOp(op), vx / np.sqrt(cn_ref), cn, 40)
+    assert np.allclose(sa[:20], sa_ref[:20], rtol=1e-11)
+    assert np.allclose(sb[:20], sb_ref[:20], rtol=1e-11)
+    ww, dsf, conv = ctx.renbun(sa, sb, -5.0, 0.01, 2000, 2.5e-3, 40, sq=cn)
+    w2, d2, c2 = oracle.renbun(sa, sb, -5.0, 0.01, 2000, 2.5e-3, 40)
+    d2 = oracle.normalize_dsf(d2, 2.5e-3, cn)
+    assert np.allclose(dsf, d2, rtol=1e-10, atol=1e-14)
+    assert np.array_equal(conv, c2)
