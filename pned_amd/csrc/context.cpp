@@ -327,6 +327,7 @@ int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
     PNED_HIP(hipMemcpy(ctx->d_diag, &dh, sizeof(DiagDev), hipMemcpyHostToDevice));
     ctx->plan_colbits = colbits;
     ctx->plan_dtype = dtype;
+    ctx->pd_threads = -1;    // address tables follow the new plan
     return PNED_OK;
 }
 
@@ -416,6 +417,7 @@ int pned_ctx_destroy(pned_ctx *ctx)
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->d_terms) (void)hipFree(ctx->d_terms);
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+    if (ctx->pd_dev) (void)hipFree(ctx->pd_dev);
     if (ctx->d_partial) (void)hipFree(ctx->d_partial);
     if (ctx->d_result) (void)hipFree(ctx->d_result);
     if (ctx->h_result) (void)hipHostFree(ctx->h_result);

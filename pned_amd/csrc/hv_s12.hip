@@ -14,6 +14,9 @@
 // Memory index m = (state << colbits) | column: batched vectors ([dim][nvec] interleaved)
 // add never-flipped column bits at the bottom, which the planner keeps in the tile so
 // the 16-byte lanes of a wave form contiguous runs.
+#include <cstring>
+#include <utility>
+
 #include "internal.h"
 
 namespace pned {
@@ -27,9 +30,20 @@ struct PassArgs {
     int diag;              // add the diagonal (first pass)
     int fresh;             // first pass: v0 input is zero
     int ncol;              // columns (1 << colbits)
+    int nlo;               // outer bits resolved by the first base table
     double shift;          // -ibond * eperbd, added to the diagonal
-    uint8_t tb[40];        // tile bit -> memory bit
-    uint8_t ob[40];        // outer bit -> memory bit
+};
+
+// Per-pass address tables in device memory (read with scalar loads):
+//   lo[t]  memory-index offset of thread t's slot (tile bits 0..LT-1)
+//   hi[r]  offset of register slot r (tile bits LT..K-1)
+//   blo/bhi  tile base = blo[tix & (2^nlo - 1)] | bhi[tix >> nlo]  (outer bits)
+constexpr int PD_LO = 512, PD_HI = 32, PD_B = 2048;
+struct PassDev {
+    uint64_t lo[PD_LO];
+    uint64_t hi[PD_HI];
+    uint64_t blo[PD_B];
+    uint64_t bhi[PD_B];
 };
 
 template <typename R> struct C2;
@@ -49,107 +63,128 @@ __device__ __forceinline__ double diag_value_re(const DiagDev &d, uint32_t s, do
     return re;
 }
 
-// Direct HBM -> LDS load of one wave's 64 consecutive tile slots (gfx950
-// global_load_lds_dwordx4 / _dwordx2): lane l writes slot (lds_wave_base + l).
-__device__ __forceinline__ void load_to_lds(const double2 *g, double2 *lds_wave_base)
+template <int... Is, typename F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, Is...>, F &&f)
 {
-    __builtin_amdgcn_global_load_lds((const void *)g, (__attribute__((address_space(3))) void *)lds_wave_base, 16, 0, 0);
+    (f(std::integral_constant<int, Is>{}), ...);
 }
-// no 8-byte form of the LDS-direct load exists: complex64 goes through a register
-__device__ __forceinline__ void load_to_lds(const float2 *g, float2 *lds_wave_base)
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
 {
-    lds_wave_base[threadIdx.x & 63] = *g;
+    static_for_impl(std::make_integer_sequence<int, N>{}, f);
 }
 
 // TPB threads, R states per thread: tile of K = log2(TPB*R) bits.  Tile slot
 // j = tid + TPB*r holds the state with memory index base | offLo(tid) | offHi(r).
-// Per tile: (1) every wave streams its R slices of v1 straight into LDS; (2) states are
-// processed CH at a time, the v0 values of the next chunk being fetched while the current
-// one computes; (3) outputs are written in place.
+// Software pipeline over the tiles a block owns: while tile t is computed from LDS, the
+// v1 values of tile t+1 are in flight into registers (written to LDS after the barrier
+// that ends tile t), and the v0 values being accumulated arrive one chunk of CH states
+// ahead of their use, across tile boundaries.  HBM latency is hidden inside one
+// workgroup, which matters when a 128 KB tile leaves room for one workgroup per CU.
 template <typename Real, int TPB, int R>
 __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *__restrict__ v1,
                                                    typename C2<Real>::T *__restrict__ v0,
                                                    const DevTerm *__restrict__ terms,
-                                                   const DiagDev *__restrict__ diagp, PassArgs a,
+                                                   const DiagDev *__restrict__ diagp,
+                                                   const PassDev *__restrict__ pd, PassArgs a,
                                                    double *__restrict__ partial)
 {
     using V = typename C2<Real>::T;
     extern __shared__ unsigned char smem_raw[];
     V *tile = reinterpret_cast<V *>(smem_raw);
-    constexpr int LT = __builtin_ctz(TPB);
-    constexpr int LR = __builtin_ctz(R);
     constexpr int CH = (R < 4) ? R : 4;      // states per compute chunk
+    constexpr int NCH = R / CH;
     const int tid = threadIdx.x;
-    const int wave_base = tid & ~63;
 
-    uint64_t offLo = 0;
-#pragma unroll
-    for (int k = 0; k < LT; ++k)
-        if ((tid >> k) & 1) offLo |= 1ull << a.tb[k];
-    uint64_t offHi[R];
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        uint64_t o = 0;
-#pragma unroll
-        for (int k = 0; k < LR; ++k)
-            if ((r >> k) & 1) o |= 1ull << a.tb[LT + k];
-        offHi[r] = o;
-    }
+    const uint64_t offLo = pd->lo[tid];
+    auto offHi = [&](int r) { return pd->hi[r]; };
     const bool rd0 = !(a.diag && a.fresh);
+    const uint64_t lomask = (1ull << a.nlo) - 1;
+    auto tile_base = [&](uint64_t tix) { return pd->blo[tix & lomask] | pd->bhi[tix >> a.nlo] | offLo; };
 
     double pr_re = 0.0, pr_im = 0.0;
-    for (uint64_t tix = blockIdx.x; tix < a.tile_count; tix += gridDim.x) {
-        uint64_t base = 0;
-        for (int k = 0; k < a.nouter; ++k)
-            if ((tix >> k) & 1) base |= 1ull << a.ob[k];
-        const uint64_t tb0 = base | offLo;
-#pragma unroll
-        for (int r = 0; r < R; ++r) load_to_lds(v1 + (tb0 | offHi[r]), tile + wave_base + TPB * r);
-        V y[CH];
-        if (rd0) {
-#pragma unroll
-            for (int c = 0; c < CH; ++c) y[c] = v0[tb0 | offHi[c]];
-        }
-        __builtin_amdgcn_s_waitcnt(0);
+    // prefetch registers kept as scalar components (a V[] array is not promoted to
+    // registers for the 16-byte type)
+    Real xr_[R], xi_[R];
+    Real ycr[CH], yci[CH];
+    auto fetch_x = [&](uint64_t b) {
+        static_for<R>([&](auto I) {
+            const V t = v1[b | offHi(I)];
+            xr_[I] = t.x;
+            xi_[I] = t.y;
+        });
+    };
+    uint64_t tix = blockIdx.x;
+    if (tix < a.tile_count) {
+        const uint64_t tb0 = tile_base(tix);
+        fetch_x(tb0);
+        if (rd0) static_for<CH>([&](auto I) { const V t = v0[tb0 | offHi(I)]; ycr[I] = t.x; yci[I] = t.y; });
+    }
+    for (; tix < a.tile_count; tix += gridDim.x) {
+        const uint64_t tb0 = tile_base(tix);
+        static_for<R>([&](auto I) {
+            V t;
+            t.x = xr_[I];
+            t.y = xi_[I];
+            tile[tid + TPB * I] = t;
+        });
         __syncthreads();
+        const uint64_t nx = tix + gridDim.x;
+        const bool has_next = nx < a.tile_count;
+        const uint64_t nb = has_next ? tile_base(nx) : 0;
+        if (has_next) fetch_x(nb);
 
-#pragma unroll
-        for (int r0 = 0; r0 < R; r0 += CH) {
+        static_for<NCH>([&](auto CI) {
+            constexpr int r0 = CI * CH;
+            // v0 of the next chunk (or of the next tile's first chunk)
+            Real ynr[CH], yni[CH];
+            if (rd0) {
+                if constexpr (r0 + CH < R) {
+                    static_for<CH>([&](auto I) {
+                        const V t = v0[tb0 | offHi(r0 + CH + I)];
+                        ynr[I] = t.x;
+                        yni[I] = t.y;
+                    });
+                } else {
+                    if (has_next)
+                        static_for<CH>([&](auto I) {
+                            const V t = v0[nb | offHi(I)];
+                            ynr[I] = t.x;
+                            yni[I] = t.y;
+                        });
+                }
+            }
             uint32_t st[CH];
             double ar[CH], ai[CH];
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                st[c] = (uint32_t)((tb0 | offHi[r0 + c]) >> a.colbits);
+            static_for<CH>([&](auto c) {
+                st[c] = (uint32_t)((tb0 | offHi(r0 + c)) >> a.colbits);
                 ar[c] = 0.0;
                 ai[c] = 0.0;
-            }
+            });
             if (a.diag) {
                 const DiagDev &dg = *diagp;
-#pragma unroll
-                for (int c = 0; c < CH; ++c) {
+                static_for<CH>([&](auto c) {
                     double di;
                     const double dr = diag_value_re(dg, st[c], di) + a.shift;
                     const V x = tile[tid + TPB * (r0 + c)];
                     ar[c] = dr * (double)x.x - di * (double)x.y;
                     ai[c] = dr * (double)x.y + di * (double)x.x;
-                }
+                });
             }
             for (int t = 0; t < a.nterm; ++t) {
                 const DevTerm T = terms[a.first_term + t];
                 if (T.is_real) {
                     const double d = T.c1r - T.c0r;
-#pragma unroll
-                    for (int c = 0; c < CH; ++c) {
+                    static_for<CH>([&](auto c) {
                         const V p = tile[(tid + TPB * (r0 + c)) ^ T.tmask];
                         const double sel = (double)(__popc(st[c] & T.selmask) & 1);
                         const double cf = fma(sel, d, T.c0r);
                         ar[c] = fma(cf, (double)p.x, ar[c]);
                         ai[c] = fma(cf, (double)p.y, ai[c]);
-                    }
+                    });
                 } else {
                     const double dr = T.c1r - T.c0r, di = T.c1i - T.c0i;
-#pragma unroll
-                    for (int c = 0; c < CH; ++c) {
+                    static_for<CH>([&](auto c) {
                         const V p = tile[(tid + TPB * (r0 + c)) ^ T.tmask];
                         const double sel = (double)(__popc(st[c] & T.selmask) & 1);
                         const double cr = fma(sel, dr, T.c0r), ci = fma(sel, di, T.c0i);
@@ -157,35 +192,25 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
                         ar[c] = fma(-ci, (double)p.y, ar[c]);
                         ai[c] = fma(cr, (double)p.y, ai[c]);
                         ai[c] = fma(ci, (double)p.x, ai[c]);
-                    }
+                    });
                 }
             }
-            // fetch the next chunk's v0 before writing this one
-            V yn[CH];
-            if (rd0 && r0 + CH < R) {
-#pragma unroll
-                for (int c = 0; c < CH; ++c) yn[c] = v0[tb0 | offHi[r0 + CH + c]];
-            }
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
+            static_for<CH>([&](auto c) {
                 const V x = tile[tid + TPB * (r0 + c)];
                 pr_re += (double)x.x * ar[c] + (double)x.y * ai[c];
                 pr_im += (double)x.x * ai[c] - (double)x.y * ar[c];
                 V o;
                 if (rd0) {
-                    o.x = (Real)((double)y[c].x + ar[c]);
-                    o.y = (Real)((double)y[c].y + ai[c]);
+                    o.x = (Real)((double)ycr[c] + ar[c]);
+                    o.y = (Real)((double)yci[c] + ai[c]);
                 } else {
                     o.x = (Real)ar[c];
                     o.y = (Real)ai[c];
                 }
-                v0[tb0 | offHi[r0 + c]] = o;
-            }
-            if (rd0 && r0 + CH < R) {
-#pragma unroll
-                for (int c = 0; c < CH; ++c) y[c] = yn[c];
-            }
-        }
+                v0[tb0 | offHi(r0 + c)] = o;
+            });
+            if (rd0) static_for<CH>([&](auto I) { ycr[I] = ynr[I]; yci[I] = yni[I]; });
+        });
         __syncthreads();
     }
 
@@ -227,7 +252,7 @@ __global__ __launch_bounds__(256) void reduce_partials(const double *__restrict_
 }
 
 template <typename Real, int TPB, int R>
-static int launch_pass(pned_ctx *ctx, const void *v1, void *v0, const DiagDev *ddiag,
+static int launch_pass(pned_ctx *ctx, const void *v1, void *v0, const DiagDev *ddiag, const PassDev *pd,
                        const PassArgs &a, double *partial, int grid)
 {
     using V = typename C2<Real>::T;
@@ -237,17 +262,17 @@ static int launch_pass(pned_ctx *ctx, const void *v1, void *v0, const DiagDev *d
     auto kern = hv_s12_pass<Real, TPB, R>;
     PNED_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(TPB), lds, ctx->stream, (const V *)v1, (V *)v0,
-                       ctx->d_terms, ddiag, a, partial);
+                       ctx->d_terms, ddiag, pd, a, partial);
     PNED_HIP(hipGetLastError());
     return PNED_OK;
 }
 
 template <typename Real>
 static int dispatch_pass(pned_ctx *ctx, int TPB, int R, const void *v1, void *v0,
-                         const DiagDev *ddiag, const PassArgs &a, double *partial, int grid)
+                         const DiagDev *ddiag, const PassDev *pd, const PassArgs &a, double *partial, int grid)
 {
 #define PNED_CASE(T_, R_) \
-    if (TPB == T_ && R == R_) return launch_pass<Real, T_, R_>(ctx, v1, v0, ddiag, a, partial, grid);
+    if (TPB == T_ && R == R_) return launch_pass<Real, T_, R_>(ctx, v1, v0, ddiag, pd, a, partial, grid);
     PNED_CASE(256, 32) PNED_CASE(256, 16) PNED_CASE(256, 8) PNED_CASE(256, 4) PNED_CASE(256, 2)
     PNED_CASE(256, 1) PNED_CASE(128, 1) PNED_CASE(64, 1) PNED_CASE(32, 1) PNED_CASE(16, 1)
     PNED_CASE(512, 32) PNED_CASE(512, 16) PNED_CASE(512, 8) PNED_CASE(512, 4) PNED_CASE(512, 2)
@@ -271,6 +296,57 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
     size_t need = (size_t)npass * grid_cap * 2 * nvec + 64;
     PNED_TRY(ensure_partial(ctx, need));
     PNED_TRY(ensure_result(ctx, 2 * (size_t)nvec + 8));
+    // per-pass address tables (rebuilt when the plan or the thread count changes)
+    const int TPBpref = (ctx->tune_threads == 256 || ctx->tune_threads == 512) ? ctx->tune_threads : 512;
+    std::vector<int> tpb(npass), rr(npass);
+    for (int p = 0; p < npass; ++p) {
+        int T = TPBpref;
+        while (T > 16 && T > (1 << pl.passes[p].K)) T >>= 1;
+        tpb[p] = T;
+        rr[p] = (1 << pl.passes[p].K) / T;
+        if (rr[p] > PD_HI) return fail(PNED_E_UNSUPPORTED, "tile too large for the thread count");
+        if ((int)pl.passes[p].ob.size() > 22) return fail(PNED_E_UNSUPPORTED, "too many outer bits");
+    }
+    if (ctx->pd_threads != TPBpref || ctx->pd_dev == nullptr) {
+        std::vector<PassDev> host(npass);
+        for (int p = 0; p < npass; ++p) {
+            const PassPlan &pp = pl.passes[p];
+            PassDev &d = host[p];
+            std::memset(&d, 0, sizeof(PassDev));
+            const int LT = __builtin_ctz(tpb[p]);
+            for (int t = 0; t < tpb[p]; ++t) {
+                uint64_t o = 0;
+                for (int k = 0; k < LT; ++k)
+                    if ((t >> k) & 1) o |= 1ull << pp.tb[k];
+                d.lo[t] = o;
+            }
+            for (int r = 0; r < rr[p]; ++r) {
+                uint64_t o = 0;
+                for (int k = 0; LT + k < pp.K; ++k)
+                    if ((r >> k) & 1) o |= 1ull << pp.tb[LT + k];
+                d.hi[r] = o;
+            }
+            const int no = (int)pp.ob.size(), nlo = no < 11 ? no : 11, nhi = no - nlo;
+            for (int x = 0; x < (1 << nlo); ++x) {
+                uint64_t o = 0;
+                for (int k = 0; k < nlo; ++k)
+                    if ((x >> k) & 1) o |= 1ull << pp.ob[k];
+                d.blo[x] = o;
+            }
+            for (int x = 0; x < (1 << nhi); ++x) {
+                uint64_t o = 0;
+                for (int k = 0; k < nhi; ++k)
+                    if ((x >> k) & 1) o |= 1ull << pp.ob[nlo + k];
+                d.bhi[x] = o;
+            }
+        }
+        PNED_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->pd_dev) PNED_HIP(hipFree(ctx->pd_dev));
+        ctx->pd_dev = nullptr;
+        PNED_HIP(hipMalloc(&ctx->pd_dev, sizeof(PassDev) * npass));
+        PNED_HIP(hipMemcpy(ctx->pd_dev, host.data(), sizeof(PassDev) * npass, hipMemcpyHostToDevice));
+        ctx->pd_threads = TPBpref;
+    }
     int total_rows = 0;
     for (int p = 0; p < npass; ++p) {
         const PassPlan &pp = pl.passes[p];
@@ -278,6 +354,7 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         a.K = pp.K;
         a.nouter = (int)pp.ob.size();
         a.tile_count = 1ull << a.nouter;
+        a.nlo = a.nouter < 11 ? a.nouter : 11;
         a.colbits = colbits;
         a.first_term = pp.first_term;
         a.nterm = pp.nterm;
@@ -285,20 +362,15 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         a.fresh = fresh ? 1 : 0;
         a.ncol = nvec;
         a.shift = -(double)ctx->ibond * eperbd;
-        for (int k = 0; k < pp.K; ++k) a.tb[k] = (uint8_t)pp.tb[k];
-        for (int k = 0; k < a.nouter; ++k) a.ob[k] = (uint8_t)pp.ob[k];
-        // threads per block: tunable (256 or 512); R = tile / threads states per thread
-        int TPB = (ctx->tune_threads == 256 || ctx->tune_threads == 512) ? ctx->tune_threads : 512;
-        while (TPB > 16 && TPB > (1 << pp.K)) TPB >>= 1;
-        const int R = (1 << pp.K) / TPB;
-        if (R > 32) return fail(PNED_E_UNSUPPORTED, "tile too large for the thread count");
+        const int TPB = tpb[p], R = rr[p];
         int grid = (int)((a.tile_count < (uint64_t)grid_cap) ? a.tile_count : (uint64_t)grid_cap);
         double *part = ctx->d_partial + (size_t)total_rows * 2 * nvec;
+        const PassDev *pdp = reinterpret_cast<const PassDev *>(ctx->pd_dev) + p;
         int rc;
         if (v1->dtype == PNED_C128)
-            rc = dispatch_pass<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_diag, a, part, grid);
+            rc = dispatch_pass<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_diag, pdp, a, part, grid);
         else
-            rc = dispatch_pass<float>(ctx, TPB, R, v1->data, v0->data, ctx->d_diag, a, part, grid);
+            rc = dispatch_pass<float>(ctx, TPB, R, v1->data, v0->data, ctx->d_diag, pdp, a, part, grid);
         if (rc != PNED_OK) return rc;
         total_rows += grid;
     }

@@ -127,6 +127,8 @@ struct pned_ctx {
     pned::DevTerm *d_terms = nullptr;
     size_t d_terms_cap = 0;
     pned::DiagDev *d_diag = nullptr;
+    void *pd_dev = nullptr;          // per-pass address tables of the current plan
+    int pd_threads = -1;
     // spin-S
     pned::GpuTables spin;
     // scratch for reductions: per-block partials and final results (pinned host mirror)
@@ -158,6 +160,7 @@ int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperb
 
 // reductions return host values (synchronizing)
 int dot(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out /*2*nvec*/);
+int dotu(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out /*2*nvec, no conjugate*/);
 int beta2(pned_ctx *ctx, const double *alpha /*2*nvec complex*/, const pned_vec *v1,
           const pned_vec *v0, double *out /*nvec*/);
 int axpby(pned_ctx *ctx, const double *a, const pned_vec *x, const double *b, pned_vec *y);

@@ -96,6 +96,25 @@ __global__ __launch_bounds__(RED_TPB) void k_dot(const typename V2<Real>::T *__r
     block_reduce_cols<2>(red, acc, ncol, partial + (size_t)blockIdx.x * 2 * ncol);
 }
 
+// bilinear (unconjugated) product: sum x y per column -> 2 values per column
+template <typename Real>
+__global__ __launch_bounds__(RED_TPB) void k_dotu(const typename V2<Real>::T *__restrict__ x,
+                                                  const typename V2<Real>::T *__restrict__ y,
+                                                  size_t nelem, int ncol, double *__restrict__ partial)
+{
+    __shared__ double red[2 * RED_TPB];
+    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    double acc[2] = {0.0, 0.0};
+    for (size_t e = (size_t)blockIdx.x * RED_TPB + threadIdx.x; e < nelem; e += stride) {
+        double xr, xi, yr, yi;
+        ld(x, e, xr, xi);
+        ld(y, e, yr, yi);
+        acc[0] = fma(xr, yr, fma(-xi, yi, acc[0]));
+        acc[1] = fma(xr, yi, fma(xi, yr, acc[1]));
+    }
+    block_reduce_cols<2>(red, acc, ncol, partial + (size_t)blockIdx.x * 2 * ncol);
+}
+
 // beta2: sum |v0 - alpha_c v1|^2 per column (alpha complex per column, in constant args)
 template <typename Real>
 __global__ __launch_bounds__(RED_TPB) void k_beta2(const typename V2<Real>::T *__restrict__ v1,
@@ -432,6 +451,23 @@ int dot(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out)
                            (const double2 *)y->data, nelem, x->nvec, ctx->d_partial);
     else
         hipLaunchKernelGGL(k_dot<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const float2 *)x->data,
+                           (const float2 *)y->data, nelem, x->nvec, ctx->d_partial);
+    PNED_HIP(hipGetLastError());
+    return finish_reduce(ctx, grid, 2 * x->nvec, out);
+}
+
+int dotu(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out)
+{
+    PNED_TRY(same_shape(x, y));
+    PNED_TRY(check_pow2_cols(x->nvec));
+    const size_t nelem = (size_t)x->dim * x->nvec;
+    const int grid = RED_GRID;
+    PNED_TRY(ensure_partial(ctx, (size_t)grid * 2 * x->nvec));
+    if (x->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_dotu<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const double2 *)x->data,
+                           (const double2 *)y->data, nelem, x->nvec, ctx->d_partial);
+    else
+        hipLaunchKernelGGL(k_dotu<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const float2 *)x->data,
                            (const float2 *)y->data, nelem, x->nvec, ctx->d_partial);
     PNED_HIP(hipGetLastError());
     return finish_reduce(ctx, grid, 2 * x->nvec, out);

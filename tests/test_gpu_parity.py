@@ -231,3 +231,60 @@ def test_dsf_parity_n12():
     d2 = oracle.normalize_dsf(d2, 2.5e-3, cn)
     assert np.allclose(dsf, d2, rtol=1e-10, atol=1e-14)
     assert np.array_equal(conv, c2)
+
+
+def test_cocg_complex_shift_spin1():
+    _say("cocg spin1 n8")
+    cl = lattice.reference_cluster(8)
+    op = model.build_operator(cl, 20, 108.0, q=3)
+    ctx = gpu.Context(op)
+    b, x, r, p, y = (ctx.vector() for _ in range(5))
+    ctx.set_initial_vec(b, 4)
+    bh = b.download()
+    e0, eta = -3.0, 0.05
+    it, res = ctx.cocg(e0, eta, b, x, r, p, y, max_it=3000, tol=1e-10)
+    assert res < 1e-10
+    xs = x.download()
+    # independent check: residual of the oracle's H.v, and a sparse direct solve
+    hx = np.zeros_like(xs)
+    oracle.hv(oracle.OracleOp(op), xs, hx)
+    resid = hx - e0 * xs + 1j * eta * xs - bh
+    assert np.linalg.norm(resid) <= 2e-10 * np.linalg.norm(bh)
+    import scipy.sparse.linalg as spla
+    from tests import dense
+    J, K = dense.model20_couplings(108.0)
+    H = dense.kh_dense(cl, J, K, K, K, q=3)
+    import scipy.sparse as sp
+    A = (sp.csr_matrix(H) - (e0 - 1j * eta) * sp.identity(H.shape[0])).tocsc()
+    xd = spla.spsolve(A, bh)
+    assert np.linalg.norm(xs - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+def test_fortran_host_driver(tmp_path):
+    """The ISO_C_BINDING layer end to end: a Fortran host reads the data_mem tables and
+    runs lnc1 + p_mltply_v3 through pned_parallelized_diag."""
+    _say("fortran driver")
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(gpu.LIB_PATH), "fortran", "_build", "pned_lanczos_driver")
+    assert os.path.exists(exe), "build the Fortran layer first (__graft_entry__.build())"
+    cl = lattice.reference_cluster(16)
+    op = model.build_operator(cl, 20, 126.0, field=0.1)
+    f = tmp_path / "tables.bin"
+    with open(f, "wb") as fh:
+        np.array([op.n, op.q, op.ibond], dtype=np.int32).tofile(fh)
+        np.asarray(op.ipair, dtype=np.int32).tofile(fh)
+        np.asarray(op.ibond_type, dtype=np.int32).tofile(fh)
+        op.fortran_xlocal().astype(np.complex128).tofile(fh)
+        op.fortran_llocal().astype(np.int32).tofile(fh)
+    out = subprocess.run([exe, str(f)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    vals = dict(line.split(None, 1) for line in out.stdout.strip().splitlines())
+    e0 = float(vals["E0"])
+    x = oracle.gaussian(2 ** 16, 20261202)
+    itr, a, b, ene = oracle.lanczos(oracle.OracleOp(op), x, max_itr=1024, nvec=8)
+    assert abs(e0 - ene[0]) <= 1e-10 * abs(ene[0])
+    pr = complex(*map(float, vals["PRDCT"].split()))
+    want = np.zeros_like(x)
+    p_ref = oracle.hv(oracle.OracleOp(op), x, want)
+    assert abs(pr - p_ref) <= 1e-12 * abs(p_ref)
