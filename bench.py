@@ -16,7 +16,7 @@ cpu_baseline: the oracle's restatement of the reference's per-element pass struc
 (OpenMP, all host threads) on a bounded sample -- output rows [0, 2^24) of the same
 N=32 operator -- reported as algorithmic H.v GB/s.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--cfg N32|N24]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--cfg N32|N24|TPS|SERVE]
 """
 import argparse
 import json
@@ -40,6 +40,8 @@ def main():
     ap.add_argument("--cfg", default="N32")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="", help="fixed plan K,L,threads (skips the autotuner)")
+    ap.add_argument("--requests", type=int, default=120, help="SERVE: number of requests")
+    ap.add_argument("--serve-big", action="store_true", help="SERVE: include the N=28 (2x7) size")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 3 device copies of a 4 GiB vector (counter calibration)")
     args = ap.parse_args()
@@ -64,6 +66,11 @@ def main():
         ctx.sync()
         print(json.dumps({"calibrate_copy_bytes": 16 * ctx.dim, "copies": 3}))
         return
+
+    if args.cfg == "TPS":
+        return bench_tps(args)
+    if args.cfg == "SERVE":
+        return bench_serve(args)
 
     if args.cfg == "N32":
         cl = lattice.torus(4, 4)
@@ -141,6 +148,112 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_tps(args):
+    """Batched mTPQ: N=24 reference cluster, MODEL 20 phi=90 deg (J=0, K1=+2 in the
+    reference's H = sum K S^g S^g, i.e. antiferromagnetic Kitaev; FM is phi=270 deg),
+    R=64 interleaved samples.  Fused complex64 storage (FP64 reductions and scalars) vs
+    the plain FP64 GPU path (complex128, one sample at a time, the reference's step
+    structure: update pass, H.v, two dots, observable pass).  Errors: E/N and T of every
+    step and C(T) of the c64 run against a complex128 run of the same fused recursion on
+    the same start vectors."""
+    import numpy as np
+    import torch  # noqa: F401
+
+    from pned_amd import gpu, lattice, model
+
+    cl = lattice.reference_cluster(24)
+    op = model.build_operator(cl, 20, 90.0, tps=True)
+    ctx = gpu.Context(op)
+    ctx.tps_set_observables(cl.site_vec, cl.sq_momenta())
+    R = 64
+    steps = args.steps if args.steps != 10 else 2000
+    L = op.shift_L
+
+    def run_fused(dtype, nsteps):
+        v1, va, vb = (ctx.vector(R, dtype) for _ in range(3))
+        ctx.set_initial_vec(v1, 20261202)
+        ctx.sync()
+        t = time.perf_counter()
+        a, b, c = ctx.tps_fused(v1, va, vb, L, nsteps)
+        ctx.sync()
+        dt = time.perf_counter() - t
+        for v in (v1, va, vb):
+            v.free()
+        return dt, a, b, c
+
+    t64, a64, b64, c64 = run_fused(gpu.C64, steps)
+    t128, a128, b128, c128 = run_fused(gpu.C128, steps)
+    # plain path: complex128, one sample, reference step structure; timed on a bounded sample
+    plain_steps = min(200, steps)
+    v1, v0, vg, vt = (ctx.vector(1, gpu.C128) for _ in range(4))
+    ctx.set_initial_vec(v1, 20261202)
+    ctx.sync()
+    t = time.perf_counter()
+    ctx.tps(v1, v0, vg, vt, L, plain_steps)
+    ctx.sync()
+    tplain = (time.perf_counter() - t) * (steps / plain_steps) * R
+    n = cl.n
+    E64, E128 = a64 / n, a128 / n
+    T64 = (L - a64) / (2.0 * np.arange(1, steps + 1))[:, None]
+    T128 = (L - a128) / (2.0 * np.arange(1, steps + 1))[:, None]
+
+    def heat(E, T):
+        dE = np.diff(E, axis=0)
+        dT = np.diff(T, axis=0)
+        return dE / dT
+    C64, C128 = heat(E64, T64), heat(E128, T128)
+    ok = np.abs(C128) < 1e3
+    out = {
+        "metric": "batched mTPQ speedup (fused complex64, R=64) over the plain FP64 GPU path",
+        "value": round(tplain / t64, 3), "unit": "x", "higher_is_better": True,
+        "config": {"workload": "mTPQ N=24 ref cluster, phi=90deg (AFM Kitaev K1=+2), R=64, L=ibond*l_b",
+                   "steps": steps, "c64_fused_s": round(t64, 3), "c128_fused_s": round(t128, 3),
+                   "plain_c128_s_extrapolated": round(tplain, 3),
+                   "plain_sample": f"1 sample x {plain_steps} steps, scaled by {R} samples and {steps}/{plain_steps}",
+                   "max_abs_err_E_per_site": float(np.max(np.abs(E64 - E128))),
+                   "max_rel_err_T": float(np.max(np.abs(T64 - T128) / np.abs(T128))),
+                   "max_abs_err_C": float(np.max(np.abs(C64 - C128)[ok])) if ok.any() else None,
+                   "speedup_fused_c128_vs_plain": round(tplain / t128, 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(args):
+    """Mixed-size ground-state serving: continuous batching vs sequential plain solves."""
+    import torch  # noqa: F401
+
+    from pned_amd import serving
+
+    specs = ["2x4", "3x3", "2x5", "3x4"] + (["2x7"] if args.serve_big else [])
+    nreq = args.requests
+    # capacity: closed loop (all requests available at t=0)
+    warm = serving.make_requests(8, specs, seed=1)
+    serving.Server(tol=1e-10).run(warm)
+    reqs = serving.make_requests(nreq, specs, seed=20261202)
+    srv = serving.Server(tol=1e-10)
+    t = time.perf_counter()
+    res = srv.run(reqs)
+    cap = len(res) / (time.perf_counter() - t)
+    # open loop: Poisson arrivals at 80% of capacity
+    reqs = serving.make_requests(nreq, specs, seed=20261203, rate=0.8 * cap)
+    srv = serving.Server(tol=1e-10)
+    t = time.perf_counter()
+    res = srv.run(reqs)
+    wall = time.perf_counter() - t
+    batched = serving.summarize(res, wall)
+    seq = serving.Server(tol=1e-10, sequential=True)
+    t = time.perf_counter()
+    res2 = seq.run(serving.make_requests(nreq, specs, seed=20261203, rate=0.8 * cap))
+    wall2 = time.perf_counter() - t
+    plain = serving.summarize(res2, wall2)
+    out = {"metric": "mixed-size serving solves/s (continuous batching) vs sequential plain solves",
+           "value": batched["solves_per_s"], "unit": "solves/s", "higher_is_better": True,
+           "config": {"sizes": specs, "requests": nreq, "capacity_closed_loop": round(cap, 3),
+                      "arrival_rate": round(0.8 * cap, 3), "batched": batched, "sequential": plain,
+                      "tol": 1e-10}}
     print(json.dumps(out), flush=True)
 
 

@@ -58,6 +58,15 @@ int pned_ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair,
                     const int32_t *ibond_type, const double *xlocal_ham,
                     const int32_t *llocal_ham, int device, pned_ctx **out);
 int pned_ctx_destroy(pned_ctx *ctx);
+/* batch context (no reference counterpart; for serving same-size requests together):
+ * nop operators on the same cluster, one per column of the context's vectors (nop a power
+ * of two <= 64).  xlocal_ham/llocal_ham hold nop consecutive tables.  The operators share
+ * one term structure; a later operator must fit it (pned_ctx_set_column_matrices). */
+int pned_ctx_create_batch(int n_sites, int q_spin, int ibond, const int32_t *ipair,
+                          const int32_t *ibond_type, int nop, const double *xlocal_ham,
+                          const int32_t *llocal_ham, int device, pned_ctx **out);
+int pned_ctx_set_column_matrices(pned_ctx *ctx, int col, const double *xlocal_ham,
+                                 const int32_t *llocal_ham);
 /* basis dimension q^N (ldim at np=1, S_one_half/Hybrid_TPS_MPIlib.f90:74-84) */
 int64_t pned_ctx_dim(const pned_ctx *ctx);
 /* replace the bond matrices in place (same sparsity structure not required),
@@ -147,6 +156,19 @@ int pned_lanczos_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double 
 int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, int fixed_steps,
               double *alpha, double *beta, double *ene, int *itr);
 
+/* Serving (no reference counterpart): one Lanczos step of every slot (column) of a batch.
+ * mode[c]: 0 idle (column zeroed, alpha = beta = 0), 1 first step (v1 holds the start
+ * vector, v0 is cleared; the lnc1z start, Hybrid_TPS_Lanczos.f90:78-85), 2 continue (the
+ * update of Hybrid_TPS_Lanczos.f90:111-116 with alpha_prev[c], beta_prev[c]).  Then one
+ * batched H.v and per-column alpha[c] = <v1|H|v1>, beta[c] = ||v0 - alpha v1||.  Every
+ * column's arithmetic is independent of the others (fixed-order per-column reductions). */
+int pned_lanczos_batch_step(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int *mode,
+                            const double *alpha_prev, const double *beta_prev, double *alpha,
+                            double *beta);
+/* column 'col' of v := normalised Gaussian start vector of 'seed' (the values of column 0
+ * of pned_set_initial_vec for that seed, whatever the column) */
+int pned_set_initial_column(pned_ctx *ctx, pned_vec *v, int col, uint64_t seed);
+
 /* bisec (Hybrid_TPS_basic.f90:25-107) and vec12 (112-213); host FP64.  coef is ndim x nvec
  * column-major. */
 int pned_bisec(const double *alpha, const double *beta, int ndim, double *ene, int ne, double eps);
@@ -189,6 +211,13 @@ int pned_tps_set_observables(pned_ctx *ctx, const double *site_vec, int nsq, con
  * [nsteps][nvec], coef is [nsteps][nvec][7+nsq] (row of eval_phys_val per column). */
 int pned_tps(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, pned_vec *vt, double lmax,
              int nsteps, double *alpha, double *beta, double *coef);
+
+/* Same outputs as pned_tps with the vector work fused (rotation instead of copies, the
+ * 1/beta normalisation folded into the next H.v, one reduction pass per step).  v1 holds
+ * the normalised start columns; va, vb are work vectors; on return all three hold work
+ * data.  At most 8 structure-factor entries. */
+int pned_tps_fused(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, double lmax, int nsteps,
+                   double *alpha, double *beta, double *coef);
 
 /* ---- DSF (Hybrid_TPS_DSF.f90) ------------------------------------------------------------ */
 /* intalv_zz (Hybrid_TPS_DSF.f90:18-87): v = normalised sum_j w(d_j) e^{i q.r_j} x with

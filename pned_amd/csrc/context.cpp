@@ -49,23 +49,25 @@ void walsh(const Cx g[4], Cx &e0, Cx &e1, Cx &e2, Cx &e12)
 }
 }  // namespace
 
-int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype, const double *xl,
-                  const int32_t *ll, S12Plan &plan)
-{
-    plan = S12Plan();
-    plan.n = n;
+struct RawDecomp {
     std::map<std::pair<uint32_t, uint32_t>, Term> terms;
-    std::vector<Cx> site_w(n);
+    std::vector<Cx> site_w;
     std::map<std::pair<int, int>, Cx> pair_w;   // (a, b) -> weight of parity(a,b)
     Cx c0;
+};
+
+static int decompose_raw(int n, int ibond, const int32_t *ipair, const int32_t *btype, const double *xl,
+                         const int32_t *ll, RawDecomp &rd)
+{
+    rd = RawDecomp();
+    rd.site_w.assign(n, Cx());
     auto X = [&](int t, int in, int out) -> Cx {
         const size_t f = (size_t)in + 4 * (size_t)out + 16 * (size_t)t;
         if (ll[f]) return Cx();
         return Cx{xl[2 * f], xl[2 * f + 1]};
     };
     auto add_term = [&](uint32_t gmask, uint32_t selmask, Cx cs0, Cx cs1) {
-        auto key = std::make_pair(gmask, selmask);
-        Term &T = terms[key];
+        Term &T = rd.terms[std::make_pair(gmask, selmask)];
         T.gmask = gmask;
         T.selmask = selmask;
         T.c0r += cs0.r;
@@ -80,7 +82,6 @@ int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype, 
         if (s_a > s_b) std::swap(s_a, s_b);
         const int t = btype[b] - 1;
         if (t < 0 || t > 8) return fail(PNED_E_ARG, "bond type outside 1..9");
-        // diagonal
         for (int l = 0; l < 4; ++l)
             if (ll[(size_t)l + 4 * l + 16 * t])
                 return fail(PNED_E_UNSUPPORTED, "a masked bond diagonal (llocal_Ham(k,k)) is not supported");
@@ -89,11 +90,10 @@ int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype, 
         Cx e0, e1, e2, e12;
         walsh(g, e0, e1, e2, e12);
         // d = e0 + e1 sa + e2 sb + e12 sa sb, sigma = 1 - 2s, sigma_a sigma_b = 1 - 2 par
-        c0 = c0 + e0 + e1 + e2 + e12;
-        site_w[s_a] = site_w[s_a] - 2.0 * e1;
-        site_w[s_b] = site_w[s_b] - 2.0 * e2;
-        pair_w[{s_a, s_b}] = pair_w[{s_a, s_b}] - 2.0 * e12;
-        // flips
+        rd.c0 = rd.c0 + e0 + e1 + e2 + e12;
+        rd.site_w[s_a] = rd.site_w[s_a] - 2.0 * e1;
+        rd.site_w[s_b] = rd.site_w[s_b] - 2.0 * e2;
+        rd.pair_w[{s_a, s_b}] = rd.pair_w[{s_a, s_b}] - 2.0 * e12;
         const uint32_t ba = 1u << s_a, bb = 1u << s_b;
         for (int f = 1; f <= 3; ++f) {
             for (int l = 0; l < 4; ++l) g[l] = X(t, l ^ f, l);
@@ -107,24 +107,168 @@ int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype, 
             if (!used && nz(base)) add_term(gm, 0u, base, base);
         }
     }
-    for (auto &kv : terms) {
-        const Term &T = kv.second;
-        if (nz(Cx{T.c0r, T.c0i}) || nz(Cx{T.c1r, T.c1i})) plan.terms.push_back(T);
+    return PNED_OK;
+}
+
+static bool term_nz(const Term &T) { return nz(Cx{T.c0r, T.c0i}) || nz(Cx{T.c1r, T.c1i}); }
+
+int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype, const double *xl,
+                  const int32_t *ll, S12Plan &plan)
+{
+    const double *xls[1] = {xl};
+    const int32_t *lls[1] = {ll};
+    return s12_decompose_batch(n, ibond, ipair, btype, 1, xls, lls, plan);
+}
+
+int s12_decompose_batch(int n, int ibond, const int32_t *ipair, const int32_t *btype, int nop,
+                        const double *const *xl, const int32_t *const *ll, S12Plan &plan)
+{
+    plan = S12Plan();
+    plan.n = n;
+    plan.nop = nop;
+    std::vector<RawDecomp> rd(nop);
+    for (int o = 0; o < nop; ++o) PNED_TRY(decompose_raw(n, ibond, ipair, btype, xl[o], ll[o], rd[o]));
+    // term structure: union of the nonzero (mask, selector) keys
+    std::map<std::pair<uint32_t, uint32_t>, int> key_index;
+    for (int o = 0; o < nop; ++o)
+        for (auto &kv : rd[o].terms)
+            if (term_nz(kv.second)) key_index[kv.first] = 0;
+    for (auto &kv : key_index) {
+        kv.second = (int)plan.terms.size();
+        Term t{};
+        t.gmask = kv.first.first;
+        t.selmask = kv.first.second;
+        auto it = rd[0].terms.find(kv.first);
+        if (it != rd[0].terms.end()) t = it->second;
+        plan.terms.push_back(t);
     }
-    // diagonal groups: sites with equal weight; bond parities grouped by (distance, weight)
-    plan.c0r = c0.r;
-    plan.c0i = c0.i;
-    std::map<std::pair<double, double>, uint32_t> sg;
-    for (int k = 0; k < n; ++k)
-        if (nz(site_w[k])) sg[{site_w[k].r, site_w[k].i}] |= 1u << k;
-    for (auto &kv : sg) plan.diag.push_back(DiagGroup{0, 0, kv.second, kv.first.first, kv.first.second});
-    std::map<std::tuple<int, double, double>, uint32_t> pg;
-    for (auto &kv : pair_w)
-        if (nz(kv.second))
-            pg[std::make_tuple(kv.first.second - kv.first.first, kv.second.r, kv.second.i)] |= 1u << kv.first.first;
-    for (auto &kv : pg)
-        plan.diag.push_back(DiagGroup{1, std::get<0>(kv.first), kv.second, std::get<1>(kv.first), std::get<2>(kv.first)});
+    plan.colcoef.assign(plan.terms.size() * nop * 4, 0.0);
+    for (int o = 0; o < nop; ++o)
+        for (auto &kv : rd[o].terms) {
+            auto it = key_index.find(kv.first);
+            if (it == key_index.end()) continue;   // zero in every operator
+            double *c = &plan.colcoef[((size_t)it->second * nop + o) * 4];
+            c[0] = kv.second.c0r;
+            c[1] = kv.second.c0i;
+            c[2] = kv.second.c1r;
+            c[3] = kv.second.c1i;
+        }
+    // diagonal groups: sites (pairs per distance) with the same weight in every operator
+    std::map<std::vector<double>, uint32_t> sg;
+    for (int k = 0; k < n; ++k) {
+        std::vector<double> key;
+        bool any = false;
+        for (int o = 0; o < nop; ++o) {
+            key.push_back(rd[o].site_w[k].r);
+            key.push_back(rd[o].site_w[k].i);
+            any = any || nz(rd[o].site_w[k]);
+        }
+        if (any) sg[key] |= 1u << k;
+    }
+    std::map<std::vector<double>, uint32_t> pg;   // key: distance, weights...
+    for (auto &kv : rd[0].pair_w) {
+        std::vector<double> key{(double)(kv.first.second - kv.first.first)};
+        bool any = false;
+        for (int o = 0; o < nop; ++o) {
+            const Cx w = rd[o].pair_w[kv.first];
+            key.push_back(w.r);
+            key.push_back(w.i);
+            any = any || nz(w);
+        }
+        if (any) pg[key] |= 1u << kv.first.first;
+    }
+    for (auto &kv : sg) plan.diag.push_back(DiagGroup{0, 0, kv.second, kv.first[0], kv.first[1]});
+    for (auto &kv : pg) plan.diag.push_back(DiagGroup{1, (int)kv.first[0], kv.second, kv.first[1], kv.first[2]});
     if (plan.diag.size() > 48) return fail(PNED_E_UNSUPPORTED, "diagonal has more than 48 coefficient groups");
+    plan.c0r = rd[0].c0.r;
+    plan.c0i = rd[0].c0.i;
+    const size_t ng = plan.diag.size();
+    plan.coldiag.assign((size_t)nop * (ng + 1) * 2, 0.0);
+    size_t g = 0;
+    for (auto &kv : sg) {
+        for (int o = 0; o < nop; ++o) {
+            plan.coldiag[((size_t)o * (ng + 1) + g) * 2] = kv.first[2 * o];
+            plan.coldiag[((size_t)o * (ng + 1) + g) * 2 + 1] = kv.first[2 * o + 1];
+        }
+        ++g;
+    }
+    for (auto &kv : pg) {
+        for (int o = 0; o < nop; ++o) {
+            plan.coldiag[((size_t)o * (ng + 1) + g) * 2] = kv.first[1 + 2 * o];
+            plan.coldiag[((size_t)o * (ng + 1) + g) * 2 + 1] = kv.first[2 + 2 * o];
+        }
+        ++g;
+    }
+    for (int o = 0; o < nop; ++o) {
+        plan.coldiag[((size_t)o * (ng + 1) + ng) * 2] = rd[o].c0.r;
+        plan.coldiag[((size_t)o * (ng + 1) + ng) * 2 + 1] = rd[o].c0.i;
+    }
+    return PNED_OK;
+}
+
+int s12_set_column(S12Plan &plan, int ibond, const int32_t *ipair, const int32_t *btype, int col,
+                   const double *xl, const int32_t *ll)
+{
+    if (col < 0 || col >= plan.nop) return fail(PNED_E_ARG, "column outside the batch");
+    RawDecomp rd;
+    PNED_TRY(decompose_raw(plan.n, ibond, ipair, btype, xl, ll, rd));
+    std::map<std::pair<uint32_t, uint32_t>, int> idx;
+    for (size_t t = 0; t < plan.terms.size(); ++t) idx[{plan.terms[t].gmask, plan.terms[t].selmask}] = (int)t;
+    std::vector<double> cc(plan.terms.size() * 4, 0.0);
+    for (auto &kv : rd.terms) {
+        if (!term_nz(kv.second)) continue;
+        auto it = idx.find(kv.first);
+        if (it == idx.end()) return fail(PNED_E_UNSUPPORTED, "operator has a term outside the batch structure");
+        double *c = &cc[(size_t)it->second * 4];
+        c[0] = kv.second.c0r;
+        c[1] = kv.second.c0i;
+        c[2] = kv.second.c1r;
+        c[3] = kv.second.c1i;
+    }
+    // diagonal: every group must carry one weight for this operator
+    const size_t ng = plan.diag.size();
+    std::vector<double> dw((ng + 1) * 2, 0.0);
+    std::vector<char> seen_site(plan.n, 0);
+    std::map<std::pair<int, int>, char> seen_pair;
+    for (size_t g = 0; g < ng; ++g) {
+        const DiagGroup &G = plan.diag[g];
+        bool first = true;
+        Cx w;
+        for (int k = 0; k < plan.n; ++k) {
+            if (!(G.mask >> k & 1)) continue;
+            Cx x;
+            if (G.kind == 0) {
+                x = rd.site_w[k];
+                seen_site[k] = 1;
+            } else {
+                auto it = rd.pair_w.find({k, k + G.shift});
+                if (it != rd.pair_w.end()) x = it->second;
+                seen_pair[{k, k + G.shift}] = 1;
+            }
+            if (first) { w = x; first = false; }
+            else if (x.r != w.r || x.i != w.i) return fail(PNED_E_UNSUPPORTED, "diagonal weights do not fit the batch groups");
+        }
+        dw[2 * g] = w.r;
+        dw[2 * g + 1] = w.i;
+    }
+    for (int k = 0; k < plan.n; ++k)
+        if (!seen_site[k] && nz(rd.site_w[k])) return fail(PNED_E_UNSUPPORTED, "site weight outside the batch groups");
+    for (auto &kv : rd.pair_w)
+        if (!seen_pair.count(kv.first) && nz(kv.second))
+            return fail(PNED_E_UNSUPPORTED, "bond weight outside the batch groups");
+    dw[2 * ng] = rd.c0.r;
+    dw[2 * ng + 1] = rd.c0.i;
+    for (size_t t = 0; t < plan.terms.size(); ++t)
+        for (int k = 0; k < 4; ++k) plan.colcoef[((size_t)t * plan.nop + col) * 4 + k] = cc[t * 4 + k];
+    for (size_t k = 0; k < dw.size(); ++k) plan.coldiag[(size_t)col * (ng + 1) * 2 + k] = dw[k];
+    // keep devcoef (pass order) in step
+    if (!plan.devcoef.empty()) {
+        for (size_t d = 0; d < plan.devterms.size(); ++d) {
+            const int t = plan.devterms[d].pad;   // index into plan.terms
+            for (int k = 0; k < 4; ++k)
+                plan.devcoef[((size_t)d * plan.nop + col) * 4 + k] = cc[(size_t)t * 4 + k];
+        }
+    }
     return PNED_OK;
 }
 
@@ -235,6 +379,7 @@ int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
                 if (mmask[i] >> pp.tb[k] & 1) tm |= 1u << k;
             d.tmask = tm;
             d.selmask = t.selmask;
+            d.pad = (uint32_t)i;   // index into plan.terms (batch coefficients)
             d.is_real = (t.c0i == 0.0 && t.c1i == 0.0) ? 1u : 0u;
             d.c0r = t.c0r;
             d.c0i = t.c0i;
@@ -244,6 +389,16 @@ int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
         }
         pp.nterm = (int)plan.devterms.size() - pp.first_term;
         plan.passes.push_back(pp);
+    }
+    plan.devcoef.clear();
+    if (plan.nop > 1) {
+        plan.devcoef.resize(plan.devterms.size() * plan.nop * 4);
+        for (size_t d = 0; d < plan.devterms.size(); ++d) {
+            const size_t t = plan.devterms[d].pad;
+            for (int o = 0; o < plan.nop; ++o)
+                for (int k = 0; k < 4; ++k)
+                    plan.devcoef[(d * plan.nop + o) * 4 + k] = plan.colcoef[(t * plan.nop + o) * 4 + k];
+        }
     }
     return PNED_OK;
 }
@@ -328,6 +483,24 @@ int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
     ctx->plan_colbits = colbits;
     ctx->plan_dtype = dtype;
     ctx->pd_threads = -1;    // address tables follow the new plan
+    return upload_batch_coefficients(ctx);
+}
+
+int upload_batch_coefficients(pned_ctx *ctx)
+{
+    const S12Plan &pl = ctx->s12;
+    if (pl.nop <= 1) return PNED_OK;
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_colcoef) PNED_HIP(hipFree(ctx->d_colcoef));
+    if (ctx->d_coldiag) PNED_HIP(hipFree(ctx->d_coldiag));
+    ctx->d_colcoef = ctx->d_coldiag = nullptr;
+    if (!pl.devcoef.empty()) {
+        PNED_HIP(hipMalloc((void **)&ctx->d_colcoef, sizeof(double) * pl.devcoef.size()));
+        PNED_HIP(hipMemcpy(ctx->d_colcoef, pl.devcoef.data(), sizeof(double) * pl.devcoef.size(),
+                           hipMemcpyHostToDevice));
+    }
+    PNED_HIP(hipMalloc((void **)&ctx->d_coldiag, sizeof(double) * pl.coldiag.size()));
+    PNED_HIP(hipMemcpy(ctx->d_coldiag, pl.coldiag.data(), sizeof(double) * pl.coldiag.size(), hipMemcpyHostToDevice));
     return PNED_OK;
 }
 
@@ -410,6 +583,39 @@ int pned_ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair, co
     return PNED_OK;
 }
 
+int pned_ctx_create_batch(int n_sites, int q_spin, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                          int nop, const double *xlocal_ham, const int32_t *llocal_ham, int device, pned_ctx **out)
+{
+    if (q_spin != 2) return fail(PNED_E_UNSUPPORTED, "batch contexts are S=1/2");
+    if (nop < 1 || nop > 64 || (nop & (nop - 1))) return fail(PNED_E_ARG, "nop must be a power of two <= 64");
+    PNED_TRY(pned_ctx_create(n_sites, q_spin, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, device, out));
+    pned_ctx *ctx = *out;
+    const size_t nx = 2 * 16 * 9, nl = 16 * 9;
+    std::vector<const double *> xs(nop);
+    std::vector<const int32_t *> ls(nop);
+    for (int o = 0; o < nop; ++o) {
+        xs[o] = xlocal_ham + o * nx;
+        ls[o] = llocal_ham + o * nl;
+    }
+    int rc = s12_decompose_batch(n_sites, ibond, ipair, ibond_type, nop, xs.data(), ls.data(), ctx->s12);
+    if (rc != PNED_OK) {
+        pned_ctx_destroy(ctx);
+        *out = nullptr;
+        return rc;
+    }
+    ctx->plan_colbits = -1;
+    return PNED_OK;
+}
+
+int pned_ctx_set_column_matrices(pned_ctx *ctx, int col, const double *xlocal_ham, const int32_t *llocal_ham)
+{
+    if (!ctx || !xlocal_ham || !llocal_ham) return fail(PNED_E_ARG, "null argument");
+    if (ctx->q != 2 || ctx->s12.nop <= 1) return fail(PNED_E_ARG, "not a batch context");
+    PNED_TRY(s12_set_column(ctx->s12, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), col, xlocal_ham, llocal_ham));
+    if (ctx->plan_colbits >= 0) PNED_TRY(upload_batch_coefficients(ctx));
+    return PNED_OK;
+}
+
 int pned_ctx_destroy(pned_ctx *ctx)
 {
     if (!ctx) return PNED_OK;
@@ -418,6 +624,8 @@ int pned_ctx_destroy(pned_ctx *ctx)
     if (ctx->d_terms) (void)hipFree(ctx->d_terms);
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
     if (ctx->pd_dev) (void)hipFree(ctx->pd_dev);
+    if (ctx->d_colcoef) (void)hipFree(ctx->d_colcoef);
+    if (ctx->d_coldiag) (void)hipFree(ctx->d_coldiag);
     if (ctx->d_partial) (void)hipFree(ctx->d_partial);
     if (ctx->d_result) (void)hipFree(ctx->d_result);
     if (ctx->h_result) (void)hipHostFree(ctx->h_result);

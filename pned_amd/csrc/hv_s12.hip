@@ -31,7 +31,11 @@ struct PassArgs {
     int fresh;             // first pass: v0 input is zero
     int ncol;              // columns (1 << colbits)
     int nlo;               // outer bits resolved by the first base table
+    int nop;               // >1: one operator per column (batch context)
     double shift;          // -ibond * eperbd, added to the diagonal
+    const double *colcoef; // [devterm][nop][4] when nop > 1
+    const double *coldiag; // [nop][ngroups+1][2] when nop > 1
+    const double *colscale;// per-column factor on v1 (H (s v1)); null = 1
 };
 
 // Per-pass address tables in device memory (read with scalar loads):
@@ -164,13 +168,46 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
             if (a.diag) {
                 const DiagDev &dg = *diagp;
                 static_for<CH>([&](auto c) {
-                    double di;
-                    const double dr = diag_value_re(dg, st[c], di) + a.shift;
+                    double di, dr;
+                    if (a.nop > 1) {
+                        // per-column weights; the thread's column is tid & (nop-1)
+                        const double *w = a.coldiag + (size_t)(tid & (a.nop - 1)) * (dg.ngroups + 1) * 2;
+                        dr = w[2 * dg.ngroups];
+                        di = w[2 * dg.ngroups + 1];
+                        for (int g = 0; g < dg.ngroups; ++g) {
+                            const uint32_t f = (dg.kind[g] == 0) ? (st[c] & dg.mask[g])
+                                                                 : ((st[c] ^ (st[c] >> dg.shift[g])) & dg.mask[g]);
+                            const double cnt = (double)__popc(f);
+                            dr = fma(cnt, w[2 * g], dr);
+                            di = fma(cnt, w[2 * g + 1], di);
+                        }
+                        dr += a.shift;
+                    } else {
+                        dr = diag_value_re(dg, st[c], di) + a.shift;
+                    }
                     const V x = tile[tid + TPB * (r0 + c)];
                     ar[c] = dr * (double)x.x - di * (double)x.y;
                     ai[c] = dr * (double)x.y + di * (double)x.x;
                 });
             }
+            if (a.nop > 1) {
+                const double *cbase = a.colcoef + (size_t)(tid & (a.nop - 1)) * 4;
+                for (int t = 0; t < a.nterm; ++t) {
+                    const DevTerm T = terms[a.first_term + t];
+                    const double *cc = cbase + (size_t)(a.first_term + t) * a.nop * 4;
+                    const double c0r = cc[0], c0i = cc[1];
+                    const double dr = cc[2] - c0r, di = cc[3] - c0i;
+                    static_for<CH>([&](auto c) {
+                        const V p = tile[(tid + TPB * (r0 + c)) ^ T.tmask];
+                        const double sel = (double)(__popc(st[c] & T.selmask) & 1);
+                        const double cr = fma(sel, dr, c0r), ci = fma(sel, di, c0i);
+                        ar[c] = fma(cr, (double)p.x, ar[c]);
+                        ar[c] = fma(-ci, (double)p.y, ar[c]);
+                        ai[c] = fma(cr, (double)p.y, ai[c]);
+                        ai[c] = fma(ci, (double)p.x, ai[c]);
+                    });
+                }
+            } else
             for (int t = 0; t < a.nterm; ++t) {
                 const DevTerm T = terms[a.first_term + t];
                 if (T.is_real) {
@@ -194,6 +231,13 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
                         ai[c] = fma(ci, (double)p.x, ai[c]);
                     });
                 }
+            }
+            if (a.colscale) {
+                const double sc = a.colscale[tid & (a.ncol - 1)];
+                static_for<CH>([&](auto c) {
+                    ar[c] *= sc;
+                    ai[c] *= sc;
+                });
             }
             static_for<CH>([&](auto c) {
                 const V x = tile[tid + TPB * (r0 + c)];
@@ -281,7 +325,7 @@ static int dispatch_pass(pned_ctx *ctx, int TPB, int R, const void *v1, void *v0
 }
 
 int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
-                  double *prdct_host)
+                  double *prdct_host, const double *d_colscale)
 {
     const int nvec = v1->nvec;
     int colbits = 0;
@@ -290,6 +334,8 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         return fail(PNED_E_UNSUPPORTED, "batched H.v needs a power-of-two column count");
     PNED_TRY(ensure_s12_plan(ctx, colbits, v1->dtype));
     const S12Plan &pl = ctx->s12;
+    if (pl.nop > 1 && pl.nop != nvec)
+        return fail(PNED_E_ARG, "a batch context's vectors need one column per operator");
     const int npass = (int)pl.passes.size();
     const int grid_cap = 4096;
     // partial layout: [pass][block][2*ncol]
@@ -362,6 +408,10 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         a.fresh = fresh ? 1 : 0;
         a.ncol = nvec;
         a.shift = -(double)ctx->ibond * eperbd;
+        a.nop = pl.nop;
+        a.colcoef = ctx->d_colcoef;
+        a.coldiag = ctx->d_coldiag;
+        a.colscale = d_colscale;
         const int TPB = tpb[p], R = rr[p];
         int grid = (int)((a.tile_count < (uint64_t)grid_cap) ? a.tile_count : (uint64_t)grid_cap);
         double *part = ctx->d_partial + (size_t)total_rows * 2 * nvec;

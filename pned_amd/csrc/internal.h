@@ -71,7 +71,20 @@ struct S12Plan {
     double c0r = 0, c0i = 0;     // constant part of the diagonal
     std::vector<PassPlan> passes;
     std::vector<DevTerm> devterms;
+    // batch contexts: one operator per column (same cluster, different couplings)
+    int nop = 1;
+    std::vector<double> colcoef;  // [term][nop][4] (c0r,c0i,c1r,c1i), plan.terms order
+    std::vector<double> coldiag;  // [nop][ngroups+1][2]: group weights, then the constant
+    std::vector<double> devcoef;  // [devterm][nop][4], pass order
 };
+
+// batch decomposition: nop operators over one cluster -> common term structure and
+// diagonal groups (the common refinement of every operator's weight classes)
+int s12_decompose_batch(int n, int ibond, const int32_t *ipair, const int32_t *btype, int nop,
+                        const double *const *xl, const int32_t *const *ll, S12Plan &plan);
+// replace the coefficients of one column; the operator must fit the existing structure
+int s12_set_column(S12Plan &plan, int ibond, const int32_t *ipair, const int32_t *btype, int col,
+                   const double *xl, const int32_t *ll);
 
 // build the S=1/2 decomposition of the reference tables (no passes yet)
 int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype,
@@ -128,6 +141,8 @@ struct pned_ctx {
     size_t d_terms_cap = 0;
     pned::DiagDev *d_diag = nullptr;
     void *pd_dev = nullptr;          // per-pass address tables of the current plan
+    double *d_colcoef = nullptr;     // batch contexts: per-column term coefficients
+    double *d_coldiag = nullptr;     // batch contexts: per-column diagonal weights
     int pd_threads = -1;
     // spin-S
     pned::GpuTables spin;
@@ -151,10 +166,13 @@ int ensure_partial(pned_ctx *ctx, size_t ndoubles);
 int ensure_result(pned_ctx *ctx, size_t ndoubles);
 int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
+int upload_batch_coefficients(pned_ctx *ctx);
 
 // kernels (launchers), all asynchronous on ctx->stream unless stated
+// d_colscale: optional device array of per-column factors s_c: v0 += (H - shift)(s_c v1_c);
+// prdct is then <v1|H|s v1> (one factor s)
 int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
-                  double *prdct_host);
+                  double *prdct_host, const double *d_colscale = nullptr);
 int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                    double *prdct_host);
 
@@ -168,9 +186,13 @@ int scale(pned_ctx *ctx, pned_vec *x, const double *a);
 int lanczos_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double *alpha,
                    const double *beta);
 int tps_shift_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, const double *xn);
+int lanczos_slots(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double *alpha, const double *beta,
+                  const int *mode);
+int gaussian_column(pned_ctx *ctx, pned_vec *v, int col, uint64_t seed);
 int gaussian_fill(pned_ctx *ctx, pned_vec *v, uint64_t seed);
 int sz_q_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *q, const double *site_vec);
 int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, double *out);
+int tps_fused_reduce(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, const pned_vec *w, double *out);
 int renbun_gpu(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,
                const double *salpha, const double *sbeta, double *w, double *dsf, int *conv);
 

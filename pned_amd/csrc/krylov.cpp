@@ -299,6 +299,31 @@ int pned_cocg(pned_ctx *ctx, double e0, double eta, const pned_vec *b, pned_vec 
     return fail(PNED_E_NOCONV, "cocg did not reach the tolerance");
 }
 
+// One Lanczos step for every slot of a batch (serving): per-column update by mode, one
+// batched H.v, per-column alpha and beta.  Idle columns return alpha = beta = 0.
+int pned_lanczos_batch_step(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int *mode, const double *alpha_prev,
+                            const double *beta_prev, double *alpha, double *beta)
+{
+    if (!ctx || !v1 || !v0 || !mode || !alpha_prev || !beta_prev || !alpha || !beta)
+        return fail(PNED_E_ARG, "null argument");
+    const int nc = v1->nvec;
+    PNED_TRY(lanczos_slots(ctx, v1, v0, alpha_prev, beta_prev, mode));
+    std::vector<double> pr(2 * nc), b2(nc);
+    PNED_TRY(pned_p_mltply_v3(ctx, v1, v0, 0.0, pr.data()));
+    PNED_TRY(pned_calc_beta(ctx, pr.data(), v1, v0, b2.data()));
+    for (int c = 0; c < nc; ++c) {
+        alpha[c] = mode[c] ? pr[2 * c] : 0.0;
+        beta[c] = mode[c] ? std::sqrt(b2[c]) : 0.0;
+    }
+    return PNED_OK;
+}
+
+int pned_set_initial_column(pned_ctx *ctx, pned_vec *v, int col, uint64_t seed)
+{
+    if (!ctx || !v) return fail(PNED_E_ARG, "null argument");
+    return gaussian_column(ctx, v, col, seed);
+}
+
 int pned_check1(pned_ctx *ctx, const pned_vec *x, pned_vec *work, double *expect, double *ratios, int max_ratios,
                 int *nratios)
 {
@@ -384,6 +409,57 @@ int pned_tps(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, pned_vec *
             for (int k = 0; k < nsq; ++k) row[7 + k] = o[6 + 2 * k];
         }
     }
+    return PNED_OK;
+}
+
+// The same recursion and coef rows as pned_tps with the vector work fused: vectors are
+// rotated instead of copied (vg <- v1 <- v0 by handle), the 1/beta normalisation rides
+// on the next H.v as a per-column input scale, and every reduction of a step is one pass
+// over (vg, v1, v0).  Per step: the H.v passes + 3 vector reads, instead of the
+// reference's update (2 reads, 3 writes), two dots and the observable passes.
+int pned_tps_fused(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, double lmax, int nsteps, double *alpha,
+                   double *beta, double *coef)
+{
+    if (!ctx || !v1 || !va || !vb || !alpha || !beta || !coef) return fail(PNED_E_ARG, "null argument");
+    if (ctx->site_vec.empty()) return fail(PNED_E_ARG, "call pned_tps_set_observables first");
+    if (ctx->q != 2) return fail(PNED_E_UNSUPPORTED, "fused mTPQ is S=1/2");
+    const int nc = v1->nvec, nsq = ctx->nsq, ncoef = 7 + nsq, width = 10 + 2 * nsq;
+    pned_vec *u = v1, *up = va, *w = vb;
+    PNED_TRY(pned_vec_zero(ctx, up));
+    std::vector<double> su(nc, 1.0), sp(nc, 0.0), pr(2 * nc), red((size_t)width * nc);
+    double *dscale = nullptr;
+    for (int i = 1; i <= nsteps; ++i) {
+        PNED_TRY(upload_scalars(ctx, su.data(), nc, &dscale));
+        PNED_TRY(launch_hv_s12(ctx, u, w, 0.0, true, pr.data(), dscale));   // w = (L-H)(su u) = v0
+        PNED_TRY(tps_fused_reduce(ctx, up, u, w, red.data()));
+        for (int c = 0; c < nc; ++c) {
+            const double *o = red.data() + (size_t)c * width;
+            const double a1 = su[c] * pr[2 * c];            // <v1|(L-H)|v1>
+            const double b1 = std::sqrt(o[0]);                // ||v0||
+            alpha[(size_t)(i - 1) * nc + c] = lmax - a1;
+            beta[(size_t)(i - 1) * nc + c] = b1;
+            double *row = coef + ((size_t)(i - 1) * nc + c) * ncoef;
+            const double gv = sp[c] * su[c], vv = su[c] * su[c];
+            row[0] = (double)(2 * i) / a1;
+            row[1] = lmax;
+            row[2] = b1;
+            row[3] = lmax - a1;
+            row[4] = lmax * gv * o[4] - sp[c] * o[2];        // Re(l <vg|v1> - <vg|v0>)
+            row[5] = vv * o[6];
+            row[6] = gv * o[8];
+            for (int k = 0; k < nsq; ++k) {
+                const bool left_g = (k % 4) % 2 == 1;
+                row[7 + k] = (left_g ? gv : vv) * o[10 + 2 * k];
+            }
+            sp[c] = su[c];
+            su[c] = 1.0 / b1;
+        }
+        pned_vec *t = up;   // vg <- v1, v1 <- v0, v0 <- free
+        up = u;
+        u = w;
+        w = t;
+    }
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
     return PNED_OK;
 }
 

@@ -186,6 +186,76 @@ __global__ void k_lanczos_update(typename V2<Real>::T *__restrict__ v1, typename
     }
 }
 
+__device__ __forceinline__ void philox(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
+                                       uint32_t k1);
+
+// per-column Lanczos update for slot batches: mode 0 idle (v1 = v0 = 0), 1 start (v0 = 0,
+// v1 holds the start vector), 2 step (the update above with that column's alpha, beta)
+template <typename Real>
+__global__ void k_lanczos_slots(typename V2<Real>::T *__restrict__ v1, typename V2<Real>::T *__restrict__ v0,
+                                const double *__restrict__ ab, const int *__restrict__ mode, size_t nelem,
+                                int ncol)
+{
+    const size_t stride = (size_t)blockDim.x * gridDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += stride) {
+        const int c = (int)(e % (size_t)ncol);
+        const int m = mode[c];
+        if (m == 1) {
+            st(v0, e, 0.0, 0.0);
+        } else if (m == 2) {
+            const double al = ab[2 * c], be = ab[2 * c + 1];
+            double xr, xi, yr, yi;
+            ld(v1, e, xr, xi);
+            ld(v0, e, yr, yi);
+            st(v0, e, -be * xr, -be * xi);
+            st(v1, e, (yr - al * xr) / be, (yi - al * xi) / be);
+        } else {
+            st(v1, e, 0.0, 0.0);
+            st(v0, e, 0.0, 0.0);
+        }
+    }
+}
+
+// Gaussian start vector of one column: values depend on (seed, state) only, not on
+// the column a request occupies; returns the column's norm^2 partials
+template <typename Real>
+__global__ __launch_bounds__(RED_TPB) void k_gauss_col(typename V2<Real>::T *__restrict__ v, uint64_t seed,
+                                                       size_t dim, int ncol, int col, double *__restrict__ partial)
+{
+    __shared__ double red[RED_TPB];
+    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    double acc[1] = {0.0};
+    for (size_t i = (size_t)blockIdx.x * RED_TPB + threadIdx.x; i < dim; i += stride) {
+        uint32_t c0 = (uint32_t)i, c1 = (uint32_t)(i >> 32), c2 = 0u, c3 = 0u;
+        philox(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const uint64_t aa = ((uint64_t)c0 << 21) ^ (c1 >> 11);
+        const uint64_t bb = ((uint64_t)c2 << 21) ^ (c3 >> 11);
+        const double u1 = ((double)(aa & ((1ull << 53) - 1)) + 1.0) * 0x1.0p-53;
+        const double u2 = (double)(bb & ((1ull << 53) - 1)) * 0x1.0p-53;
+        const double r = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincos(6.283185307179586476925286766559 * u2, &sn, &cs);
+        const size_t e = i * (size_t)ncol + col;
+        st(v, e, r * cs, r * sn);
+        double qr, qi;
+        ld(v, e, qr, qi);
+        acc[0] = fma(qr, qr, fma(qi, qi, acc[0]));
+    }
+    block_reduce_cols<1>(red, acc, 1, partial + blockIdx.x);
+}
+
+template <typename Real>
+__global__ void k_scale_col(typename V2<Real>::T *__restrict__ v, double s, size_t dim, int ncol, int col)
+{
+    const size_t stride = (size_t)blockDim.x * gridDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+        const size_t e = i * (size_t)ncol + col;
+        double xr, xi;
+        ld(v, e, xr, xi);
+        st(v, e, s * xr, s * xi);
+    }
+}
+
 // mTPQ update (Hybrid_TPS_TPS.f90:380-386): vg = v1; v1 = xnorm*v0; v0 = 0
 template <typename Real>
 __global__ void k_tps_update(typename V2<Real>::T *__restrict__ v1, typename V2<Real>::T *__restrict__ v0,
@@ -346,6 +416,79 @@ __global__ __launch_bounds__(RED_TPB) void k_tps_obs(const typename V2<Real>::T 
     }
     (void)nout;
     // per-column block reduction of NOUT values
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) red[NOUT * tid + k] = acc[k];
+    __syncthreads();
+    for (int s2 = RED_TPB / 2; s2 >= ncol && s2 > 0; s2 >>= 1) {
+        if (tid < s2) {
+#pragma unroll
+            for (int k = 0; k < NOUT; ++k) red[NOUT * tid + k] += red[NOUT * (tid + s2) + k];
+        }
+        __syncthreads();
+    }
+    if (tid < ncol) {
+#pragma unroll
+        for (int k = 0; k < NOUT; ++k)
+            partial[(size_t)blockIdx.x * NOUT * ncol + NOUT * tid + k] = red[NOUT * tid + k];
+    }
+}
+
+// Fused mTPQ step reductions over (uprev, u, w), per column:
+//  [0] |w|^2  [2..3] <uprev|w>  [4..5] <uprev|u>  [6] <u|Wzz|u>  [8..9] <uprev|Wzz|u>
+//  [10+2k..] <a_k|W_k|u> (a_k = u or uprev) for the stored structure-factor entries
+template <typename Real, int NOUT>
+__global__ __launch_bounds__(RED_TPB) void k_tps_fused(const typename V2<Real>::T *__restrict__ up,
+                                                       const typename V2<Real>::T *__restrict__ u,
+                                                       const typename V2<Real>::T *__restrict__ w,
+                                                       const ObsArgs *__restrict__ ap, size_t nelem, int ncol,
+                                                       double *__restrict__ partial)
+{
+    extern __shared__ double red[];
+    const ObsArgs &a = *ap;
+    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    double acc[NOUT];
+#pragma unroll
+    for (int k = 0; k < NOUT; ++k) acc[k] = 0.0;
+    for (size_t e = (size_t)blockIdx.x * RED_TPB + threadIdx.x; e < nelem; e += stride) {
+        const uint32_t s = (uint32_t)(e / (size_t)ncol);
+        double pr, pi, xr, xi, wr, wi;
+        ld(up, e, pr, pi);
+        ld(u, e, xr, xi);
+        ld(w, e, wr, wi);
+        acc[0] = fma(wr, wr, fma(wi, wi, acc[0]));
+        acc[2] = fma(pr, wr, fma(pi, wi, acc[2]));
+        acc[3] = fma(pr, wi, fma(-pi, wr, acc[3]));
+        acc[4] = fma(pr, xr, fma(pi, xi, acc[4]));
+        acc[5] = fma(pr, xi, fma(-pi, xr, acc[5]));
+        double wzz = 0.0;
+        for (int b = 0; b < a.nzz; ++b) {
+            const double s3 = (s & a.zz_mask1[b]) ? 0.5 : -0.5;
+            const double s4 = (s & a.zz_mask2[b]) ? 0.5 : -0.5;
+            wzz = fma(s3, s4, wzz);
+        }
+        acc[6] = fma(wzz, xr * xr + xi * xi, acc[6]);
+        acc[8] = fma(wzz, pr * xr + pi * xi, acc[8]);
+        acc[9] = fma(wzz, pr * xi - pi * xr, acc[9]);
+        const int s0 = (int)(s & 1u);
+        for (int k = 0; k < a.nsq && 10 + 2 * k + 1 < NOUT; ++k) {
+            const int qs = a.sq_q[k];
+            double cr = 0.0, ci = 0.0;
+            for (int j = 0; j < a.n; ++j) {
+                const int sj = (int)((s >> j) & 1u);
+                double wt;
+                if (a.sq_mode[k] == 0) wt = ((double)sj - 0.5) * ((double)s0 - 0.5);
+                else wt = (sj != s0) ? 0.25 : 0.0;
+                cr = fma(wt, a.phr[qs][j], cr);
+                ci = fma(wt, a.phi[qs][j], ci);
+            }
+            const double yr = cr * xr - ci * xi, yi = cr * xi + ci * xr;
+            double lr = xr, li = xi;
+            if (a.sq_left[k]) { lr = pr; li = pi; }
+            acc[10 + 2 * k] = fma(lr, yr, fma(li, yi, acc[10 + 2 * k]));
+            acc[10 + 2 * k + 1] = fma(lr, yi, fma(-li, yr, acc[10 + 2 * k + 1]));
+        }
+    }
     const int tid = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < NOUT; ++k) red[NOUT * tid + k] = acc[k];
@@ -546,6 +689,61 @@ int lanczos_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double *alph
     return PNED_OK;
 }
 
+int lanczos_slots(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double *alpha, const double *beta,
+                  const int *mode)
+{
+    PNED_TRY(same_shape(v1, v0));
+    const int nc = v1->nvec;
+    const size_t nelem = (size_t)v1->dim * nc;
+    // scalars: [alpha,beta]*nc then the modes as doubles (converted on upload)
+    std::vector<double> buf(2 * nc + nc);
+    for (int c = 0; c < nc; ++c) {
+        buf[2 * c] = alpha[c];
+        buf[2 * c + 1] = beta[c];
+        if (mode[c] == 2 && !(beta[c] > 0.0)) return fail(PNED_E_ARG, "stepping a column with beta <= 0");
+    }
+    std::vector<int> modes(mode, mode + nc);
+    std::memcpy(buf.data() + 2 * nc, modes.data(), sizeof(int) * nc);
+    double *d;
+    PNED_TRY(upload_scalars(ctx, buf.data(), (int)buf.size(), &d));
+    const int *dm = reinterpret_cast<const int *>(d + 2 * nc);
+    const int grid = grid_for(nelem, 256);
+    if (v1->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_lanczos_slots<double>, dim3(grid), dim3(256), 0, ctx->stream, (double2 *)v1->data,
+                           (double2 *)v0->data, (const double *)d, dm, nelem, nc);
+    else
+        hipLaunchKernelGGL(k_lanczos_slots<float>, dim3(grid), dim3(256), 0, ctx->stream, (float2 *)v1->data,
+                           (float2 *)v0->data, (const double *)d, dm, nelem, nc);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
+int gaussian_column(pned_ctx *ctx, pned_vec *v, int col, uint64_t seed)
+{
+    if (col < 0 || col >= v->nvec) return fail(PNED_E_ARG, "column outside the vector");
+    const int grid = RED_GRID;
+    PNED_TRY(ensure_partial(ctx, (size_t)grid));
+    if (v->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_gauss_col<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (double2 *)v->data, seed,
+                           (size_t)v->dim, v->nvec, col, ctx->d_partial);
+    else
+        hipLaunchKernelGGL(k_gauss_col<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (float2 *)v->data, seed,
+                           (size_t)v->dim, v->nvec, col, ctx->d_partial);
+    PNED_HIP(hipGetLastError());
+    double nrm;
+    PNED_TRY(finish_reduce(ctx, grid, 1, &nrm));
+    const double sc = 1.0 / std::sqrt(nrm);
+    const int g2 = grid_for((size_t)v->dim, 256);
+    if (v->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_scale_col<double>, dim3(g2), dim3(256), 0, ctx->stream, (double2 *)v->data, sc,
+                           (size_t)v->dim, v->nvec, col);
+    else
+        hipLaunchKernelGGL(k_scale_col<float>, dim3(g2), dim3(256), 0, ctx->stream, (float2 *)v->data, sc,
+                           (size_t)v->dim, v->nvec, col);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
 int tps_shift_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, const double *xn)
 {
     PNED_TRY(same_shape(v1, v0));
@@ -609,13 +807,10 @@ int sz_q_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, 
     return PNED_OK;
 }
 
-// out layout per column: [vg.v1 (2)] [v1 Wzz v1 (2)] [vg Wzz v1 (2)] [sq entries (2 each)]
-int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, double *out)
+static int build_obs_args(pned_ctx *ctx, ObsArgs &a)
 {
-    PNED_TRY(same_shape(vg, v1));
-    PNED_TRY(check_pow2_cols(v1->nvec));
     if (ctx->q != 2) return fail(PNED_E_UNSUPPORTED, "mTPQ observables are defined for S=1/2");
-    ObsArgs a{};
+    a = ObsArgs{};
     a.n = ctx->n;
     a.nzz = 0;
     for (int b = 0; b < ctx->ibond; ++b) {
@@ -646,6 +841,17 @@ int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, doubl
             a.phr[s][j] = std::cos(qn);
             a.phi[s][j] = std::sin(qn);
         }
+    return PNED_OK;
+}
+
+// out layout per column: [vg.v1 (2)] [v1 Wzz v1 (2)] [vg Wzz v1 (2)] [sq entries (2 each)]
+int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, double *out)
+{
+    PNED_TRY(same_shape(vg, v1));
+    PNED_TRY(check_pow2_cols(v1->nvec));
+    ObsArgs a;
+    PNED_TRY(build_obs_args(ctx, a));
+    const int nsq = a.nsq;
     constexpr int NOUT = 6 + 2 * OBS_MAX_SQ;   // fixed instance; unused slots stay zero
     const int width = (6 + 2 * nsq);
     const size_t nelem = (size_t)v1->dim * v1->nvec;
@@ -673,6 +879,48 @@ int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, doubl
     std::vector<double> all((size_t)NOUT * v1->nvec);
     PNED_TRY(finish_reduce(ctx, grid, NOUT * v1->nvec, all.data()));
     for (int c = 0; c < v1->nvec; ++c)
+        for (int k = 0; k < width; ++k) out[(size_t)c * width + k] = all[(size_t)c * NOUT + k];
+    return PNED_OK;
+}
+
+// out per column: the 10 + 2*nsq values of k_tps_fused
+int tps_fused_reduce(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, const pned_vec *w, double *out)
+{
+    PNED_TRY(same_shape(up, u));
+    PNED_TRY(same_shape(u, w));
+    PNED_TRY(check_pow2_cols(u->nvec));
+    ObsArgs a;
+    PNED_TRY(build_obs_args(ctx, a));
+    const int nsq = a.nsq;
+    constexpr int NOUT = 10 + 2 * 8;
+    if (10 + 2 * nsq > NOUT) return fail(PNED_E_UNSUPPORTED, "fused mTPQ reduction: at most 8 structure-factor entries");
+    const int width = 10 + 2 * nsq;
+    const size_t nelem = (size_t)u->dim * u->nvec;
+    const int grid = RED_GRID / 4;
+    PNED_TRY(ensure_partial(ctx, (size_t)grid * NOUT * u->nvec));
+    double *dargs;
+    const int nd = (int)((sizeof(ObsArgs) + sizeof(double) - 1) / sizeof(double));
+    std::vector<double> hbuf(nd);
+    std::memcpy(hbuf.data(), &a, sizeof(ObsArgs));
+    PNED_TRY(upload_scalars(ctx, hbuf.data(), nd, &dargs));
+    const size_t lds = sizeof(double) * NOUT * RED_TPB;
+    if (u->dtype == PNED_C128) {
+        auto k = k_tps_fused<double, NOUT>;
+        PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k, dim3(grid), dim3(RED_TPB), lds, ctx->stream, (const double2 *)up->data,
+                           (const double2 *)u->data, (const double2 *)w->data, (const ObsArgs *)dargs, nelem,
+                           u->nvec, ctx->d_partial);
+    } else {
+        auto k = k_tps_fused<float, NOUT>;
+        PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k, dim3(grid), dim3(RED_TPB), lds, ctx->stream, (const float2 *)up->data,
+                           (const float2 *)u->data, (const float2 *)w->data, (const ObsArgs *)dargs, nelem,
+                           u->nvec, ctx->d_partial);
+    }
+    PNED_HIP(hipGetLastError());
+    std::vector<double> all((size_t)NOUT * u->nvec);
+    PNED_TRY(finish_reduce(ctx, grid, NOUT * u->nvec, all.data()));
+    for (int c = 0; c < u->nvec; ++c)
         for (int k = 0; k < width; ++k) out[(size_t)c * width + k] = all[(size_t)c * NOUT + k];
     return PNED_OK;
 }

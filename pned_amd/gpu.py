@@ -84,9 +84,14 @@ def lib():
     L.pned_residual.argtypes = [P, P, f64, P, C.POINTER(f64)]
     L.pned_tps_set_observables.argtypes = [P, _dp, i32, _dp]
     L.pned_tps.argtypes = [P, P, P, P, P, f64, i32, _dp, _dp, _dp]
+    L.pned_tps_fused.argtypes = [P, P, P, P, f64, i32, _dp, _dp, _dp]
     L.pned_intalv_zz.argtypes = [P, P, P, _dp, _dp, C.POINTER(f64)]
     L.pned_dsflnc1z.argtypes = [P, P, P, f64, i32, _dp, _dp]
     L.pned_renbun.argtypes = [P, i32, i32, f64, f64, f64, _dp, _dp, f64, _dp, _dp, _ip]
+    L.pned_ctx_create_batch.argtypes = [i32, i32, i32, _ip, _ip, i32, _dp, _ip, i32, C.POINTER(P)]
+    L.pned_ctx_set_column_matrices.argtypes = [P, i32, _dp, _ip]
+    L.pned_lanczos_batch_step.argtypes = [P, P, P, _ip, _dp, _dp, _dp, _dp]
+    L.pned_set_initial_column.argtypes = [P, P, i32, u64]
     L.pned_s12_plan_export.argtypes = [i32, i32, _ip, _ip, _dp, _ip, i32, i32, i32, C.POINTER(i32),
                                        C.POINTER(i32), C.POINTER(i32), _ip, _ip, _dp,
                                        np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), _dp, i32, i32]
@@ -154,16 +159,43 @@ class Vector:
 class Context:
     """One operator on one GPU: the reference's data_mem tables handed to the device."""
 
-    def __init__(self, op, device: int = 0):
+    def __init__(self, op, device: int = 0, batch_ops=None):
+        """``batch_ops``: a list of operators on the same cluster (power-of-two length),
+        one per column of this context's vectors (pned_ctx_create_batch)."""
         self.op = op
         self.n, self.q, self.ibond = op.n, op.q, op.ibond
         h = C.c_void_p()
-        check(lib().pned_ctx_create(op.n, op.q, op.ibond, np.ascontiguousarray(op.ipair, dtype=np.int32),
-                                    np.ascontiguousarray(op.ibond_type, dtype=np.int32),
-                                    np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
-                                    op.fortran_llocal(), device, C.byref(h)))
+        ipair = np.ascontiguousarray(op.ipair, dtype=np.int32)
+        btype = np.ascontiguousarray(op.ibond_type, dtype=np.int32)
+        if batch_ops is None:
+            check(lib().pned_ctx_create(op.n, op.q, op.ibond, ipair, btype,
+                                        np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
+                                        op.fortran_llocal(), device, C.byref(h)))
+            self.nop = 1
+        else:
+            xl = np.ascontiguousarray(np.concatenate([o.fortran_xlocal().view(np.float64) for o in batch_ops]))
+            ll = np.ascontiguousarray(np.concatenate([o.fortran_llocal() for o in batch_ops]))
+            check(lib().pned_ctx_create_batch(op.n, op.q, op.ibond, ipair, btype, len(batch_ops), xl, ll, device,
+                                              C.byref(h)))
+            self.nop = len(batch_ops)
         self.h = h
         self.dim = int(lib().pned_ctx_dim(h))
+
+    def set_column_matrices(self, col: int, op):
+        check(lib().pned_ctx_set_column_matrices(self.h, col, np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
+                                                 op.fortran_llocal()))
+
+    def set_initial_column(self, v: "Vector", col: int, seed: int):
+        check(lib().pned_set_initial_column(self.h, v.h, col, seed))
+
+    def lanczos_batch_step(self, v1: "Vector", v0: "Vector", mode, alpha_prev, beta_prev):
+        nc = v1.nvec
+        a = np.zeros(nc)
+        b = np.zeros(nc)
+        check(lib().pned_lanczos_batch_step(self.h, v1.h, v0.h, np.ascontiguousarray(mode, dtype=np.int32),
+                                            np.ascontiguousarray(alpha_prev, dtype=float),
+                                            np.ascontiguousarray(beta_prev, dtype=float), a, b))
+        return a, b
 
     # -- management -------------------------------------------------------------------
     def vector(self, nvec: int = 1, dtype: int = C128) -> Vector:
@@ -311,6 +343,15 @@ class Context:
         beta = np.zeros(nsteps * nc)
         coef = np.zeros(nsteps * nc * ncoef)
         check(lib().pned_tps(self.h, v1.h, v0.h, vg.h, vt.h, lmax, nsteps, alpha, beta, coef))
+        return alpha.reshape(nsteps, nc), beta.reshape(nsteps, nc), coef.reshape(nsteps, nc, ncoef)
+
+    def tps_fused(self, v1: Vector, va: Vector, vb: Vector, lmax: float, nsteps: int):
+        nc = v1.nvec
+        ncoef = 7 + getattr(self, "_nsq", 0)
+        alpha = np.zeros(nsteps * nc)
+        beta = np.zeros(nsteps * nc)
+        coef = np.zeros(nsteps * nc * ncoef)
+        check(lib().pned_tps_fused(self.h, v1.h, va.h, vb.h, lmax, nsteps, alpha, beta, coef))
         return alpha.reshape(nsteps, nc), beta.reshape(nsteps, nc), coef.reshape(nsteps, nc, ncoef)
 
     def intalv_zz(self, x: Vector, v: Vector, q, site_vec) -> float:

@@ -288,3 +288,24 @@ def test_fortran_host_driver(tmp_path):
     want = np.zeros_like(x)
     p_ref = oracle.hv(oracle.OracleOp(op), x, want)
     assert abs(pr - p_ref) <= 1e-12 * abs(p_ref)
+
+
+@pytest.mark.parametrize("dtype,tol", [(gpu.C128, 1e-10), (gpu.C64, 2e-4)])
+def test_tps_fused_matches_oracle(dtype, tol):
+    _say("tps fused", dtype)
+    cl = lattice.reference_cluster(12)
+    op = model.build_operator(cl, 20, 90.0, tps=True)
+    ctx = gpu.Context(op)
+    ctx.tps_set_observables(cl.site_vec, cl.sq_momenta())
+    nvec = 8
+    v1, va, vb = (ctx.vector(nvec, dtype) for _ in range(3))
+    ctx.set_initial_vec(v1, 21)
+    X = v1.download()
+    steps = 25
+    al, be, co = ctx.tps_fused(v1, va, vb, op.shift_L, steps)
+    for c in (0, 5):
+        x = np.ascontiguousarray(X[:, c]).astype(np.complex128)
+        a_ref, b_ref, c_ref = oracle.tps(oracle.OracleOp(op), cl.site_vec, cl.sq_momenta(), op.shift_L, steps, x)
+        assert np.allclose(al[:, c], a_ref, rtol=tol), c
+        assert np.allclose(be[:, c], b_ref, rtol=tol), c
+        assert np.allclose(co[:, c, :], c_ref, rtol=max(tol, 1e-9), atol=1e-9 if dtype == gpu.C128 else 1e-4), c

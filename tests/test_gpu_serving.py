@@ -1,0 +1,101 @@
+"""Batch contexts and the continuous-batching server (GPU, through the C ABI).
+
+* per-column operators: each column of a batched H.v equals the oracle H.v of its own
+  operator (1e-12 normwise);
+* batch invariance: a request's alpha/beta sequence is bit-identical whether its pool's
+  other slots are idle or busy with other requests;
+* serving: every result's E0 agrees with the oracle's Lanczos ground energy of the same
+  operator to 1e-10 relative, and its residual bound is below 1e-10.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from pned_amd import gpu, lattice, model, serving
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if gpu.device_count() < 1:
+        pytest.skip("no GPU")
+
+
+def test_batch_hv_per_column_operators():
+    _say("batch hv")
+    cl = lattice.get_cluster("2x4")
+    rng = np.random.default_rng(3)
+    ops = [model.build_operator(cl, 20, float(rng.uniform(0, 360)), field=float(rng.uniform(0, 0.5)))
+           for _ in range(4)]
+    ctx = gpu.Context(ops[0], batch_ops=ops)
+    d = ctx.dim
+    X = rng.standard_normal((d, 4)) + 1j * rng.standard_normal((d, 4))
+    Y = rng.standard_normal((d, 4)) + 1j * rng.standard_normal((d, 4))
+    v1, v0 = ctx.vector(4), ctx.vector(4)
+    v1.upload(X)
+    v0.upload(Y)
+    pr = ctx.p_mltply_v3(v1, v0, 0.03)
+    got = v0.download()
+    for c in range(4):
+        want = Y[:, c].copy()
+        p_ref = oracle.hv(oracle.OracleOp(ops[c]), np.ascontiguousarray(X[:, c]), want, eperbd=0.03)
+        assert np.linalg.norm(got[:, c] - want) <= 1e-12 * np.linalg.norm(want), c
+        assert abs(pr[c] - p_ref) <= 1e-13 * np.linalg.norm(X[:, c]) * np.linalg.norm(want - Y[:, c])
+    # replace one column's operator
+    new = model.build_operator(cl, 20, 201.0, field=0.37)
+    ctx.set_column_matrices(2, new)
+    ctx.hv_fresh(v1, v0)
+    want = np.zeros(d, complex)
+    oracle.hv(oracle.OracleOp(new), np.ascontiguousarray(X[:, 2]), want)
+    assert np.linalg.norm(v0.download()[:, 2] - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def test_alone_equals_in_the_mix():
+    _say("batch invariance")
+    reqs = serving.make_requests(8, ["3x3"], seed=5)
+    target = reqs[3]
+
+    def run(batch):
+        pool = serving.Pool("3x3", 8, 1e-10, 300)
+        for c, r in enumerate(batch):
+            if r is not None:
+                pool.admit(c, r, 0.0)
+        got = None
+        while pool.active():
+            for res in pool.step(lambda: 0.0):
+                if res.rid == target.rid:
+                    got = res
+        pool.close()
+        return got
+
+    alone = run([None, None, None, target, None, None, None, None])
+    mixed = run(reqs)
+    assert alone is not None and mixed is not None
+    assert alone.steps == mixed.steps
+    assert np.array_equal(alone.alpha, mixed.alpha)
+    assert np.array_equal(alone.beta, mixed.beta)
+    assert alone.e0 == mixed.e0
+
+
+def test_server_results_match_oracle():
+    _say("server")
+    reqs = serving.make_requests(10, ["2x4", "3x3", "2x5"], seed=11)
+    srv = serving.Server(hbm_budget_bytes=50e9, tol=1e-10)
+    res = srv.run(reqs)
+    assert len(res) == len(reqs)
+    for r, q in zip(res, reqs):
+        assert r.rid == q.rid
+        assert r.resid < 1e-10
+        op = serving.operator_for(q)
+        x = oracle.gaussian(op.dim, q.seed)
+        itr, a, b, ene = oracle.lanczos(oracle.OracleOp(op), x, max_itr=1500, nvec=1)
+        assert abs(r.e0 - ene[0]) <= 1e-10 * abs(ene[0]), (q, r.e0, ene[0])
