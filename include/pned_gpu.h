@@ -79,6 +79,8 @@ int pned_ctx_tune(pned_ctx *ctx, int tile_bits, int low_bits, int threads);
  * fastest for this context; v0 is overwritten.  Outputs may be NULL. */
 int pned_ctx_autotune(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, int reps, int *best_tile_bits,
                       int *best_low_bits, int *best_threads, double *best_ms);
+/* spin-S kernel choice: 1 = tiled passes (default, q = 3..5), 0 = per-state gather */
+int pned_ctx_spin_kernel(pned_ctx *ctx, int tiled);
 /* diagnostic: number of tile passes one H.v makes, and terms per pass (max 64) */
 int pned_ctx_plan_info(const pned_ctx *ctx, int *npasses, int *terms_per_pass, int max);
 /* diagnostic, needs no GPU: the S=1/2 term decomposition and tile passes the planner

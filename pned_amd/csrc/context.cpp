@@ -20,6 +20,7 @@ int fail(int code, const std::string &msg)
 }
 
 void reset_spin_tables(pned_ctx *ctx);
+void free_spin_plan(pned_ctx *ctx);
 
 // ------------------------------------------------------------------------------------
 // S=1/2 decomposition.  For every bond (sites a < b, local index t = 2*s_b + s_a) the
@@ -633,6 +634,7 @@ int pned_ctx_destroy(pned_ctx *ctx)
     if (ctx->spin.pairs) (void)hipFree(ctx->spin.pairs);
     if (ctx->spin.btype) (void)hipFree(ctx->spin.btype);
     reset_spin_tables(ctx);
+    free_spin_plan(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return PNED_OK;
@@ -658,6 +660,13 @@ int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t
     return PNED_OK;
 }
 
+int pned_ctx_spin_kernel(pned_ctx *ctx, int tiled)
+{
+    if (!ctx) return fail(PNED_E_ARG, "null context");
+    ctx->spin_tiled = tiled ? 1 : 0;
+    return PNED_OK;
+}
+
 int pned_ctx_tune(pned_ctx *ctx, int tile_bits, int low_bits, int threads)
 {
     if (!ctx) return fail(PNED_E_ARG, "null context");
@@ -667,6 +676,7 @@ int pned_ctx_tune(pned_ctx *ctx, int tile_bits, int low_bits, int threads)
     ctx->tune_low_bits = low_bits;
     ctx->tune_threads = threads;
     ctx->plan_colbits = -1;
+    free_spin_plan(ctx);
     return PNED_OK;
 }
 

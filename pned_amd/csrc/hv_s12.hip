@@ -96,6 +96,7 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
     using V = typename C2<Real>::T;
     extern __shared__ unsigned char smem_raw[];
     V *tile = reinterpret_cast<V *>(smem_raw);
+    const unsigned char *tbytes = smem_raw;  // partner slot j ^ m sits at byte (16 j) ^ (16 m)
     constexpr int CH = (R < 4) ? R : 4;      // states per compute chunk
     constexpr int NCH = R / CH;
     const int tid = threadIdx.x;
@@ -158,9 +159,10 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
                         });
                 }
             }
-            uint32_t st[CH];
+            uint32_t st[CH], jb[CH];
             double ar[CH], ai[CH];
             static_for<CH>([&](auto c) {
+                jb[c] = (uint32_t)((tid + TPB * (r0 + c)) * sizeof(V));   // LDS byte offset of the slot
                 st[c] = (uint32_t)((tb0 | offHi(r0 + c)) >> a.colbits);
                 ar[c] = 0.0;
                 ai[c] = 0.0;
@@ -197,8 +199,9 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
                     const double *cc = cbase + (size_t)(a.first_term + t) * a.nop * 4;
                     const double c0r = cc[0], c0i = cc[1];
                     const double dr = cc[2] - c0r, di = cc[3] - c0i;
+                    const uint32_t tmb = T.tmask * (uint32_t)sizeof(V);
                     static_for<CH>([&](auto c) {
-                        const V p = tile[(tid + TPB * (r0 + c)) ^ T.tmask];
+                        const V p = *reinterpret_cast<const V *>(tbytes + (jb[c] ^ tmb));
                         const double sel = (double)(__popc(st[c] & T.selmask) & 1);
                         const double cr = fma(sel, dr, c0r), ci = fma(sel, di, c0i);
                         ar[c] = fma(cr, (double)p.x, ar[c]);
@@ -212,8 +215,9 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
                 const DevTerm T = terms[a.first_term + t];
                 if (T.is_real) {
                     const double d = T.c1r - T.c0r;
+                    const uint32_t tmb = T.tmask * (uint32_t)sizeof(V);
                     static_for<CH>([&](auto c) {
-                        const V p = tile[(tid + TPB * (r0 + c)) ^ T.tmask];
+                        const V p = *reinterpret_cast<const V *>(tbytes + (jb[c] ^ tmb));
                         const double sel = (double)(__popc(st[c] & T.selmask) & 1);
                         const double cf = fma(sel, d, T.c0r);
                         ar[c] = fma(cf, (double)p.x, ar[c]);
@@ -221,8 +225,9 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
                     });
                 } else {
                     const double dr = T.c1r - T.c0r, di = T.c1i - T.c0i;
+                    const uint32_t tmb = T.tmask * (uint32_t)sizeof(V);
                     static_for<CH>([&](auto c) {
-                        const V p = tile[(tid + TPB * (r0 + c)) ^ T.tmask];
+                        const V p = *reinterpret_cast<const V *>(tbytes + (jb[c] ^ tmb));
                         const double sel = (double)(__popc(st[c] & T.selmask) & 1);
                         const double cr = fma(sel, dr, T.c0r), ci = fma(sel, di, T.c0i);
                         ar[c] = fma(cr, (double)p.x, ar[c]);

@@ -7,6 +7,9 @@
 // write conflicts, no atomics); v0 is read and written once.  Per (type, output local
 // state) the nonzero input states are precomputed on the host, so the inner loop visits
 // only the elements the reference does not skip (llocal_Ham).
+#include <algorithm>
+#include <cstring>
+
 #include "internal.h"
 
 namespace pned {
@@ -170,11 +173,16 @@ static int build_spin_tables(pned_ctx *ctx, SpinTables &out)
     return PNED_OK;
 }
 
+int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                         double *prdct_host);
+
 int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                    double *prdct_host)
 {
     if (v1->nvec != 1 || v1->dtype != PNED_C128)
         return fail(PNED_E_UNSUPPORTED, "spin-S H.v takes one complex128 column");
+    if (ctx->spin_tiled && ctx->q >= 3 && ctx->q <= 5)
+        return launch_hv_spin_tiled(ctx, v1, v0, eperbd, fresh, prdct_host);
     if (ctx->q > 15 || ctx->n > 32) return fail(PNED_E_UNSUPPORTED, "spin-S H.v: q <= 15, N <= 32");
     SpinTables tb;
     PNED_TRY(build_spin_tables(ctx, tb));
@@ -214,12 +222,305 @@ int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperb
     return PNED_OK;
 }
 
+// ------------------------------------------------------------------------------------
+// Tiled spin-S H.v.  A pass owns K tile sites (the lowest L digits always among them, so a
+// wave's lanes read contiguous runs); every bond with both sites in the tile is applied in
+// that pass from the tile staged in LDS.  The tile is the q^K states sharing the digits
+// of the other sites; tile-local index j = sum digit_k q^k over the tile sites in order.
+// Each pass reads v1 once and read-modify-writes v0 once (pass 0 may start from zero).
+// ------------------------------------------------------------------------------------
+constexpr int SP_MAX_TILE = 8192;     // states per tile (128 KB of complex128)
+constexpr int SP_MAX_BOND = 64;
+constexpr int SP_BASE_TAB = 4096;
+
+struct SpinPassDev {
+    uint64_t off[SP_MAX_TILE];        // memory offset of tile-local state j
+    uint64_t blo[SP_BASE_TAB];        // tile base, low outer digits
+    uint64_t bhi[SP_BASE_TAB];        // tile base, high outer digits
+    int32_t bond_t[SP_MAX_BOND];      // bond type (0-based)
+    int32_t bond_pa[SP_MAX_BOND];     // tile position of the min site
+    int32_t bond_pb[SP_MAX_BOND];     // tile position of the max site
+    int32_t bond_wa[SP_MAX_BOND];     // q^pa
+    int32_t bond_wb[SP_MAX_BOND];     // q^pb
+};
+
+struct SpinPassArgs {
+    uint64_t tile_count;
+    int tsz;                          // q^K
+    int K, nbond, diag_first, fresh;
+    int nlo_mod;                      // tile_count split: tix % nlo_mod, tix / nlo_mod
+    double shift;                     // -eperbd per bond diagonal
+};
+
+template <int Q, int TPB, int R>
+__global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ v1, double2 *__restrict__ v0,
+                                                    SpinTables tb, const SpinPassDev *__restrict__ pd,
+                                                    SpinPassArgs a, double *__restrict__ partial)
+{
+    __shared__ double2 tile[TPB * R];
+    __shared__ double red[2 * TPB];
+    constexpr int QQ = Q * Q;
+    const int tid = threadIdx.x;
+    const bool rd0 = !(a.diag_first && a.fresh);
+    double pr_re = 0.0, pr_im = 0.0;
+    for (uint64_t tix = blockIdx.x; tix < a.tile_count; tix += gridDim.x) {
+        const uint64_t base = pd->blo[tix % (uint64_t)a.nlo_mod] + pd->bhi[tix / (uint64_t)a.nlo_mod];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = tid + TPB * r;
+            if (j < a.tsz) tile[j] = v1[base + pd->off[j]];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = tid + TPB * r;
+            if (j >= a.tsz) continue;
+            const uint64_t m = base + pd->off[j];
+            double2 y = make_double2(0.0, 0.0);
+            if (rd0) y = v0[m];
+            // digits of j, 4 bits each
+            uint64_t dig = 0;
+            {
+                int x = j;
+                for (int k = 0; k < a.K; ++k) {
+                    dig |= (uint64_t)(x % Q) << (4 * k);
+                    x /= Q;
+                }
+            }
+            double ar = 0.0, ai = 0.0;
+            for (int b = 0; b < a.nbond; ++b) {
+                const int pa = pd->bond_pa[b], pb = pd->bond_pb[b];
+                const int d1 = (int)((dig >> (4 * pa)) & 15u), d2 = (int)((dig >> (4 * pb)) & 15u);
+                const int istat = Q * d2 + d1;
+                const int bt = pd->bond_t[b];
+                const int row = bt * QQ + istat;
+                const int cnt = tb.nzcnt[row];
+                const int wa = pd->bond_wa[b], wb = pd->bond_wb[b];
+                for (int k = 0; k < cnt; ++k) {
+                    const int ml = tb.nzin[row * QQ + k];
+                    double cr = tb.nzc[2 * (row * QQ + k)], ci = tb.nzc[2 * (row * QQ + k) + 1];
+                    if (ml == istat) cr += a.shift;
+                    const int pj = j + (ml % Q - d1) * wa + (ml / Q - d2) * wb;
+                    const double2 p = tile[pj];
+                    ar = fma(cr, p.x, fma(-ci, p.y, ar));
+                    ai = fma(cr, p.y, fma(ci, p.x, ai));
+                }
+            }
+            const double2 x = tile[j];
+            pr_re += x.x * ar + x.y * ai;
+            pr_im += x.x * ai - x.y * ar;
+            v0[m] = make_double2(y.x + ar, y.y + ai);
+        }
+        __syncthreads();
+    }
+    red[2 * tid] = pr_re;
+    red[2 * tid + 1] = pr_im;
+    __syncthreads();
+    for (int st = TPB / 2; st > 0; st >>= 1) {
+        if (tid < st) {
+            red[2 * tid] += red[2 * (tid + st)];
+            red[2 * tid + 1] += red[2 * (tid + st) + 1];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        partial[2 * blockIdx.x] = red[0];
+        partial[2 * blockIdx.x + 1] = red[1];
+    }
+}
+
+// greedy cover of the bonds by tile-site sets (lowest L sites forced)
+static std::vector<std::vector<int>> spin_cover(int n, const std::vector<int32_t> &sp, int ibond, int K, int L)
+{
+    std::vector<char> done(ibond, 0);
+    int remaining = ibond;
+    std::vector<std::vector<int>> out;
+    while (remaining > 0) {
+        std::vector<char> in(n, 0);
+        int cnt = 0;
+        for (int k = 0; k < L && k < n; ++k) { in[k] = 1; ++cnt; }
+        while (cnt < K && cnt < n) {
+            int best = -1;
+            long bs = -1;
+            for (int st = 0; st < n; ++st) {
+                if (in[st]) continue;
+                long newly = 0, touch = 0;
+                for (int b = 0; b < ibond; ++b) {
+                    if (done[b]) continue;
+                    const int x = sp[2 * b], y = sp[2 * b + 1];
+                    if (x != st && y != st) continue;
+                    ++touch;
+                    const int other = (x == st) ? y : x;
+                    if (in[other]) ++newly;
+                }
+                const long score = newly * (1L << 30) + touch * (1L << 10) + (n - st);
+                if (score > bs) { bs = score; best = st; }
+            }
+            if (best < 0) break;
+            in[best] = 1;
+            ++cnt;
+        }
+        std::vector<int> bonds;
+        for (int b = 0; b < ibond; ++b)
+            if (!done[b] && in[sp[2 * b]] && in[sp[2 * b + 1]]) { done[b] = 1; --remaining; bonds.push_back(b); }
+        if (bonds.empty()) return {};
+        std::vector<int> sites;
+        for (int st = 0; st < n; ++st)
+            if (in[st]) sites.push_back(st);
+        out.push_back(sites);
+        out.push_back(bonds);
+    }
+    return out;
+}
+
+int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                         double *prdct_host)
+{
+    const int q = ctx->q, n = ctx->n;
+    SpinTables tb;
+    PNED_TRY(build_spin_tables(ctx, tb));
+    // tile sites: the largest K with q^K <= 8192
+    int K = 0;
+    {
+        int64_t p = 1;
+        while (K < n && p * q <= SP_MAX_TILE) { p *= q; ++K; }
+    }
+    if (ctx->tune_tile_bits > 0 && ctx->tune_tile_bits < K) K = ctx->tune_tile_bits;
+    const int L = std::min(2, K);
+    if (ctx->spin_pd == nullptr || ctx->spin_K != K) {
+        auto cov = spin_cover(n, ctx->site_pair, ctx->ibond, K, L);
+        if (cov.empty()) return fail(PNED_E_UNSUPPORTED, "spin-S tile planner made no progress");
+        const int np = (int)cov.size() / 2;
+        std::vector<SpinPassDev> host(np);
+        std::vector<int> nb(np), tsz(np), nlo(np), nout(np);
+        for (int p = 0; p < np; ++p) {
+            const std::vector<int> &sites = cov[2 * p], &bonds = cov[2 * p + 1];
+            SpinPassDev &d = host[p];
+            std::memset(&d, 0, sizeof(d));
+            if ((int)bonds.size() > SP_MAX_BOND) return fail(PNED_E_UNSUPPORTED, "too many bonds in a spin-S pass");
+            std::vector<uint64_t> pw(n);
+            uint64_t x = 1;
+            for (int k = 0; k < n; ++k) { pw[k] = x; x *= (uint64_t)q; }
+            const int Kp = (int)sites.size();
+            int ts = 1;
+            for (int k = 0; k < Kp; ++k) ts *= q;
+            for (int j = 0; j < ts; ++j) {
+                uint64_t o = 0;
+                int y = j;
+                for (int k = 0; k < Kp; ++k) { o += (uint64_t)(y % q) * pw[sites[k]]; y /= q; }
+                d.off[j] = o;
+            }
+            std::vector<int> outer;
+            for (int st = 0; st < n; ++st)
+                if (std::find(sites.begin(), sites.end(), st) == sites.end()) outer.push_back(st);
+            const int no = (int)outer.size();
+            int nl = 0;
+            int64_t lm = 1;
+            while (nl < no && lm * q <= SP_BASE_TAB) { lm *= q; ++nl; }
+            int64_t hm = 1;
+            for (int k = nl; k < no; ++k) hm *= q;
+            if (hm > SP_BASE_TAB) return fail(PNED_E_UNSUPPORTED, "too many outer sites for the spin-S tables");
+            for (int64_t t2 = 0; t2 < lm; ++t2) {
+                uint64_t o = 0;
+                int64_t y = t2;
+                for (int k = 0; k < nl; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
+                d.blo[t2] = o;
+            }
+            for (int64_t t2 = 0; t2 < hm; ++t2) {
+                uint64_t o = 0;
+                int64_t y = t2;
+                for (int k = nl; k < no; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
+                d.bhi[t2] = o;
+            }
+            for (size_t b = 0; b < bonds.size(); ++b) {
+                const int bi = bonds[b];
+                const int sa = ctx->site_pair[2 * bi], sb = ctx->site_pair[2 * bi + 1];
+                const int pa = (int)(std::find(sites.begin(), sites.end(), sa) - sites.begin());
+                const int pbp = (int)(std::find(sites.begin(), sites.end(), sb) - sites.begin());
+                d.bond_t[b] = ctx->btype[bi] - 1;
+                d.bond_pa[b] = pa;
+                d.bond_pb[b] = pbp;
+                int wa = 1, wb = 1;
+                for (int k = 0; k < pa; ++k) wa *= q;
+                for (int k = 0; k < pbp; ++k) wb *= q;
+                d.bond_wa[b] = wa;
+                d.bond_wb[b] = wb;
+            }
+            nb[p] = (int)bonds.size();
+            tsz[p] = ts;
+            nlo[p] = (int)lm;
+            nout[p] = no;
+            (void)nout;
+        }
+        PNED_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->spin_pd) PNED_HIP(hipFree(ctx->spin_pd));
+        PNED_HIP(hipMalloc(&ctx->spin_pd, sizeof(SpinPassDev) * np));
+        PNED_HIP(hipMemcpy(ctx->spin_pd, host.data(), sizeof(SpinPassDev) * np, hipMemcpyHostToDevice));
+        ctx->spin_K = K;
+        ctx->spin_np = np;
+        ctx->spin_nb = nb;
+        ctx->spin_tsz = tsz;
+        ctx->spin_nlo = nlo;
+    }
+    const int np = ctx->spin_np;
+    const int grid_cap = 2048;
+    PNED_TRY(ensure_partial(ctx, 2 * (size_t)grid_cap * np + 16));
+    PNED_TRY(ensure_result(ctx, 8));
+    int rows = 0;
+    for (int p = 0; p < np; ++p) {
+        SpinPassArgs a{};
+        a.tsz = ctx->spin_tsz[p];
+        a.tile_count = (uint64_t)(ctx->dim / a.tsz);
+        a.K = 0;
+        for (int t = 1; t < a.tsz; t *= q) ++a.K;
+        a.nbond = ctx->spin_nb[p];
+        a.diag_first = (p == 0);
+        a.fresh = fresh ? 1 : 0;
+        a.nlo_mod = ctx->spin_nlo[p];
+        a.shift = -eperbd;
+        const int grid = (int)std::min<uint64_t>(a.tile_count, (uint64_t)grid_cap);
+        const SpinPassDev *pd = reinterpret_cast<const SpinPassDev *>(ctx->spin_pd) + p;
+        double *part = ctx->d_partial + 2 * (size_t)rows;
+        // 256 threads x 32 states covers 8192; pick R from the tile size
+        const int R = (a.tsz + 255) / 256;
+#define PNED_SPIN_CASE(QV, RV)                                                                                      \
+        if (q == QV && R <= RV) {                                                                                  \
+            hipLaunchKernelGGL((hv_spin_pass<QV, 256, RV>), dim3(grid), dim3(256), 0, ctx->stream,                  \
+                               (const double2 *)v1->data, (double2 *)v0->data, tb, pd, a, part);                   \
+        } else
+        PNED_SPIN_CASE(3, 8) PNED_SPIN_CASE(3, 16) PNED_SPIN_CASE(3, 32)
+        PNED_SPIN_CASE(4, 16) PNED_SPIN_CASE(4, 32)
+        PNED_SPIN_CASE(5, 32)
+        { return fail(PNED_E_UNSUPPORTED, "no tiled spin-S kernel instance for this q / tile"); }
+#undef PNED_SPIN_CASE
+        PNED_HIP(hipGetLastError());
+        rows += grid;
+    }
+    hipLaunchKernelGGL(reduce_two, dim3(1), dim3(64), 0, ctx->stream, ctx->d_partial, rows, ctx->d_result);
+    PNED_HIP(hipGetLastError());
+    if (prdct_host) {
+        PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+        PNED_HIP(hipStreamSynchronize(ctx->stream));
+        prdct_host[0] = ctx->h_result[0];
+        prdct_host[1] = ctx->h_result[1];
+    }
+    return PNED_OK;
+}
+
 // drop cached device tables (matrices changed)
 void reset_spin_tables(pned_ctx *ctx)
 {
     if (ctx->spin.mk) (void)hipFree(ctx->spin.mk);
     ctx->spin.mk = nullptr;
     ctx->spin.xl = nullptr;
+}
+
+void free_spin_plan(pned_ctx *ctx)
+{
+    if (ctx->spin_pd) (void)hipFree(ctx->spin_pd);
+    ctx->spin_pd = nullptr;
+    ctx->spin_K = -1;
 }
 
 }  // namespace pned

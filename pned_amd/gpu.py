@@ -51,6 +51,7 @@ def lib():
     L.pned_ctx_set_matrices.argtypes = [P, _dp, _ip]
     L.pned_ctx_tune.argtypes = [P, i32, i32, i32]
     L.pned_ctx_plan_info.argtypes = [P, C.POINTER(i32), _ip, i32]
+    L.pned_ctx_spin_kernel.argtypes = [P, i32]
     L.pned_ctx_autotune.argtypes = [P, P, P, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                     C.POINTER(f64)]
     L.pned_ctx_sync.argtypes = [P]
@@ -214,6 +215,9 @@ class Context:
         k, l, t, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         check(lib().pned_ctx_autotune(self.h, v1.h, v0.h, reps, C.byref(k), C.byref(l), C.byref(t), C.byref(ms)))
         return {"tile_bits": k.value, "low_bits": l.value, "threads": t.value, "ms": ms.value}
+
+    def spin_kernel(self, tiled: bool):
+        check(lib().pned_ctx_spin_kernel(self.h, 1 if tiled else 0))
 
     def plan_info(self):
         n = C.c_int()

@@ -242,7 +242,8 @@ def test_cocg_complex_shift_spin1():
     ctx.set_initial_vec(b, 4)
     bh = b.download()
     e0, eta = -3.0, 0.05
-    it, res = ctx.cocg(e0, eta, b, x, r, p, y, max_it=3000, tol=1e-10)
+    # a CPU run of the same recurrence needs about 3300 iterations at this shift
+    it, res = ctx.cocg(e0, eta, b, x, r, p, y, max_it=8000, tol=1e-10)
     assert res < 1e-10
     xs = x.download()
     # independent check: residual of the oracle's H.v, and a sparse direct solve
@@ -309,3 +310,22 @@ def test_tps_fused_matches_oracle(dtype, tol):
         assert np.allclose(al[:, c], a_ref, rtol=tol), c
         assert np.allclose(be[:, c], b_ref, rtol=tol), c
         assert np.allclose(co[:, c, :], c_ref, rtol=max(tol, 1e-9), atol=1e-9 if dtype == gpu.C128 else 1e-4), c
+
+
+def test_spin1_tiled_vs_gather_n18():
+    """cfg4 size (3^18 states): the tiled spin-S kernel against the per-state gather kernel
+    (both pinned to the oracle at N=8, 12 by test_hv_parity)."""
+    _say("spin1 tiled n18")
+    op = model.build_operator(lattice.reference_cluster(18), 20, 108.0, q=3)
+    ctx = gpu.Context(op)
+    v1, a, b = ctx.vector(), ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 7)
+    ctx.spin_kernel(True)
+    p1 = ctx.hv_fresh(v1, a, 0.02)
+    ctx.spin_kernel(False)
+    p2 = ctx.hv_fresh(v1, b, 0.02)
+    ctx.axpby(1.0, a, -1.0, b)           # b = a - b (axpby: y = a x + b y)
+    diff = np.sqrt(ctx.calc_vec_inprdct(b, b).real)
+    ref = np.sqrt(ctx.calc_vec_inprdct(a, a).real)
+    assert diff <= 1e-12 * ref
+    assert abs(p1 - p2) <= 1e-12 * abs(p2)
