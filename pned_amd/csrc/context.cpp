@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <set>
@@ -226,9 +227,16 @@ int s12_set_column(S12Plan &plan, int ibond, const int32_t *ipair, const int32_t
         c[2] = kv.second.c1r;
         c[3] = kv.second.c1i;
     }
-    // diagonal: every group must carry one weight for this operator
+    // diagonal: every group must carry one weight for this operator.  Weights that are
+    // equal in exact arithmetic can differ in the last bits (a site's weight is summed over
+    // its bonds in bond order, and the x/y/z bond contributions round differently), so a
+    // group accepts sites within a few ulps of the largest weight and uses its first site's.
     const size_t ng = plan.diag.size();
     std::vector<double> dw((ng + 1) * 2, 0.0);
+    double wscale = 0.0;
+    for (int k = 0; k < plan.n; ++k) wscale = std::max(wscale, std::hypot(rd.site_w[k].r, rd.site_w[k].i));
+    for (auto &kv : rd.pair_w) wscale = std::max(wscale, std::hypot(kv.second.r, kv.second.i));
+    const double wtol = 64.0 * std::numeric_limits<double>::epsilon() * wscale;
     std::vector<char> seen_site(plan.n, 0);
     std::map<std::pair<int, int>, char> seen_pair;
     for (size_t g = 0; g < ng; ++g) {
@@ -247,7 +255,8 @@ int s12_set_column(S12Plan &plan, int ibond, const int32_t *ipair, const int32_t
                 seen_pair[{k, k + G.shift}] = 1;
             }
             if (first) { w = x; first = false; }
-            else if (x.r != w.r || x.i != w.i) return fail(PNED_E_UNSUPPORTED, "diagonal weights do not fit the batch groups");
+            else if (std::hypot(x.r - w.r, x.i - w.i) > wtol)
+                return fail(PNED_E_UNSUPPORTED, "diagonal weights do not fit the batch groups");
         }
         dw[2 * g] = w.r;
         dw[2 * g + 1] = w.i;
