@@ -350,6 +350,56 @@ static std::vector<uint64_t> cover_once(const std::vector<uint64_t> &mm, int col
     return tiles;
 }
 
+// Local search for a cover with P tiles: every tile is the forced low bits plus K-L free
+// bits; a move swaps one free bit of one tile for a bit outside it, accepted by the
+// Metropolis rule on the number of flip masks no tile contains.  Starts from `init`
+// (P tiles), fixed seed, so the result is reproducible.
+static bool anneal_cover(const std::vector<uint64_t> &masks, int nb, int L, int K, std::vector<uint64_t> tiles,
+                         Lcg &rng, int iters, std::vector<uint64_t> &out)
+{
+    const int P = (int)tiles.size();
+    const int m = K - L;
+    if (P == 0 || m <= 0 || m >= nb - L) return false;
+    const size_t nm = masks.size();
+    std::vector<int> cov(nm, 0);
+    int unc = 0;
+    for (size_t i = 0; i < nm; ++i) {
+        for (int p = 0; p < P; ++p) cov[i] += ((masks[i] & ~tiles[p]) == 0);
+        unc += (cov[i] == 0);
+    }
+    if (unc == 0) { out = tiles; return true; }
+    std::vector<int> in_bits, out_bits;
+    for (int it = 0; it < iters; ++it) {
+        const double temp = 1.5 * (1.0 - (double)it / iters) + 0.02;
+        const int p = (int)(rng.next() % (uint32_t)P);
+        const uint64_t T = tiles[p];
+        in_bits.clear();
+        out_bits.clear();
+        for (int b = L; b < nb; ++b) (T >> b & 1 ? in_bits : out_bits).push_back(b);
+        const int a = in_bits[rng.next() % in_bits.size()];
+        const int c = out_bits[rng.next() % out_bits.size()];
+        const uint64_t T2 = (T & ~(1ull << a)) | (1ull << c);
+        int delta = 0;
+        for (size_t i = 0; i < nm; ++i) {
+            const int was = (masks[i] & ~T) == 0, now = (masks[i] & ~T2) == 0;
+            if (was == now) continue;
+            const int c2 = cov[i] - was + now;
+            delta += (c2 == 0) - (cov[i] == 0);
+        }
+        const double u = (double)(rng.next() & 0xFFFFFF) / 16777216.0;
+        if (delta <= 0 || u < std::exp(-(double)delta / temp)) {
+            for (size_t i = 0; i < nm; ++i) {
+                const int was = (masks[i] & ~T) == 0, now = (masks[i] & ~T2) == 0;
+                cov[i] += now - was;
+            }
+            tiles[p] = T2;
+            unc += delta;
+            if (unc == 0) { out = tiles; return true; }
+        }
+    }
+    return false;
+}
+
 int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
 {
     plan.passes.clear();
@@ -368,6 +418,37 @@ int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
         if (!t.empty() && (best.empty() || t.size() < best.size())) best = t;
     }
     if (best.empty()) return fail(PNED_E_UNSUPPORTED, "tile planner made no progress");
+    // refine: try one tile fewer (start from the greedy tiles minus the one that covers
+    // the fewest masks on its own) until the local search fails
+    if (nb > K && best.size() > 1) {
+        std::vector<uint64_t> um(mmask);
+        std::sort(um.begin(), um.end());
+        um.erase(std::unique(um.begin(), um.end()), um.end());
+        const uint64_t low = (1ull << L) - 1;
+        for (;;) {
+            size_t drop = 0, fewest = SIZE_MAX;
+            for (size_t p = 0; p < best.size(); ++p) {
+                size_t own = 0;
+                for (uint64_t mk : um) {
+                    if ((mk & ~best[p]) != 0) continue;
+                    bool other = false;
+                    for (size_t q = 0; q < best.size() && !other; ++q) other = (q != p) && (mk & ~best[q]) == 0;
+                    own += !other;
+                }
+                if (own < fewest) { fewest = own; drop = p; }
+            }
+            std::vector<uint64_t> init;
+            for (size_t p = 0; p < best.size(); ++p)
+                if (p != drop) init.push_back(best[p] | low);
+            std::vector<uint64_t> found;
+            bool ok = false;
+            for (int attempt = 0; attempt < 3 && !ok; ++attempt)
+                ok = anneal_cover(um, nb, L, K, init, rng, 60000, found);
+            if (!ok) break;
+            best = found;
+            if (best.size() <= 1) break;
+        }
+    }
     std::vector<char> done(plan.terms.size(), 0);
     for (size_t p = 0; p < best.size(); ++p) {
         const uint64_t T = best[p];
@@ -398,6 +479,7 @@ int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
             plan.devterms.push_back(d);
         }
         pp.nterm = (int)plan.devterms.size() - pp.first_term;
+        if (pp.nterm == 0 && !pp.diag) continue;   // every term already placed
         plan.passes.push_back(pp);
     }
     plan.devcoef.clear();
