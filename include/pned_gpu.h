@@ -81,6 +81,9 @@ int pned_ctx_autotune(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, int reps,
                       int *best_low_bits, int *best_threads, double *best_ms);
 /* spin-S kernel choice: 1 = tiled passes (default, q = 3..5), 0 = per-state gather */
 int pned_ctx_spin_kernel(pned_ctx *ctx, int tiled);
+/* S=1/2 pass kernel choice (no reference counterpart): 1 = term-major (default; per-term
+ * selector and partner address resolved once per tile), 0 = state-major */
+int pned_ctx_s12_kernel(pned_ctx *ctx, int kind);
 /* diagnostic: number of tile passes one H.v makes, and terms per pass (max 64) */
 int pned_ctx_plan_info(const pned_ctx *ctx, int *npasses, int *terms_per_pass, int max);
 /* diagnostic, needs no GPU: the S=1/2 term decomposition and tile passes the planner

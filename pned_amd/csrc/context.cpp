@@ -716,6 +716,7 @@ int pned_ctx_destroy(pned_ctx *ctx)
     if (ctx->d_terms) (void)hipFree(ctx->d_terms);
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
     if (ctx->pd_dev) (void)hipFree(ctx->pd_dev);
+    if (ctx->d_tm) (void)hipFree(ctx->d_tm);
     if (ctx->d_colcoef) (void)hipFree(ctx->d_colcoef);
     if (ctx->d_coldiag) (void)hipFree(ctx->d_coldiag);
     if (ctx->d_partial) (void)hipFree(ctx->d_partial);
@@ -755,6 +756,14 @@ int pned_ctx_spin_kernel(pned_ctx *ctx, int tiled)
 {
     if (!ctx) return fail(PNED_E_ARG, "null context");
     ctx->spin_tiled = tiled ? 1 : 0;
+    return PNED_OK;
+}
+
+int pned_ctx_s12_kernel(pned_ctx *ctx, int kind)
+{
+    if (!ctx) return fail(PNED_E_ARG, "null context");
+    if (kind != 0 && kind != 1) return fail(PNED_E_ARG, "kernel kind 0 or 1");
+    ctx->s12_kernel = kind;
     return PNED_OK;
 }
 

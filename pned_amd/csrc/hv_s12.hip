@@ -36,6 +36,7 @@ struct PassArgs {
     const double *colcoef; // [devterm][nop][4] when nop > 1
     const double *coldiag; // [nop][ngroups+1][2] when nop > 1
     const double *colscale;// per-column factor on v1 (H (s v1)); null = 1
+    int ncat[4];           // term-major kernel: terms per category (see build_tm_terms)
 };
 
 // Per-pass address tables in device memory (read with scalar loads):
@@ -282,6 +283,216 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
     }
 }
 
+// Term-major pass kernel.  Same tiles, tables and results as hv_s12_pass, but the loop
+// runs over terms outside and over the R register slots of a thread inside, with one
+// accumulator per slot.  For a term with flip mask m and selector mask s (tile slot
+// j = tid + TPB r):
+//   partner slot  j ^ m = (tid ^ m_lo) + TPB (r ^ m_hi)
+//   selector      parity(tid & s_lo) ^ parity(outer & s_out) ^ parity(r & s_hi)
+// The first two parts are fixed for the (thread, tile): they pick the two coefficients
+// once per term per tile; the register part is uniform across the wave.  A state-term
+// then costs one address add, one LDS read and 2 (real) or 4 (complex) FMAs, plus a
+// 64-bit select for terms whose selector has register bits.  The host sorts each pass's
+// terms into these four categories so every category is one straight-line loop (the
+// accumulators never pass through divergent merges).  Accumulates in Real (FP64 for
+// complex128, FP32 for complex64 storage); the <x|Hx> partials are FP64.
+template <typename Real, int TPB, int R>
+__global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__restrict__ v1,
+                                                 typename C2<Real>::T *__restrict__ v0,
+                                                 const TmTerm *__restrict__ terms,
+                                                 const DiagDev *__restrict__ diagp,
+                                                 const PassDev *__restrict__ pd, PassArgs a,
+                                                 double *__restrict__ partial)
+{
+    using V = typename C2<Real>::T;
+    using A = Real;
+    extern __shared__ unsigned char smem_raw[];
+    V *tile = reinterpret_cast<V *>(smem_raw);
+    const int tid = threadIdx.x;
+    const uint64_t offLo = pd->lo[tid];
+    const bool rd0 = !(a.diag && a.fresh);
+    const uint64_t lomask = (1ull << a.nlo) - 1;
+    auto outer_base = [&](uint64_t tix) { return pd->blo[tix & lomask] | pd->bhi[tix >> a.nlo]; };
+    constexpr uint32_t RSTRIDE = (uint32_t)(TPB * sizeof(V));   // bytes between register slots
+    const uint32_t lane_b = (uint32_t)tid * (uint32_t)sizeof(V);
+
+    double pr_re = 0.0, pr_im = 0.0;
+    Real xr[R], xi[R];
+    auto fetch_x = [&](uint64_t b) {
+        static_for<R>([&](auto I) {
+            const V t = v1[b | pd->hi[I]];
+            xr[I] = t.x;
+            xi[I] = t.y;
+        });
+    };
+    uint64_t tix = blockIdx.x;
+    if (tix < a.tile_count) fetch_x(outer_base(tix) | offLo);
+    for (; tix < a.tile_count; tix += gridDim.x) {
+        const uint64_t ob = outer_base(tix);
+        const uint64_t tb0 = ob | offLo;
+        static_for<R>([&](auto I) {
+            V t;
+            t.x = xr[I];
+            t.y = xi[I];
+            tile[tid + TPB * I] = t;
+        });
+        __syncthreads();
+        Real yr[R], yi[R];
+        if (rd0)
+            static_for<R>([&](auto I) {
+                const V t = v0[tb0 | pd->hi[I]];
+                yr[I] = t.x;
+                yi[I] = t.y;
+            });
+        const uint64_t nx = tix + gridDim.x;
+        if (nx < a.tile_count) fetch_x(outer_base(nx) | offLo);
+
+        A ar[R], ai[R];
+        if (a.diag) {
+            const DiagDev &dg = *diagp;
+            static_for<R>([&](auto I) {
+                const uint32_t st = (uint32_t)((tb0 | pd->hi[I]) >> a.colbits);
+                double dr, di;
+                if (a.nop > 1) {
+                    const double *w = a.coldiag + (size_t)(tid & (a.nop - 1)) * (dg.ngroups + 1) * 2;
+                    dr = w[2 * dg.ngroups];
+                    di = w[2 * dg.ngroups + 1];
+                    for (int g = 0; g < dg.ngroups; ++g) {
+                        const uint32_t f = (dg.kind[g] == 0) ? (st & dg.mask[g])
+                                                             : ((st ^ (st >> dg.shift[g])) & dg.mask[g]);
+                        const double cnt = (double)__popc(f);
+                        dr = fma(cnt, w[2 * g], dr);
+                        di = fma(cnt, w[2 * g + 1], di);
+                    }
+                    dr += a.shift;
+                } else {
+                    dr = diag_value_re(dg, st, di) + a.shift;
+                }
+                const V x = tile[tid + TPB * I];
+                ar[I] = (A)(dr * (double)x.x - di * (double)x.y);
+                ai[I] = (A)(dr * (double)x.y + di * (double)x.x);
+            });
+        } else {
+            static_for<R>([&](auto I) {
+                ar[I] = (A)0;
+                ai[I] = (A)0;
+            });
+        }
+
+        // terms come sorted by category: [real | real, register selector | complex |
+        // complex, register selector]; one straight-line loop per category
+        int t = a.first_term;
+        auto prelude = [&](const TmTerm &T, A &car, A &cai, A &cbr, A &cbi) {
+            const uint32_t par = (uint32_t)(__popc((uint32_t)tid & T.slo) + __popcll(ob & T.sout)) & 1u;
+            A c0r, c0i, c1r, c1i;
+            if (a.nop > 1) {
+                const double *cc = a.colcoef + ((size_t)T.term * a.nop + (size_t)(tid & (a.nop - 1))) * 4;
+                c0r = (A)cc[0];
+                c0i = (A)cc[1];
+                c1r = (A)cc[2];
+                c1i = (A)cc[3];
+            } else {
+                c0r = (A)T.c0r;
+                c0i = (A)T.c0i;
+                c1r = (A)T.c1r;
+                c1i = (A)T.c1i;
+            }
+            car = par ? c1r : c0r;   // coefficient where the register part of the selector is even
+            cai = par ? c1i : c0i;
+            cbr = par ? c0r : c1r;   // ... and where it is odd
+            cbi = par ? c0i : c1i;
+            return smem_raw + (lane_b ^ (T.mlo * (uint32_t)sizeof(V)));
+        };
+        for (const int e = t + a.ncat[0]; t < e; ++t) {
+            const TmTerm T = terms[t];
+            A car, cai, cbr, cbi;
+            const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
+            static_for<R>([&](auto I) {
+                const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)I ^ T.mhi) * RSTRIDE);
+                ar[I] = fma(car, (A)p.x, ar[I]);
+                ai[I] = fma(car, (A)p.y, ai[I]);
+            });
+        }
+        for (const int e = t + a.ncat[1]; t < e; ++t) {
+            const TmTerm T = terms[t];
+            A car, cai, cbr, cbi;
+            const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
+            static_for<R>([&](auto I) {
+                const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)I ^ T.mhi) * RSTRIDE);
+                const A cr = (__popc((uint32_t)I & T.shi) & 1) ? cbr : car;
+                ar[I] = fma(cr, (A)p.x, ar[I]);
+                ai[I] = fma(cr, (A)p.y, ai[I]);
+            });
+        }
+        for (const int e = t + a.ncat[2]; t < e; ++t) {
+            const TmTerm T = terms[t];
+            A car, cai, cbr, cbi;
+            const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
+            static_for<R>([&](auto I) {
+                const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)I ^ T.mhi) * RSTRIDE);
+                ar[I] = fma(car, (A)p.x, ar[I]);
+                ar[I] = fma(-cai, (A)p.y, ar[I]);
+                ai[I] = fma(car, (A)p.y, ai[I]);
+                ai[I] = fma(cai, (A)p.x, ai[I]);
+            });
+        }
+        for (const int e = t + a.ncat[3]; t < e; ++t) {
+            const TmTerm T = terms[t];
+            A car, cai, cbr, cbi;
+            const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
+            static_for<R>([&](auto I) {
+                const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)I ^ T.mhi) * RSTRIDE);
+                const bool f = __popc((uint32_t)I & T.shi) & 1;
+                const A cr = f ? cbr : car, ci = f ? cbi : cai;
+                ar[I] = fma(cr, (A)p.x, ar[I]);
+                ar[I] = fma(-ci, (A)p.y, ar[I]);
+                ai[I] = fma(cr, (A)p.y, ai[I]);
+                ai[I] = fma(ci, (A)p.x, ai[I]);
+            });
+        }
+
+        if (a.colscale) {
+            const A sc = (A)a.colscale[tid & (a.ncol - 1)];
+            static_for<R>([&](auto I) {
+                ar[I] *= sc;
+                ai[I] *= sc;
+            });
+        }
+        static_for<R>([&](auto I) {
+            const V x = tile[tid + TPB * I];
+            const double hr = (double)ar[I], hi = (double)ai[I];
+            pr_re += (double)x.x * hr + (double)x.y * hi;
+            pr_im += (double)x.x * hi - (double)x.y * hr;
+            V o;
+            if (rd0) {
+                o.x = (Real)(yr[I] + ar[I]);
+                o.y = (Real)(yi[I] + ai[I]);
+            } else {
+                o.x = (Real)ar[I];
+                o.y = (Real)ai[I];
+            }
+            v0[tb0 | pd->hi[I]] = o;
+        });
+        __syncthreads();
+    }
+
+    double *red = reinterpret_cast<double *>(smem_raw);
+    red[2 * tid] = pr_re;
+    red[2 * tid + 1] = pr_im;
+    __syncthreads();
+    for (int s = TPB / 2; s >= a.ncol && s > 0; s >>= 1) {
+        if (tid < s) {
+            red[2 * tid] += red[2 * (tid + s)];
+            red[2 * tid + 1] += red[2 * (tid + s) + 1];
+        }
+        __syncthreads();
+    }
+    if (tid < a.ncol) {
+        partial[(size_t)blockIdx.x * 2 * a.ncol + 2 * tid] = red[2 * tid];
+        partial[(size_t)blockIdx.x * 2 * a.ncol + 2 * tid + 1] = red[2 * tid + 1];
+    }
+}
+
 // out[c] = sum over rows of partial[row][c]: one block per column, fixed strides and a
 // fixed tree, so the result is the same for the same partials
 __global__ __launch_bounds__(256) void reduce_partials(const double *__restrict__ partial, int nrows, int ncol2,
@@ -327,6 +538,83 @@ static int dispatch_pass(pned_ctx *ctx, int TPB, int R, const void *v1, void *v0
     PNED_CASE(512, 32) PNED_CASE(512, 16) PNED_CASE(512, 8) PNED_CASE(512, 4) PNED_CASE(512, 2)
 #undef PNED_CASE
     return fail(PNED_E_UNSUPPORTED, "no H.v kernel instance for this tile shape");
+}
+
+template <typename Real, int TPB, int R>
+static int launch_tm(pned_ctx *ctx, const void *v1, void *v0, const TmTerm *tm, const DiagDev *ddiag,
+                     const PassDev *pd, const PassArgs &a, double *partial, int grid)
+{
+    using V = typename C2<Real>::T;
+    size_t lds = sizeof(V) * (size_t)TPB * R;
+    size_t need_red = sizeof(double) * 2 * TPB;
+    if (lds < need_red) lds = need_red;
+    auto kern = hv_s12_tm<Real, TPB, R>;
+    PNED_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(TPB), lds, ctx->stream, (const V *)v1, (V *)v0, tm, ddiag, pd, a,
+                       partial);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
+// term-major instances: accumulators, prefetched v1 and v0 of R slots stay in registers,
+// so R is bounded by the register file (16 slots of complex128, 32 of complex64)
+template <typename Real>
+static bool has_tm(int TPB, int R)
+{
+    if (TPB != 256 && TPB != 512) return false;
+    return R == 1 || R == 2 || R == 4 || R == 8 || R == 16 || (R == 32 && sizeof(Real) == 4);
+}
+
+template <typename Real>
+static int dispatch_tm(pned_ctx *ctx, int TPB, int R, const void *v1, void *v0, const TmTerm *tm,
+                       const DiagDev *ddiag, const PassDev *pd, const PassArgs &a, double *partial, int grid)
+{
+#define PNED_CASE(T_, R_) \
+    if (TPB == T_ && R == R_) return launch_tm<Real, T_, R_>(ctx, v1, v0, tm, ddiag, pd, a, partial, grid);
+    PNED_CASE(256, 16) PNED_CASE(256, 8) PNED_CASE(256, 4) PNED_CASE(256, 2) PNED_CASE(256, 1)
+    PNED_CASE(512, 16) PNED_CASE(512, 8) PNED_CASE(512, 4) PNED_CASE(512, 2) PNED_CASE(512, 1)
+    if constexpr (sizeof(Real) == 4) {
+        PNED_CASE(256, 32) PNED_CASE(512, 32)
+    }
+#undef PNED_CASE
+    return fail(PNED_E_UNSUPPORTED, "no term-major H.v kernel instance for this tile shape");
+}
+
+// Tile-coordinate term table of one pass (see TmTerm), sorted into the term-major
+// kernel's categories: 0 real, 1 real with register selector bits, 2 complex, 3 complex
+// with register selector bits (batch contexts: every term complex).  ncat gets the counts.
+static void build_tm_terms(const S12Plan &pl, const PassPlan &pp, int TPB, int colbits, std::vector<TmTerm> &out,
+                           int ncat[4])
+{
+    const int LT = __builtin_ctz(TPB);
+    uint64_t tilemask = 0;
+    for (int b : pp.tb) tilemask |= 1ull << b;
+    std::vector<TmTerm> cat[4];
+    for (int i = 0; i < pp.nterm; ++i) {
+        const DevTerm &d = pl.devterms[pp.first_term + i];
+        TmTerm t{};
+        t.mlo = d.tmask & (uint32_t)(TPB - 1);
+        t.mhi = d.tmask >> LT;
+        const uint64_t selm = (uint64_t)d.selmask << colbits;
+        uint32_t stile = 0;
+        for (int k = 0; k < pp.K; ++k)
+            if (selm >> pp.tb[k] & 1) stile |= 1u << k;
+        t.slo = stile & (uint32_t)(TPB - 1);
+        t.shi = stile >> LT;
+        t.sout = selm & ~tilemask;
+        const bool cx = !d.is_real || pl.nop > 1;
+        t.flags = (cx ? 1u : 0u) | ((t.shi != 0) ? 2u : 0u);
+        t.term = (uint32_t)(pp.first_term + i);
+        t.c0r = d.c0r;
+        t.c0i = d.c0i;
+        t.c1r = d.c1r;
+        t.c1i = d.c1i;
+        cat[(cx ? 2 : 0) + (t.shi != 0 ? 1 : 0)].push_back(t);
+    }
+    for (int c = 0; c < 4; ++c) {
+        ncat[c] = (int)cat[c].size();
+        out.insert(out.end(), cat[c].begin(), cat[c].end());
+    }
 }
 
 int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
@@ -391,11 +679,19 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
                 d.bhi[x] = o;
             }
         }
+        std::vector<TmTerm> tmh;
+        ctx->tm_ncat.assign(4 * (size_t)npass, 0);
+        for (int p = 0; p < npass; ++p) build_tm_terms(pl, pl.passes[p], tpb[p], colbits, tmh, &ctx->tm_ncat[4 * p]);
+        if (tmh.empty()) tmh.push_back(TmTerm{});
         PNED_HIP(hipStreamSynchronize(ctx->stream));
         if (ctx->pd_dev) PNED_HIP(hipFree(ctx->pd_dev));
         ctx->pd_dev = nullptr;
+        if (ctx->d_tm) PNED_HIP(hipFree(ctx->d_tm));
+        ctx->d_tm = nullptr;
         PNED_HIP(hipMalloc(&ctx->pd_dev, sizeof(PassDev) * npass));
         PNED_HIP(hipMemcpy(ctx->pd_dev, host.data(), sizeof(PassDev) * npass, hipMemcpyHostToDevice));
+        PNED_HIP(hipMalloc((void **)&ctx->d_tm, sizeof(TmTerm) * tmh.size()));
+        PNED_HIP(hipMemcpy(ctx->d_tm, tmh.data(), sizeof(TmTerm) * tmh.size(), hipMemcpyHostToDevice));
         ctx->pd_threads = TPBpref;
     }
     int total_rows = 0;
@@ -417,15 +713,24 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         a.colcoef = ctx->d_colcoef;
         a.coldiag = ctx->d_coldiag;
         a.colscale = d_colscale;
+        for (int c = 0; c < 4; ++c) a.ncat[c] = ctx->tm_ncat[4 * p + c];
         const int TPB = tpb[p], R = rr[p];
         int grid = (int)((a.tile_count < (uint64_t)grid_cap) ? a.tile_count : (uint64_t)grid_cap);
         double *part = ctx->d_partial + (size_t)total_rows * 2 * nvec;
         const PassDev *pdp = reinterpret_cast<const PassDev *>(ctx->pd_dev) + p;
         int rc;
-        if (v1->dtype == PNED_C128)
+        const bool c128 = v1->dtype == PNED_C128;
+        if (ctx->s12_kernel == 1 && (c128 ? has_tm<double>(TPB, R) : has_tm<float>(TPB, R))) {
+            // the term table is in pass order, so the kernel indexes it by first_term as well
+            if (c128)
+                rc = dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid);
+            else
+                rc = dispatch_tm<float>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid);
+        } else if (c128) {
             rc = dispatch_pass<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_diag, pdp, a, part, grid);
-        else
+        } else {
             rc = dispatch_pass<float>(ctx, TPB, R, v1->data, v0->data, ctx->d_diag, pdp, a, part, grid);
+        }
         if (rc != PNED_OK) return rc;
         total_rows += grid;
     }

@@ -56,6 +56,18 @@ struct DevTerm {
     double c0r, c0i, c1r, c1i;
 };
 
+// A term as the term-major pass kernel reads it, in the coordinates of one pass's tile
+// (tile slot j = thread + TPB * register): the flip mask and the selector split into
+// thread bits (lo), register bits (hi) and, for the selector, outer memory bits.
+struct TmTerm {
+    uint32_t mlo, mhi;   // flip mask: thread bits, register bits (>> log2 TPB)
+    uint32_t slo, shi;   // selector mask: thread bits, register bits
+    uint64_t sout;       // selector mask on the outer memory-index bits
+    uint32_t flags;      // bit 0: complex coefficients; bit 1: selector has register bits
+    uint32_t term;       // index into the pass-ordered DevTerm table (batch coefficients)
+    double c0r, c0i, c1r, c1i;
+};
+
 struct PassPlan {
     int K;                   // tile bits (over memory-index bits)
     std::vector<int> tb;     // tile bit positions (memory index), ascending
@@ -141,6 +153,9 @@ struct pned_ctx {
     size_t d_terms_cap = 0;
     pned::DiagDev *d_diag = nullptr;
     void *pd_dev = nullptr;          // per-pass address tables of the current plan
+    pned::TmTerm *d_tm = nullptr;    // per-pass tile-coordinate terms (term-major kernel)
+    std::vector<int> tm_ncat;        // [pass][4] terms per term-major category
+    int s12_kernel = 1;              // 1: term-major pass kernel, 0: state-major pass kernel
     double *d_colcoef = nullptr;     // batch contexts: per-column term coefficients
     double *d_coldiag = nullptr;     // batch contexts: per-column diagonal weights
     int pd_threads = -1;

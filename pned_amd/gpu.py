@@ -52,6 +52,7 @@ def lib():
     L.pned_ctx_tune.argtypes = [P, i32, i32, i32]
     L.pned_ctx_plan_info.argtypes = [P, C.POINTER(i32), _ip, i32]
     L.pned_ctx_spin_kernel.argtypes = [P, i32]
+    L.pned_ctx_s12_kernel.argtypes = [P, i32]
     L.pned_ctx_autotune.argtypes = [P, P, P, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                     C.POINTER(f64)]
     L.pned_ctx_sync.argtypes = [P]
@@ -218,6 +219,10 @@ class Context:
 
     def spin_kernel(self, tiled: bool):
         check(lib().pned_ctx_spin_kernel(self.h, 1 if tiled else 0))
+
+    def s12_kernel(self, kind: int):
+        """S=1/2 pass kernel: 1 term-major (default), 0 state-major."""
+        check(lib().pned_ctx_s12_kernel(self.h, kind))
 
     def plan_info(self):
         n = C.c_int()

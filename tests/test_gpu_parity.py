@@ -40,12 +40,17 @@ HV_CASES = [
 ]
 
 
+KERNELS = [1, 0]   # S=1/2 pass kernel: term-major, state-major
+
+
+@pytest.mark.parametrize("kern", KERNELS)
 @pytest.mark.parametrize("spec,phi,h,q", HV_CASES)
-def test_hv_parity(spec, phi, h, q):
-    _say("hv", spec, phi, h, q)
+def test_hv_parity(spec, phi, h, q, kern):
+    _say("hv", spec, phi, h, q, "kernel", kern)
     cl = lattice.get_cluster(spec)
     op = model.build_operator(cl, 20, phi, q=q, field=h)
     ctx = gpu.Context(op)
+    ctx.s12_kernel(kern)
     d = ctx.dim
     x = _rand(d, 1)
     y = _rand(d, 2)
@@ -71,12 +76,14 @@ def test_hv_parity(spec, phi, h, q):
     assert abs(pr2 - pr_ref) <= 1e-13 * scale
 
 
-@pytest.mark.parametrize("tile_bits", [8, 10, 13])
-def test_hv_tile_shapes(tile_bits):
-    _say("hv tile", tile_bits)
+@pytest.mark.parametrize("kern", KERNELS)
+@pytest.mark.parametrize("tile_bits,threads", [(8, 0), (10, 0), (12, 256), (12, 512), (13, 512), (11, 256)])
+def test_hv_tile_shapes(tile_bits, threads, kern):
+    _say("hv tile", tile_bits, threads, "kernel", kern)
     op = model.build_operator(lattice.reference_cluster(16), 20, 77.0, field=0.2)
     ctx = gpu.Context(op)
-    ctx.tune(tile_bits=tile_bits, low_bits=3)
+    ctx.s12_kernel(kern)
+    ctx.tune(tile_bits=tile_bits, low_bits=3, threads=threads)
     x = _rand(ctx.dim, 3)
     v1, v0 = ctx.vector(), ctx.vector()
     v1.upload(x)
@@ -86,11 +93,13 @@ def test_hv_tile_shapes(tile_bits):
     assert np.linalg.norm(v0.download() - want) <= 1e-12 * np.linalg.norm(want)
 
 
-@pytest.mark.parametrize("nvec,dtype", [(4, gpu.C128), (8, gpu.C64), (1, gpu.C64)])
-def test_hv_batched_and_c64(nvec, dtype):
-    _say("hv batched", nvec, dtype)
+@pytest.mark.parametrize("kern", KERNELS)
+@pytest.mark.parametrize("nvec,dtype", [(4, gpu.C128), (8, gpu.C64), (1, gpu.C64), (64, gpu.C64)])
+def test_hv_batched_and_c64(nvec, dtype, kern):
+    _say("hv batched", nvec, dtype, "kernel", kern)
     op = model.build_operator(lattice.reference_cluster(16), 20, 126.0, field=0.1)
     ctx = gpu.Context(op)
+    ctx.s12_kernel(kern)
     X = _rand(ctx.dim, 5, nvec)
     if nvec == 1:
         X = X.reshape(-1, 1)
