@@ -37,6 +37,7 @@ struct PassArgs {
     const double *coldiag; // [nop][ngroups+1][2] when nop > 1
     const double *colscale;// per-column factor on v1 (H (s v1)); null = 1
     int ncat[4];           // term-major kernel: terms per category (see build_tm_terms)
+    int ngroups;           // diagonal groups (batch contexts stage their weights)
 };
 
 // Per-pass address tables in device memory (read with scalar loads):
@@ -77,6 +78,15 @@ template <int N, typename F>
 __device__ __forceinline__ void static_for(F &&f)
 {
     static_for_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+// a wave-uniform double held in scalar registers
+__device__ __forceinline__ double uniform(double v)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 // TPB threads, R states per thread: tile of K = log2(TPB*R) bits.  Tile slot
@@ -316,6 +326,19 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
     constexpr uint32_t RSTRIDE = (uint32_t)(TPB * sizeof(V));   // bytes between register slots
     const uint32_t lane_b = (uint32_t)tid * (uint32_t)sizeof(V);
 
+    // batch contexts: this pass's per-column coefficients [term][column][4] and, in the
+    // diagonal pass, the per-column diagonal weights [column][groups+1][2], after the tile
+    double *ccol = reinterpret_cast<double *>(smem_raw + sizeof(V) * TPB * R);
+    double *cdiag = ccol + (a.nop > 1 ? (size_t)a.nterm * a.nop * 4 : 0);
+    if (a.nop > 1) {
+        const int n = a.nterm * a.nop * 4;
+        const double *src = a.colcoef + (size_t)a.first_term * a.nop * 4;
+        for (int i = tid; i < n; i += TPB) ccol[i] = src[i];
+        if (a.diag)
+            for (int i = tid; i < a.nop * (a.ngroups + 1) * 2; i += TPB) cdiag[i] = a.coldiag[i];
+    }
+    const A colsc = a.colscale ? (A)a.colscale[tid & (a.ncol - 1)] : (A)1;
+
     double pr_re = 0.0, pr_im = 0.0;
     Real xr[R], xi[R];
     auto fetch_x = [&](uint64_t b) {
@@ -354,7 +377,7 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
                 const uint32_t st = (uint32_t)((tb0 | pd->hi[I]) >> a.colbits);
                 double dr, di;
                 if (a.nop > 1) {
-                    const double *w = a.coldiag + (size_t)(tid & (a.nop - 1)) * (dg.ngroups + 1) * 2;
+                    const double *w = cdiag + (size_t)(tid & (a.nop - 1)) * (dg.ngroups + 1) * 2;
                     dr = w[2 * dg.ngroups];
                     di = w[2 * dg.ngroups + 1];
                     for (int g = 0; g < dg.ngroups; ++g) {
@@ -386,16 +409,19 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
             const uint32_t par = (uint32_t)(__popc((uint32_t)tid & T.slo) + __popcll(ob & T.sout)) & 1u;
             A c0r, c0i, c1r, c1i;
             if (a.nop > 1) {
-                const double *cc = a.colcoef + ((size_t)T.term * a.nop + (size_t)(tid & (a.nop - 1))) * 4;
+                // per-column coefficients, staged in LDS for this pass
+                const double *cc = ccol + ((size_t)(T.term - a.first_term) * a.nop + (size_t)(tid & (a.nop - 1))) * 4;
                 c0r = (A)cc[0];
                 c0i = (A)cc[1];
                 c1r = (A)cc[2];
                 c1i = (A)cc[3];
             } else {
-                c0r = (A)T.c0r;
-                c0i = (A)T.c0i;
-                c1r = (A)T.c1r;
-                c1i = (A)T.c1i;
+                // uniform: kept in scalar registers (a per-lane select of two loaded values
+                // would otherwise become a per-lane load)
+                c0r = (A)uniform(T.c0r);
+                c0i = (A)uniform(T.c0i);
+                c1r = (A)uniform(T.c1r);
+                c1i = (A)uniform(T.c1i);
             }
             car = par ? c1r : c0r;   // coefficient where the register part of the selector is even
             cai = par ? c1i : c0i;
@@ -403,59 +429,52 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
             cbi = par ? c0i : c1i;
             return smem_raw + (lane_b ^ (T.mlo * (uint32_t)sizeof(V)));
         };
-        for (const int e = t + a.ncat[0]; t < e; ++t) {
-            const TmTerm T = terms[t];
-            A car, cai, cbr, cbi;
-            const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
-            static_for<R>([&](auto I) {
-                const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)I ^ T.mhi) * RSTRIDE);
-                ar[I] = fma(car, (A)p.x, ar[I]);
-                ai[I] = fma(car, (A)p.y, ai[I]);
-            });
-        }
-        for (const int e = t + a.ncat[1]; t < e; ++t) {
-            const TmTerm T = terms[t];
-            A car, cai, cbr, cbi;
-            const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
-            static_for<R>([&](auto I) {
-                const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)I ^ T.mhi) * RSTRIDE);
-                const A cr = (__popc((uint32_t)I & T.shi) & 1) ? cbr : car;
-                ar[I] = fma(cr, (A)p.x, ar[I]);
-                ai[I] = fma(cr, (A)p.y, ai[I]);
-            });
-        }
-        for (const int e = t + a.ncat[2]; t < e; ++t) {
-            const TmTerm T = terms[t];
-            A car, cai, cbr, cbi;
-            const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
-            static_for<R>([&](auto I) {
-                const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)I ^ T.mhi) * RSTRIDE);
-                ar[I] = fma(car, (A)p.x, ar[I]);
-                ar[I] = fma(-cai, (A)p.y, ar[I]);
-                ai[I] = fma(car, (A)p.y, ai[I]);
-                ai[I] = fma(cai, (A)p.x, ai[I]);
-            });
-        }
-        for (const int e = t + a.ncat[3]; t < e; ++t) {
-            const TmTerm T = terms[t];
-            A car, cai, cbr, cbi;
-            const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
-            static_for<R>([&](auto I) {
-                const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)I ^ T.mhi) * RSTRIDE);
-                const bool f = __popc((uint32_t)I & T.shi) & 1;
-                const A cr = f ? cbr : car, ci = f ? cbi : cai;
-                ar[I] = fma(cr, (A)p.x, ar[I]);
-                ar[I] = fma(-ci, (A)p.y, ar[I]);
-                ai[I] = fma(cr, (A)p.y, ai[I]);
-                ai[I] = fma(ci, (A)p.x, ai[I]);
-            });
-        }
+        constexpr int G = (R < 4) ? R : 4;   // partner reads issued ahead of their FMAs
+        auto run_cat = [&](int count, auto CXc, auto SELc) {
+            constexpr bool CX = decltype(CXc)::value;
+            constexpr bool SEL = decltype(SELc)::value;
+            for (const int e = t + count; t < e; ++t) {
+                const TmTerm T = terms[t];
+                A car, cai, cbr, cbi;
+                const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
+                static_for<R / G>([&](auto GI) {
+                    Real qr[G], qi[G];
+                    static_for<G>([&](auto k) {
+                        constexpr int r = GI * G + k;
+                        const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)r ^ T.mhi) * RSTRIDE);
+                        qr[k] = p.x;
+                        qi[k] = p.y;
+                    });
+                    static_for<G>([&](auto k) {
+                        constexpr int r = GI * G + k;
+                        A cr = car, ci = cai;
+                        if constexpr (SEL) {
+                            const bool f = __popc((uint32_t)r & T.shi) & 1;
+                            cr = f ? cbr : car;
+                            if constexpr (CX) ci = f ? cbi : cai;
+                        }
+                        if constexpr (CX) {
+                            ar[r] = fma(cr, (A)qr[k], ar[r]);
+                            ar[r] = fma(-ci, (A)qi[k], ar[r]);
+                            ai[r] = fma(cr, (A)qi[k], ai[r]);
+                            ai[r] = fma(ci, (A)qr[k], ai[r]);
+                        } else {
+                            ar[r] = fma(cr, (A)qr[k], ar[r]);
+                            ai[r] = fma(cr, (A)qi[k], ai[r]);
+                        }
+                    });
+                });
+            }
+        };
+        run_cat(a.ncat[0], std::false_type{}, std::false_type{});
+        run_cat(a.ncat[1], std::false_type{}, std::true_type{});
+        run_cat(a.ncat[2], std::true_type{}, std::false_type{});
+        run_cat(a.ncat[3], std::true_type{}, std::true_type{});
 
         if (a.colscale) {
-            const A sc = (A)a.colscale[tid & (a.ncol - 1)];
             static_for<R>([&](auto I) {
-                ar[I] *= sc;
-                ai[I] *= sc;
+                ar[I] *= colsc;
+                ai[I] *= colsc;
             });
         }
         static_for<R>([&](auto I) {
@@ -540,12 +559,20 @@ static int dispatch_pass(pned_ctx *ctx, int TPB, int R, const void *v1, void *v0
     return fail(PNED_E_UNSUPPORTED, "no H.v kernel instance for this tile shape");
 }
 
+// LDS the term-major kernel stages per-column coefficients in (batch contexts)
+static size_t tm_coef_bytes(const PassArgs &a)
+{
+    if (a.nop <= 1) return 0;
+    return sizeof(double) * ((size_t)4 * a.nterm * a.nop + (a.diag ? (size_t)2 * (a.ngroups + 1) * a.nop : 0));
+}
+constexpr size_t LDS_LIMIT = 160 * 1024;
+
 template <typename Real, int TPB, int R>
 static int launch_tm(pned_ctx *ctx, const void *v1, void *v0, const TmTerm *tm, const DiagDev *ddiag,
                      const PassDev *pd, const PassArgs &a, double *partial, int grid)
 {
     using V = typename C2<Real>::T;
-    size_t lds = sizeof(V) * (size_t)TPB * R;
+    size_t lds = sizeof(V) * (size_t)TPB * R + tm_coef_bytes(a);
     size_t need_red = sizeof(double) * 2 * TPB;
     if (lds < need_red) lds = need_red;
     auto kern = hv_s12_tm<Real, TPB, R>;
@@ -714,13 +741,16 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         a.coldiag = ctx->d_coldiag;
         a.colscale = d_colscale;
         for (int c = 0; c < 4; ++c) a.ncat[c] = ctx->tm_ncat[4 * p + c];
+        a.ngroups = ctx->diag_host.ngroups;
         const int TPB = tpb[p], R = rr[p];
         int grid = (int)((a.tile_count < (uint64_t)grid_cap) ? a.tile_count : (uint64_t)grid_cap);
         double *part = ctx->d_partial + (size_t)total_rows * 2 * nvec;
         const PassDev *pdp = reinterpret_cast<const PassDev *>(ctx->pd_dev) + p;
         int rc;
         const bool c128 = v1->dtype == PNED_C128;
-        if (ctx->s12_kernel == 1 && (c128 ? has_tm<double>(TPB, R) : has_tm<float>(TPB, R))) {
+        const size_t tile_bytes = (c128 ? 16 : 8) * (size_t)TPB * R;
+        if (ctx->s12_kernel == 1 && (c128 ? has_tm<double>(TPB, R) : has_tm<float>(TPB, R)) &&
+            tile_bytes + tm_coef_bytes(a) <= LDS_LIMIT) {
             // the term table is in pass order, so the kernel indexes it by first_term as well
             if (c128)
                 rc = dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid);
