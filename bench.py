@@ -12,9 +12,16 @@ A step is one H.v in the reference's accumulate form, p_mltply_v3: v0 += H v1 wi
 HIP events recorded on the library's own stream around the timed H.v calls; the wall
 clock around the same region (barrier + synchronize on both sides) gives `value`.
 
+One H.v is P launches of the tile-pass kernel (P = passes of the plan) plus one small
+reduction; `roofline.achieved` is the algorithmic bytes of the H.v over the event time of
+those launches, `roofline.per_launch` the same per pass launch.  `roofline.traffic` is the
+measured HBM bytes per H.v from the committed counter summary
+(profiles/hv_s12_N32_counters.json: FETCH_SIZE and WRITE_SIZE per launch, calibrated on
+device copies of known size) when that summary was taken with the same tile plan, else null.
+
 cpu_baseline: the oracle's restatement of the reference's per-element pass structure
-(OpenMP, all host threads) on a bounded sample -- output rows [0, 2^24) of the same
-N=32 operator -- reported as algorithmic H.v GB/s.
+(OpenMP, all host threads) on a bounded sample -- output rows [0, 2^27) of the same
+N=32 operator, about 10-15 s on 16 cores -- reported as algorithmic H.v GB/s.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--cfg N32|N24|TPS|SERVE]
 """
@@ -36,14 +43,15 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=1 << 24)
+    ap.add_argument("--cpu-rows", type=int, default=1 << 27)
     ap.add_argument("--cfg", default="N32")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="", help="fixed plan K,L,threads (skips the autotuner)")
     ap.add_argument("--requests", type=int, default=120, help="SERVE: number of requests")
     ap.add_argument("--serve-big", action="store_true", help="SERVE: include the N=28 (2x7) size")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="only run 3 device copies of a 4 GiB vector (counter calibration)")
+                    help="only run 3 device copies and 3 elementwise y += x of 4 GiB vectors "
+                         "(counter calibration and the plain-streaming reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -64,7 +72,22 @@ def main():
         for _ in range(3):
             b.copy_from(a)
         ctx.sync()
-        print(json.dumps({"calibrate_copy_bytes": 16 * ctx.dim, "copies": 3}))
+        # plain 3-stream kernel of known size: y += x over 2^28 complex128
+        x = torch.zeros(1 << 28, dtype=torch.complex128, device="cuda")
+        y = torch.zeros_like(x)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        y.add_(x)
+        e0.record()
+        for _ in range(3):
+            y.add_(x)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        print(json.dumps({"calibrate_copy_bytes": 16 * ctx.dim, "copies": 3,
+                          "rmw_bytes": 3 * 16 * (1 << 28), "rmw_ms": round(ms, 4),
+                          "rmw_GBps": round(3 * 16 * (1 << 28) / (ms * 1e-3) / 1e9, 1)}))
         return
 
     if args.cfg == "TPS":
@@ -127,6 +150,15 @@ def main():
                "sample": f"oracle reference-pass-structure H.v (OpenMP) on output rows [0,{rows}) of the "
                          f"same N=32 operator, {dt:.2f} s"}
 
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "hv_s12_N32_counters.json")
+    if args.cfg == "N32" and os.path.exists(tfile):
+        with open(tfile) as fh:
+            prof = json.load(fh)
+        plan = [tune.get("tile_bits"), tune.get("low_bits"), tune.get("threads")] if tune else None
+        if plan is not None and prof.get("tune") == plan and prof.get("kernel") == "hv_s12_tm":
+            traffic = int(prof["hbm_bytes_per_hv"])
+
     out = {
         "metric": "N=32 H·v HBM GB/s (% peak); batched/mixed solves/s + p99 vs plain @1e-10",
         "value": round(value, 2),
@@ -145,7 +177,9 @@ def main():
                    "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
                    "prdct_re": float(np.real(prd))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "hv_s12_tm", "launches_per_unit": npass,
+                     "per_launch": round(achieved * npass, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
