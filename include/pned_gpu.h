@@ -72,7 +72,9 @@ int64_t pned_ctx_dim(const pned_ctx *ctx);
 /* replace the bond matrices in place (same sparsity structure not required),
  * e.g. change_to_TPS_hamiltonian (S_one_half/Hybrid_TPS_params.f90:1600-1623) */
 int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t *llocal_ham);
-/* tile/pass tuning knobs of the S=1/2 H.v (0 = autotuned default) */
+/* tile/pass tuning knobs of the S=1/2 H.v (no reference counterpart): tile bits <= 14,
+ * forced low bits, threads per block 256, 512 or 1024 (1024: term-major kernel only);
+ * 0 = default */
 int pned_ctx_tune(pned_ctx *ctx, int tile_bits, int low_bits, int threads);
 /* autotuner (no reference counterpart): times one fresh H.v v1 -> v0 for every
  * (tile bits, forced low bits, threads per block) candidate that fits LDS and keeps the

@@ -770,8 +770,9 @@ int pned_ctx_s12_kernel(pned_ctx *ctx, int kind)
 int pned_ctx_tune(pned_ctx *ctx, int tile_bits, int low_bits, int threads)
 {
     if (!ctx) return fail(PNED_E_ARG, "null context");
-    if (tile_bits < 0 || tile_bits > 14 || low_bits < 0 || low_bits > 8 || (threads != 0 && threads != 256 && threads != 512))
-        return fail(PNED_E_ARG, "tile_bits in 0..14, low_bits in 0..8, threads 0, 256 or 512");
+    if (tile_bits < 0 || tile_bits > 14 || low_bits < 0 || low_bits > 8 ||
+        (threads != 0 && threads != 256 && threads != 512 && threads != 1024))
+        return fail(PNED_E_ARG, "tile_bits in 0..14, low_bits in 0..8, threads 0, 256, 512 or 1024");
     ctx->tune_tile_bits = tile_bits;
     ctx->tune_low_bits = low_bits;
     ctx->tune_threads = threads;
@@ -799,7 +800,8 @@ int pned_ctx_autotune(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, int reps,
     int bk = 0, bl = 0, bt = 0;
     for (int K = kmax - 2; K <= kmax; ++K)
         for (int L = 3; L <= 4; ++L)
-            for (int T : {256, 512}) {
+            for (int T : {256, 512, 1024}) {
+                if (T == 1024 && ctx->s12_kernel != 1) continue;   // term-major kernel only
                 ctx->tune_tile_bits = K;
                 ctx->tune_low_bits = L;
                 ctx->tune_threads = T;

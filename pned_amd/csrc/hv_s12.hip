@@ -44,7 +44,7 @@ struct PassArgs {
 //   lo[t]  memory-index offset of thread t's slot (tile bits 0..LT-1)
 //   hi[r]  offset of register slot r (tile bits LT..K-1)
 //   blo/bhi  tile base = blo[tix & (2^nlo - 1)] | bhi[tix >> nlo]  (outer bits)
-constexpr int PD_LO = 512, PD_HI = 32, PD_B = 2048;
+constexpr int PD_LO = 1024, PD_HI = 32, PD_B = 2048;
 struct PassDev {
     uint64_t lo[PD_LO];
     uint64_t hi[PD_HI];
@@ -294,19 +294,23 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
 }
 
 // Term-major pass kernel.  Same tiles, tables and results as hv_s12_pass, but the loop
-// runs over terms outside and over the R register slots of a thread inside, with one
+// runs over terms outside and over register slots of a thread inside, with one
 // accumulator per slot.  For a term with flip mask m and selector mask s (tile slot
 // j = tid + TPB r):
 //   partner slot  j ^ m = (tid ^ m_lo) + TPB (r ^ m_hi)
 //   selector      parity(tid & s_lo) ^ parity(outer & s_out) ^ parity(r & s_hi)
 // The first two parts are fixed for the (thread, tile): they pick the two coefficients
-// once per term per tile; the register part is uniform across the wave.  A state-term
+// once per term per sweep; the register part is uniform across the wave.  A state-term
 // then costs one address add, one LDS read and 2 (real) or 4 (complex) FMAs, plus a
 // 64-bit select for terms whose selector has register bits.  The host sorts each pass's
 // terms into these four categories so every category is one straight-line loop (the
-// accumulators never pass through divergent merges).  Accumulates in Real (FP64 for
-// complex128, FP32 for complex64 storage); the <x|Hx> partials are FP64.
-template <typename Real, int TPB, int R>
+// accumulators never pass through divergent merges).
+//
+// The R slots of a tile are accumulated in R/S sweeps of S slots, so only S accumulators
+// and S v0 values are live next to the R prefetched v1 values of the next tile; a sweep's
+// v0 values are in flight while its terms are applied.  Accumulates in Real (FP64 for complex128, FP32 for complex64
+// storage); the <x|Hx> partials are FP64.
+template <typename Real, int TPB, int R, int S>
 __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__restrict__ v1,
                                                  typename C2<Real>::T *__restrict__ v0,
                                                  const TmTerm *__restrict__ terms,
@@ -314,8 +318,10 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
                                                  const PassDev *__restrict__ pd, PassArgs a,
                                                  double *__restrict__ partial)
 {
+    static_assert(R % S == 0, "sweeps must tile the register slots");
     using V = typename C2<Real>::T;
     using A = Real;
+    constexpr int NS = R / S;
     extern __shared__ unsigned char smem_raw[];
     V *tile = reinterpret_cast<V *>(smem_raw);
     const int tid = threadIdx.x;
@@ -348,6 +354,13 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
             xi[I] = t.y;
         });
     };
+    auto fetch_y = [&](uint64_t b, int r0, Real *yr, Real *yi) {
+        static_for<S>([&](auto I) {
+            const V t = v0[b | pd->hi[r0 + I]];
+            yr[I] = t.x;
+            yi[I] = t.y;
+        });
+    };
     uint64_t tix = blockIdx.x;
     if (tix < a.tile_count) fetch_x(outer_base(tix) | offLo);
     for (; tix < a.tile_count; tix += gridDim.x) {
@@ -360,137 +373,142 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
             tile[tid + TPB * I] = t;
         });
         __syncthreads();
-        Real yr[R], yi[R];
-        if (rd0)
-            static_for<R>([&](auto I) {
-                const V t = v0[tb0 | pd->hi[I]];
-                yr[I] = t.x;
-                yi[I] = t.y;
-            });
+        // v0 of the first sweep goes out before the next tile's v1, so waiting for it does
+        // not wait for the prefetch (vector memory counters retire in order)
+        Real ycr[S], yci[S];
+        if (rd0) fetch_y(tb0, 0, ycr, yci);
         const uint64_t nx = tix + gridDim.x;
         if (nx < a.tile_count) fetch_x(outer_base(nx) | offLo);
 
-        A ar[R], ai[R];
-        if (a.diag) {
-            const DiagDev &dg = *diagp;
-            static_for<R>([&](auto I) {
-                const uint32_t st = (uint32_t)((tb0 | pd->hi[I]) >> a.colbits);
-                double dr, di;
-                if (a.nop > 1) {
-                    const double *w = cdiag + (size_t)(tid & (a.nop - 1)) * (dg.ngroups + 1) * 2;
-                    dr = w[2 * dg.ngroups];
-                    di = w[2 * dg.ngroups + 1];
-                    for (int g = 0; g < dg.ngroups; ++g) {
-                        const uint32_t f = (dg.kind[g] == 0) ? (st & dg.mask[g])
-                                                             : ((st ^ (st >> dg.shift[g])) & dg.mask[g]);
-                        const double cnt = (double)__popc(f);
-                        dr = fma(cnt, w[2 * g], dr);
-                        di = fma(cnt, w[2 * g + 1], di);
-                    }
-                    dr += a.shift;
-                } else {
-                    dr = diag_value_re(dg, st, di) + a.shift;
-                }
-                const V x = tile[tid + TPB * I];
-                ar[I] = (A)(dr * (double)x.x - di * (double)x.y);
-                ai[I] = (A)(dr * (double)x.y + di * (double)x.x);
-            });
-        } else {
-            static_for<R>([&](auto I) {
-                ar[I] = (A)0;
-                ai[I] = (A)0;
-            });
-        }
-
-        // terms come sorted by category: [real | real, register selector | complex |
-        // complex, register selector]; one straight-line loop per category
-        int t = a.first_term;
-        auto prelude = [&](const TmTerm &T, A &car, A &cai, A &cbr, A &cbi) {
-            const uint32_t par = (uint32_t)(__popc((uint32_t)tid & T.slo) + __popcll(ob & T.sout)) & 1u;
-            A c0r, c0i, c1r, c1i;
-            if (a.nop > 1) {
-                // per-column coefficients, staged in LDS for this pass
-                const double *cc = ccol + ((size_t)(T.term - a.first_term) * a.nop + (size_t)(tid & (a.nop - 1))) * 4;
-                c0r = (A)cc[0];
-                c0i = (A)cc[1];
-                c1r = (A)cc[2];
-                c1i = (A)cc[3];
-            } else {
-                // uniform: kept in scalar registers (a per-lane select of two loaded values
-                // would otherwise become a per-lane load)
-                c0r = (A)uniform(T.c0r);
-                c0i = (A)uniform(T.c0i);
-                c1r = (A)uniform(T.c1r);
-                c1i = (A)uniform(T.c1i);
+        static_for<NS>([&](auto HI) {
+            constexpr int r0 = HI * S;
+            if constexpr (r0 > 0) {
+                if (rd0) fetch_y(tb0, r0, ycr, yci);
             }
-            car = par ? c1r : c0r;   // coefficient where the register part of the selector is even
-            cai = par ? c1i : c0i;
-            cbr = par ? c0r : c1r;   // ... and where it is odd
-            cbi = par ? c0i : c1i;
-            return smem_raw + (lane_b ^ (T.mlo * (uint32_t)sizeof(V)));
-        };
-        constexpr int G = (R < 4) ? R : 4;   // partner reads issued ahead of their FMAs
-        auto run_cat = [&](int count, auto CXc, auto SELc) {
-            constexpr bool CX = decltype(CXc)::value;
-            constexpr bool SEL = decltype(SELc)::value;
-            for (const int e = t + count; t < e; ++t) {
-                const TmTerm T = terms[t];
-                A car, cai, cbr, cbi;
-                const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
-                static_for<R / G>([&](auto GI) {
-                    Real qr[G], qi[G];
-                    static_for<G>([&](auto k) {
-                        constexpr int r = GI * G + k;
-                        const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)r ^ T.mhi) * RSTRIDE);
-                        qr[k] = p.x;
-                        qi[k] = p.y;
-                    });
-                    static_for<G>([&](auto k) {
-                        constexpr int r = GI * G + k;
-                        A cr = car, ci = cai;
-                        if constexpr (SEL) {
-                            const bool f = __popc((uint32_t)r & T.shi) & 1;
-                            cr = f ? cbr : car;
-                            if constexpr (CX) ci = f ? cbi : cai;
+            A ar[S], ai[S];
+            if (a.diag) {
+                const DiagDev &dg = *diagp;
+                static_for<S>([&](auto I) {
+                    const uint32_t st = (uint32_t)((tb0 | pd->hi[r0 + I]) >> a.colbits);
+                    double dr, di;
+                    if (a.nop > 1) {
+                        const double *w = cdiag + (size_t)(tid & (a.nop - 1)) * (dg.ngroups + 1) * 2;
+                        dr = w[2 * dg.ngroups];
+                        di = w[2 * dg.ngroups + 1];
+                        for (int g = 0; g < dg.ngroups; ++g) {
+                            const uint32_t f = (dg.kind[g] == 0) ? (st & dg.mask[g])
+                                                                 : ((st ^ (st >> dg.shift[g])) & dg.mask[g]);
+                            const double cnt = (double)__popc(f);
+                            dr = fma(cnt, w[2 * g], dr);
+                            di = fma(cnt, w[2 * g + 1], di);
                         }
-                        if constexpr (CX) {
-                            ar[r] = fma(cr, (A)qr[k], ar[r]);
-                            ar[r] = fma(-ci, (A)qi[k], ar[r]);
-                            ai[r] = fma(cr, (A)qi[k], ai[r]);
-                            ai[r] = fma(ci, (A)qr[k], ai[r]);
-                        } else {
-                            ar[r] = fma(cr, (A)qr[k], ar[r]);
-                            ai[r] = fma(cr, (A)qi[k], ai[r]);
-                        }
-                    });
+                        dr += a.shift;
+                    } else {
+                        dr = diag_value_re(dg, st, di) + a.shift;
+                    }
+                    const V x = tile[tid + TPB * (r0 + I)];
+                    ar[I] = (A)(dr * (double)x.x - di * (double)x.y);
+                    ai[I] = (A)(dr * (double)x.y + di * (double)x.x);
+                });
+            } else {
+                static_for<S>([&](auto I) {
+                    ar[I] = (A)0;
+                    ai[I] = (A)0;
                 });
             }
-        };
-        run_cat(a.ncat[0], std::false_type{}, std::false_type{});
-        run_cat(a.ncat[1], std::false_type{}, std::true_type{});
-        run_cat(a.ncat[2], std::true_type{}, std::false_type{});
-        run_cat(a.ncat[3], std::true_type{}, std::true_type{});
 
-        if (a.colscale) {
-            static_for<R>([&](auto I) {
-                ar[I] *= colsc;
-                ai[I] *= colsc;
-            });
-        }
-        static_for<R>([&](auto I) {
-            const V x = tile[tid + TPB * I];
-            const double hr = (double)ar[I], hi = (double)ai[I];
-            pr_re += (double)x.x * hr + (double)x.y * hi;
-            pr_im += (double)x.x * hi - (double)x.y * hr;
-            V o;
-            if (rd0) {
-                o.x = (Real)(yr[I] + ar[I]);
-                o.y = (Real)(yi[I] + ai[I]);
-            } else {
-                o.x = (Real)ar[I];
-                o.y = (Real)ai[I];
+            // terms come sorted by category: [real | real, register selector | complex |
+            // complex, register selector]; one straight-line loop per category
+            int t = a.first_term;
+            auto prelude = [&](const TmTerm &T, A &car, A &cai, A &cbr, A &cbi) {
+                const uint32_t par = (uint32_t)(__popc((uint32_t)tid & T.slo) + __popcll(ob & T.sout)) & 1u;
+                A c0r, c0i, c1r, c1i;
+                if (a.nop > 1) {
+                    // per-column coefficients, staged in LDS for this pass
+                    const double *cc =
+                        ccol + ((size_t)(T.term - a.first_term) * a.nop + (size_t)(tid & (a.nop - 1))) * 4;
+                    c0r = (A)cc[0];
+                    c0i = (A)cc[1];
+                    c1r = (A)cc[2];
+                    c1i = (A)cc[3];
+                } else {
+                    // uniform: kept in scalar registers (a per-lane select of two loaded
+                    // values would otherwise become a per-lane load)
+                    c0r = (A)uniform(T.c0r);
+                    c0i = (A)uniform(T.c0i);
+                    c1r = (A)uniform(T.c1r);
+                    c1i = (A)uniform(T.c1i);
+                }
+                car = par ? c1r : c0r;   // coefficient where the register part of the selector is even
+                cai = par ? c1i : c0i;
+                cbr = par ? c0r : c1r;   // ... and where it is odd
+                cbi = par ? c0i : c1i;
+                return smem_raw + (lane_b ^ (T.mlo * (uint32_t)sizeof(V)));
+            };
+            constexpr int G = (S < 4) ? S : 4;   // partner reads issued ahead of their FMAs
+            auto run_cat = [&](int count, auto CXc, auto SELc) {
+                constexpr bool CX = decltype(CXc)::value;
+                constexpr bool SEL = decltype(SELc)::value;
+                for (const int e = t + count; t < e; ++t) {
+                    const TmTerm T = terms[t];
+                    A car, cai, cbr, cbi;
+                    const unsigned char *lb = prelude(T, car, cai, cbr, cbi);
+                    static_for<S / G>([&](auto GI) {
+                        Real qr[G], qi[G];
+                        static_for<G>([&](auto k) {
+                            constexpr int r = r0 + GI * G + k;
+                            const V p = *reinterpret_cast<const V *>(lb + ((uint32_t)r ^ T.mhi) * RSTRIDE);
+                            qr[k] = p.x;
+                            qi[k] = p.y;
+                        });
+                        static_for<G>([&](auto k) {
+                            constexpr int r = r0 + GI * G + k;
+                            constexpr int u = GI * G + k;
+                            A cr = car, ci = cai;
+                            if constexpr (SEL) {
+                                const bool f = __popc((uint32_t)r & T.shi) & 1;
+                                cr = f ? cbr : car;
+                                if constexpr (CX) ci = f ? cbi : cai;
+                            }
+                            if constexpr (CX) {
+                                ar[u] = fma(cr, (A)qr[k], ar[u]);
+                                ar[u] = fma(-ci, (A)qi[k], ar[u]);
+                                ai[u] = fma(cr, (A)qi[k], ai[u]);
+                                ai[u] = fma(ci, (A)qr[k], ai[u]);
+                            } else {
+                                ar[u] = fma(cr, (A)qr[k], ar[u]);
+                                ai[u] = fma(cr, (A)qi[k], ai[u]);
+                            }
+                        });
+                    });
+                }
+            };
+            run_cat(a.ncat[0], std::false_type{}, std::false_type{});
+            run_cat(a.ncat[1], std::false_type{}, std::true_type{});
+            run_cat(a.ncat[2], std::true_type{}, std::false_type{});
+            run_cat(a.ncat[3], std::true_type{}, std::true_type{});
+
+            if (a.colscale) {
+                static_for<S>([&](auto I) {
+                    ar[I] *= colsc;
+                    ai[I] *= colsc;
+                });
             }
-            v0[tb0 | pd->hi[I]] = o;
+            static_for<S>([&](auto I) {
+                const V x = tile[tid + TPB * (r0 + I)];
+                const double hr = (double)ar[I], hi = (double)ai[I];
+                pr_re += (double)x.x * hr + (double)x.y * hi;
+                pr_im += (double)x.x * hi - (double)x.y * hr;
+                V o;
+                if (rd0) {
+                    o.x = (Real)(ycr[I] + ar[I]);
+                    o.y = (Real)(yci[I] + ai[I]);
+                } else {
+                    o.x = (Real)ar[I];
+                    o.y = (Real)ai[I];
+                }
+                v0[tb0 | pd->hi[r0 + I]] = o;
+            });
         });
         __syncthreads();
     }
@@ -567,7 +585,7 @@ static size_t tm_coef_bytes(const PassArgs &a)
 }
 constexpr size_t LDS_LIMIT = 160 * 1024;
 
-template <typename Real, int TPB, int R>
+template <typename Real, int TPB, int R, int S>
 static int launch_tm(pned_ctx *ctx, const void *v1, void *v0, const TmTerm *tm, const DiagDev *ddiag,
                      const PassDev *pd, const PassArgs &a, double *partial, int grid)
 {
@@ -575,7 +593,7 @@ static int launch_tm(pned_ctx *ctx, const void *v1, void *v0, const TmTerm *tm, 
     size_t lds = sizeof(V) * (size_t)TPB * R + tm_coef_bytes(a);
     size_t need_red = sizeof(double) * 2 * TPB;
     if (lds < need_red) lds = need_red;
-    auto kern = hv_s12_tm<Real, TPB, R>;
+    auto kern = hv_s12_tm<Real, TPB, R, S>;
     PNED_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(TPB), lds, ctx->stream, (const V *)v1, (V *)v0, tm, ddiag, pd, a,
                        partial);
@@ -583,25 +601,34 @@ static int launch_tm(pned_ctx *ctx, const void *v1, void *v0, const TmTerm *tm, 
     return PNED_OK;
 }
 
-// term-major instances: accumulators, prefetched v1 and v0 of R slots stay in registers,
-// so R is bounded by the register file (16 slots of complex128, 32 of complex64)
+// term-major instances: the prefetched v1 of R slots plus the accumulators and v0 of one
+// sweep of S slots stay in registers; S is chosen so that nothing spills at the block's
+// occupancy (1024 threads leave 128 registers per lane, 512 threads 256)
 template <typename Real>
 static bool has_tm(int TPB, int R)
 {
-    if (TPB != 256 && TPB != 512) return false;
-    return R == 1 || R == 2 || R == 4 || R == 8 || R == 16 || (R == 32 && sizeof(Real) == 4);
+    if (TPB != 256 && TPB != 512 && TPB != 1024) return false;
+    if (sizeof(Real) == 8) return R == 1 || R == 2 || R == 4 || R == 8 || (R == 16 && TPB <= 512);
+    return R == 1 || R == 2 || R == 4 || R == 8 || R == 16 || (R == 32 && TPB <= 512);
 }
 
 template <typename Real>
 static int dispatch_tm(pned_ctx *ctx, int TPB, int R, const void *v1, void *v0, const TmTerm *tm,
                        const DiagDev *ddiag, const PassDev *pd, const PassArgs &a, double *partial, int grid)
 {
-#define PNED_CASE(T_, R_) \
-    if (TPB == T_ && R == R_) return launch_tm<Real, T_, R_>(ctx, v1, v0, tm, ddiag, pd, a, partial, grid);
-    PNED_CASE(256, 16) PNED_CASE(256, 8) PNED_CASE(256, 4) PNED_CASE(256, 2) PNED_CASE(256, 1)
-    PNED_CASE(512, 16) PNED_CASE(512, 8) PNED_CASE(512, 4) PNED_CASE(512, 2) PNED_CASE(512, 1)
-    if constexpr (sizeof(Real) == 4) {
-        PNED_CASE(256, 32) PNED_CASE(512, 32)
+#define PNED_CASE(T_, R_, S_) \
+    if (TPB == T_ && R == R_) return launch_tm<Real, T_, R_, S_>(ctx, v1, v0, tm, ddiag, pd, a, partial, grid);
+    if constexpr (sizeof(Real) == 8) {
+        PNED_CASE(256, 16, 8) PNED_CASE(256, 8, 8) PNED_CASE(256, 4, 4) PNED_CASE(256, 2, 2) PNED_CASE(256, 1, 1)
+        PNED_CASE(512, 16, 8) PNED_CASE(512, 8, 8) PNED_CASE(512, 4, 4) PNED_CASE(512, 2, 2) PNED_CASE(512, 1, 1)
+        PNED_CASE(1024, 8, 4) PNED_CASE(1024, 4, 4) PNED_CASE(1024, 2, 2) PNED_CASE(1024, 1, 1)
+    } else {
+        PNED_CASE(256, 32, 16) PNED_CASE(256, 16, 16) PNED_CASE(256, 8, 8) PNED_CASE(256, 4, 4)
+        PNED_CASE(256, 2, 2) PNED_CASE(256, 1, 1)
+        PNED_CASE(512, 32, 16) PNED_CASE(512, 16, 16) PNED_CASE(512, 8, 8) PNED_CASE(512, 4, 4)
+        PNED_CASE(512, 2, 2) PNED_CASE(512, 1, 1)
+        PNED_CASE(1024, 16, 8) PNED_CASE(1024, 8, 8) PNED_CASE(1024, 4, 4) PNED_CASE(1024, 2, 2)
+        PNED_CASE(1024, 1, 1)
     }
 #undef PNED_CASE
     return fail(PNED_E_UNSUPPORTED, "no term-major H.v kernel instance for this tile shape");
@@ -663,7 +690,8 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
     PNED_TRY(ensure_partial(ctx, need));
     PNED_TRY(ensure_result(ctx, 2 * (size_t)nvec + 8));
     // per-pass address tables (rebuilt when the plan or the thread count changes)
-    const int TPBpref = (ctx->tune_threads == 256 || ctx->tune_threads == 512) ? ctx->tune_threads : 512;
+    const int TPBpref = (ctx->tune_threads == 256 || ctx->tune_threads == 512 || ctx->tune_threads == 1024)
+                            ? ctx->tune_threads : 512;
     std::vector<int> tpb(npass), rr(npass);
     for (int p = 0; p < npass; ++p) {
         int T = TPBpref;
