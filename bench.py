@@ -37,6 +37,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,7 +210,19 @@ def bench_tps(args):
     steps = args.steps if args.steps != 10 else 2000
     L = op.shift_L
 
+    tunes = {}
+
+    def tune_for(nvec, dtype, name):
+        # every path gets the tile plan the autotuner picks on vectors of its own shape
+        tv1, tv0 = ctx.vector(nvec, dtype), ctx.vector(nvec, dtype)
+        ctx.set_initial_vec(tv1, 1)
+        tunes[name] = ctx.autotune(tv1, tv0, reps=1)
+        tv1.free()
+        tv0.free()
+        _progress(f"autotuned tile plan for {name}: {tunes[name]}")
+
     def run_fused(dtype, nsteps):
+        tune_for(R, dtype, "fused_c64" if dtype == gpu.C64 else "fused_c128")
         v1, va, vb = (ctx.vector(R, dtype) for _ in range(3))
         ctx.set_initial_vec(v1, 20261202)
         ctx.sync()
@@ -219,9 +235,12 @@ def bench_tps(args):
         return dt, a, b, c
 
     t64, a64, b64, c64 = run_fused(gpu.C64, steps)
+    _progress(f"fused complex64 mTPQ: {steps} steps x {R} samples in {t64:.2f} s")
     t128, a128, b128, c128 = run_fused(gpu.C128, steps)
+    _progress(f"fused complex128 mTPQ: {steps} steps x {R} samples in {t128:.2f} s")
     # plain path: complex128, one sample, reference step structure; timed on a bounded sample
     plain_steps = min(200, steps)
+    tune_for(1, gpu.C128, "plain_c128")
     v1, v0, vg, vt = (ctx.vector(1, gpu.C128) for _ in range(4))
     ctx.set_initial_vec(v1, 20261202)
     ctx.sync()
@@ -229,6 +248,7 @@ def bench_tps(args):
     ctx.tps(v1, v0, vg, vt, L, plain_steps)
     ctx.sync()
     tplain = (time.perf_counter() - t) * (steps / plain_steps) * R
+    _progress(f"plain complex128 path: {plain_steps} steps of one sample, extrapolated {tplain:.2f} s")
     n = cl.n
     E64, E128 = a64 / n, a128 / n
     T64 = (L - a64) / (2.0 * np.arange(1, steps + 1))[:, None]
@@ -244,7 +264,7 @@ def bench_tps(args):
         "metric": "batched mTPQ speedup (fused complex64, R=64) over the plain FP64 GPU path",
         "value": round(tplain / t64, 3), "unit": "x", "higher_is_better": True,
         "config": {"workload": "mTPQ N=24 ref cluster, phi=90deg (AFM Kitaev K1=+2), R=64, L=ibond*l_b",
-                   "steps": steps, "c64_fused_s": round(t64, 3), "c128_fused_s": round(t128, 3),
+                   "steps": steps, "tune": tunes, "c64_fused_s": round(t64, 3), "c128_fused_s": round(t128, 3),
                    "plain_c128_s_extrapolated": round(tplain, 3),
                    "plain_sample": f"1 sample x {plain_steps} steps, scaled by {R} samples and {steps}/{plain_steps}",
                    "max_abs_err_E_per_site": float(np.max(np.abs(E64 - E128))),
@@ -271,6 +291,7 @@ def bench_serve(args):
     t = time.perf_counter()
     res = srv.run(reqs)
     cap = len(res) / (time.perf_counter() - t)
+    _progress(f"closed-loop capacity {cap:.2f} solves/s over {len(res)} requests")
     # open loop: Poisson arrivals at 80% of capacity
     reqs = serving.make_requests(nreq, specs, seed=20261203, rate=0.8 * cap)
     srv = serving.Server(tol=1e-10)
@@ -278,6 +299,7 @@ def bench_serve(args):
     res = srv.run(reqs)
     wall = time.perf_counter() - t
     batched = serving.summarize(res, wall)
+    _progress(f"continuous batching at 80% load: {batched}")
     seq = serving.Server(tol=1e-10, sequential=True)
     t = time.perf_counter()
     res2 = seq.run(serving.make_requests(nreq, specs, seed=20261203, rate=0.8 * cap))

@@ -541,10 +541,11 @@ int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr)
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
 {
     if (ctx->plan_colbits == colbits && ctx->plan_dtype == dtype) return PNED_OK;
-    // the whole tile sits in LDS (160 KB on gfx950): 2^13 complex128 or 2^14 complex64
+    // the whole tile sits in LDS (160 KB on gfx950): 2^13 complex128 or 2^14 complex64;
+    // untuned default one bit below that (the autotuner's usual choice at 2^24..2^32)
     const int kmax = (dtype == PNED_C128) ? 13 : 14;
     int K = ctx->tune_tile_bits;
-    if (K <= 0) K = kmax;
+    if (K <= 0) K = kmax - 1;
     if (K > kmax) K = kmax;
     int L = ctx->tune_low_bits > 0 ? ctx->tune_low_bits : 3;
     PNED_TRY(s12_make_passes(ctx->s12, colbits, K, L));

@@ -3,6 +3,7 @@
 // All reductions are two-stage and order-fixed (fixed grid, fixed per-thread strides,
 // fixed tree), so a result depends only on the vector contents: the same column gives
 // the same bits alone or batched with others.  Sums are FP64 for both storage types.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -437,59 +438,95 @@ __global__ __launch_bounds__(RED_TPB) void k_tps_obs(const typename V2<Real>::T 
 // Fused mTPQ step reductions over (uprev, u, w), per column:
 //  [0] |w|^2  [2..3] <uprev|w>  [4..5] <uprev|u>  [6] <u|Wzz|u>  [8..9] <uprev|Wzz|u>
 //  [10+2k..] <a_k|W_k|u> (a_k = u or uprev) for the stored structure-factor entries
+// Fused mTPQ reduction.  The observables' state weights (z-bond sum, structure-factor
+// sums W_k(s) over sites) depend on the state only, and the ncol interleaved columns of a
+// state share them: each chunk of CS states first computes them once per state into LDS
+// (phase A, spread over the block's threads), then every element of the chunk reads its
+// state's weights (phase B).  A thread's elements all lie in one column, so the block
+// reduction per column is the fixed tree of reduce_rows.
+constexpr int TPS_CS_MAX = 256;   // states per chunk
+
 template <typename Real, int NOUT>
 __global__ __launch_bounds__(RED_TPB) void k_tps_fused(const typename V2<Real>::T *__restrict__ up,
                                                        const typename V2<Real>::T *__restrict__ u,
                                                        const typename V2<Real>::T *__restrict__ w,
-                                                       const ObsArgs *__restrict__ ap, size_t nelem, int ncol,
-                                                       double *__restrict__ partial)
+                                                       const ObsArgs *__restrict__ ap, size_t nstates, int ncol,
+                                                       int cs, double *__restrict__ partial)
 {
     extern __shared__ double red[];
+    double *wt = red + NOUT * RED_TPB;   // [cs][1 + 2 nsq]: wzz, then (Re, Im) W_k
     const ObsArgs &a = *ap;
-    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    const int tid = threadIdx.x;
+    const int nsq = (10 + 2 * a.nsq <= NOUT) ? a.nsq : (NOUT - 10) / 2;
+    const int nw = 1 + 2 * nsq;
+    const int lc = __ffs(ncol) - 1;   // log2 ncol
     double acc[NOUT];
 #pragma unroll
     for (int k = 0; k < NOUT; ++k) acc[k] = 0.0;
-    for (size_t e = (size_t)blockIdx.x * RED_TPB + threadIdx.x; e < nelem; e += stride) {
-        const uint32_t s = (uint32_t)(e / (size_t)ncol);
-        double pr, pi, xr, xi, wr, wi;
-        ld(up, e, pr, pi);
-        ld(u, e, xr, xi);
-        ld(w, e, wr, wi);
-        acc[0] = fma(wr, wr, fma(wi, wi, acc[0]));
-        acc[2] = fma(pr, wr, fma(pi, wi, acc[2]));
-        acc[3] = fma(pr, wi, fma(-pi, wr, acc[3]));
-        acc[4] = fma(pr, xr, fma(pi, xi, acc[4]));
-        acc[5] = fma(pr, xi, fma(-pi, xr, acc[5]));
-        double wzz = 0.0;
-        for (int b = 0; b < a.nzz; ++b) {
-            const double s3 = (s & a.zz_mask1[b]) ? 0.5 : -0.5;
-            const double s4 = (s & a.zz_mask2[b]) ? 0.5 : -0.5;
-            wzz = fma(s3, s4, wzz);
-        }
-        acc[6] = fma(wzz, xr * xr + xi * xi, acc[6]);
-        acc[8] = fma(wzz, pr * xr + pi * xi, acc[8]);
-        acc[9] = fma(wzz, pr * xi - pi * xr, acc[9]);
-        const int s0 = (int)(s & 1u);
-        for (int k = 0; k < a.nsq && 10 + 2 * k + 1 < NOUT; ++k) {
-            const int qs = a.sq_q[k];
-            double cr = 0.0, ci = 0.0;
-            for (int j = 0; j < a.n; ++j) {
-                const int sj = (int)((s >> j) & 1u);
-                double wt;
-                if (a.sq_mode[k] == 0) wt = ((double)sj - 0.5) * ((double)s0 - 0.5);
-                else wt = (sj != s0) ? 0.25 : 0.0;
-                cr = fma(wt, a.phr[qs][j], cr);
-                ci = fma(wt, a.phi[qs][j], ci);
+    const size_t nchunk = (nstates + cs - 1) / cs;
+    for (size_t ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+        const size_t s0c = ch * (size_t)cs;
+        const int ns = (int)((nstates - s0c < (size_t)cs) ? (nstates - s0c) : (size_t)cs);
+        // phase A: per-state weights
+        for (int task = tid; task < ns * (1 + nsq); task += RED_TPB) {
+            const int sl = task % ns, kind = task / ns;
+            const uint32_t s = (uint32_t)(s0c + sl);
+            if (kind == 0) {
+                double wzz = 0.0;
+                for (int b = 0; b < a.nzz; ++b) {
+                    const double s3 = (s & a.zz_mask1[b]) ? 0.5 : -0.5;
+                    const double s4 = (s & a.zz_mask2[b]) ? 0.5 : -0.5;
+                    wzz = fma(s3, s4, wzz);
+                }
+                wt[sl * nw] = wzz;
+            } else {
+                const int k = kind - 1, qs = a.sq_q[k];
+                const int s0 = (int)(s & 1u);
+                double cr = 0.0, ci = 0.0;
+                for (int j = 0; j < a.n; ++j) {
+                    const int sj = (int)((s >> j) & 1u);
+                    double wj;
+                    if (a.sq_mode[k] == 0) wj = ((double)sj - 0.5) * ((double)s0 - 0.5);
+                    else wj = (sj != s0) ? 0.25 : 0.0;
+                    cr = fma(wj, a.phr[qs][j], cr);
+                    ci = fma(wj, a.phi[qs][j], ci);
+                }
+                wt[sl * nw + 1 + 2 * k] = cr;
+                wt[sl * nw + 2 + 2 * k] = ci;
             }
-            const double yr = cr * xr - ci * xi, yi = cr * xi + ci * xr;
-            double lr = xr, li = xi;
-            if (a.sq_left[k]) { lr = pr; li = pi; }
-            acc[10 + 2 * k] = fma(lr, yr, fma(li, yi, acc[10 + 2 * k]));
-            acc[10 + 2 * k + 1] = fma(lr, yi, fma(-li, yr, acc[10 + 2 * k + 1]));
         }
+        __syncthreads();
+        // phase B: the chunk's elements
+        const int nel = ns << lc;
+        for (int le = tid; le < nel; le += RED_TPB) {
+            const size_t e = (s0c << lc) + (size_t)le;
+            const double *ws = wt + (le >> lc) * nw;
+            double pr, pi, xr, xi, wr, wi;
+            ld(up, e, pr, pi);
+            ld(u, e, xr, xi);
+            ld(w, e, wr, wi);
+            acc[0] = fma(wr, wr, fma(wi, wi, acc[0]));
+            acc[2] = fma(pr, wr, fma(pi, wi, acc[2]));
+            acc[3] = fma(pr, wi, fma(-pi, wr, acc[3]));
+            acc[4] = fma(pr, xr, fma(pi, xi, acc[4]));
+            acc[5] = fma(pr, xi, fma(-pi, xr, acc[5]));
+            const double wzz = ws[0];
+            acc[6] = fma(wzz, xr * xr + xi * xi, acc[6]);
+            acc[8] = fma(wzz, pr * xr + pi * xi, acc[8]);
+            acc[9] = fma(wzz, pr * xi - pi * xr, acc[9]);
+#pragma unroll
+            for (int k = 0; 10 + 2 * k + 1 < NOUT; ++k) {
+                if (k >= nsq) break;
+                const double cr = ws[1 + 2 * k], ci = ws[2 + 2 * k];
+                const double yr = cr * xr - ci * xi, yi = cr * xi + ci * xr;
+                double lr = xr, li = xi;
+                if (a.sq_left[k]) { lr = pr; li = pi; }
+                acc[10 + 2 * k] = fma(lr, yr, fma(li, yi, acc[10 + 2 * k]));
+                acc[10 + 2 * k + 1] = fma(lr, yi, fma(-li, yr, acc[10 + 2 * k + 1]));
+            }
+        }
+        __syncthreads();
     }
-    const int tid = threadIdx.x;
 #pragma unroll
     for (int k = 0; k < NOUT; ++k) red[NOUT * tid + k] = acc[k];
     __syncthreads();
@@ -895,27 +932,31 @@ int tps_fused_reduce(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, const
     constexpr int NOUT = 10 + 2 * 8;
     if (10 + 2 * nsq > NOUT) return fail(PNED_E_UNSUPPORTED, "fused mTPQ reduction: at most 8 structure-factor entries");
     const int width = 10 + 2 * nsq;
-    const size_t nelem = (size_t)u->dim * u->nvec;
-    const int grid = RED_GRID / 4;
+    if (u->nvec > RED_TPB) return fail(PNED_E_UNSUPPORTED, "fused mTPQ reduction: at most 256 columns");
+    // chunk of cs states: about 16 elements per thread, at most TPS_CS_MAX states
+    const int cs = std::max(1, std::min(TPS_CS_MAX, 16 * RED_TPB / u->nvec));
+    const size_t nchunk = ((size_t)u->dim + cs - 1) / cs;
+    const int grid = (int)std::min<size_t>(nchunk, (size_t)RED_GRID);
     PNED_TRY(ensure_partial(ctx, (size_t)grid * NOUT * u->nvec));
     double *dargs;
     const int nd = (int)((sizeof(ObsArgs) + sizeof(double) - 1) / sizeof(double));
     std::vector<double> hbuf(nd);
     std::memcpy(hbuf.data(), &a, sizeof(ObsArgs));
     PNED_TRY(upload_scalars(ctx, hbuf.data(), nd, &dargs));
-    const size_t lds = sizeof(double) * NOUT * RED_TPB;
+    const size_t lds = sizeof(double) * ((size_t)NOUT * RED_TPB + (size_t)cs * (1 + 2 * nsq));
+    const size_t nstates = (size_t)u->dim;
     if (u->dtype == PNED_C128) {
         auto k = k_tps_fused<double, NOUT>;
         PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k, dim3(grid), dim3(RED_TPB), lds, ctx->stream, (const double2 *)up->data,
-                           (const double2 *)u->data, (const double2 *)w->data, (const ObsArgs *)dargs, nelem,
-                           u->nvec, ctx->d_partial);
+                           (const double2 *)u->data, (const double2 *)w->data, (const ObsArgs *)dargs, nstates,
+                           u->nvec, cs, ctx->d_partial);
     } else {
         auto k = k_tps_fused<float, NOUT>;
         PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k, dim3(grid), dim3(RED_TPB), lds, ctx->stream, (const float2 *)up->data,
-                           (const float2 *)u->data, (const float2 *)w->data, (const ObsArgs *)dargs, nelem,
-                           u->nvec, ctx->d_partial);
+                           (const float2 *)u->data, (const float2 *)w->data, (const ObsArgs *)dargs, nstates,
+                           u->nvec, cs, ctx->d_partial);
     }
     PNED_HIP(hipGetLastError());
     std::vector<double> all((size_t)NOUT * u->nvec);
