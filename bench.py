@@ -276,40 +276,51 @@ def bench_tps(args):
 
 
 def bench_serve(args):
-    """Mixed-size ground-state serving: continuous batching vs sequential plain solves."""
+    """Mixed-size ground-state serving: continuous batching vs sequential plain solves.
+
+    Capacity of each mode: all requests available at t = 0 (closed loop).  Latency: both
+    modes on the same Poisson arrivals at 80 % of the sequential capacity, a load both can
+    sustain."""
     import torch  # noqa: F401
 
     from pned_amd import serving
 
     specs = ["2x4", "3x3", "2x5", "3x4"] + (["2x7"] if args.serve_big else [])
     nreq = args.requests
-    # capacity: closed loop (all requests available at t=0)
     warm = serving.make_requests(8, specs, seed=1)
     serving.Server(tol=1e-10).run(warm)
-    reqs = serving.make_requests(nreq, specs, seed=20261202)
-    srv = serving.Server(tol=1e-10)
-    t = time.perf_counter()
-    res = srv.run(reqs)
-    cap = len(res) / (time.perf_counter() - t)
-    _progress(f"closed-loop capacity {cap:.2f} solves/s over {len(res)} requests")
-    # open loop: Poisson arrivals at 80% of capacity
-    reqs = serving.make_requests(nreq, specs, seed=20261203, rate=0.8 * cap)
-    srv = serving.Server(tol=1e-10)
-    t = time.perf_counter()
-    res = srv.run(reqs)
-    wall = time.perf_counter() - t
-    batched = serving.summarize(res, wall)
-    _progress(f"continuous batching at 80% load: {batched}")
-    seq = serving.Server(tol=1e-10, sequential=True)
-    t = time.perf_counter()
-    res2 = seq.run(serving.make_requests(nreq, specs, seed=20261203, rate=0.8 * cap))
-    wall2 = time.perf_counter() - t
-    plain = serving.summarize(res2, wall2)
-    out = {"metric": "mixed-size serving solves/s (continuous batching) vs sequential plain solves",
-           "value": batched["solves_per_s"], "unit": "solves/s", "higher_is_better": True,
-           "config": {"sizes": specs, "requests": nreq, "capacity_closed_loop": round(cap, 3),
-                      "arrival_rate": round(0.8 * cap, 3), "batched": batched, "sequential": plain,
-                      "tol": 1e-10}}
+    serving.Server(tol=1e-10, sequential=True).run(warm)
+
+    def capacity(sequential):
+        reqs = serving.make_requests(nreq, specs, seed=20261202)
+        srv = serving.Server(tol=1e-10, sequential=sequential)
+        t = time.perf_counter()
+        res = srv.run(reqs)
+        return len(res) / (time.perf_counter() - t)
+
+    cap_b = capacity(False)
+    _progress(f"continuous batching closed-loop capacity {cap_b:.2f} solves/s over {nreq} requests")
+    cap_s = capacity(True)
+    _progress(f"sequential closed-loop capacity {cap_s:.2f} solves/s over {nreq} requests")
+    rate = 0.8 * cap_s
+
+    def open_loop(sequential):
+        reqs = serving.make_requests(nreq, specs, seed=20261203, rate=rate)
+        srv = serving.Server(tol=1e-10, sequential=sequential)
+        t = time.perf_counter()
+        res = srv.run(reqs)
+        return serving.summarize(res, time.perf_counter() - t)
+
+    batched = open_loop(False)
+    _progress(f"continuous batching, Poisson {rate:.2f}/s: {batched}")
+    plain = open_loop(True)
+    _progress(f"sequential, Poisson {rate:.2f}/s: {plain}")
+    out = {"metric": "mixed-size serving capacity solves/s (continuous batching) vs sequential plain solves",
+           "value": round(cap_b, 3), "unit": "solves/s", "higher_is_better": True,
+           "config": {"sizes": specs, "requests": nreq, "tol": 1e-10,
+                      "capacity_batched": round(cap_b, 3), "capacity_sequential": round(cap_s, 3),
+                      "capacity_ratio": round(cap_b / cap_s, 3), "arrival_rate": round(rate, 3),
+                      "batched": batched, "sequential": plain}}
     print(json.dumps(out), flush=True)
 
 

@@ -538,14 +538,21 @@ int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr)
     return PNED_OK;
 }
 
+bool default_large(const pned_ctx *ctx, int colbits)
+{
+    return (double)ctx->dim * (double)(1ull << colbits) > (double)(1ull << 25);
+}
+
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
 {
     if (ctx->plan_colbits == colbits && ctx->plan_dtype == dtype) return PNED_OK;
-    // the whole tile sits in LDS (160 KB on gfx950): 2^13 complex128 or 2^14 complex64;
-    // untuned default one bit below that (the autotuner's usual choice at 2^24..2^32)
+    // the whole tile sits in LDS (160 KB on gfx950): 2^13 complex128 or 2^14 complex64.
+    // Untuned default (the autotuner's usual choice): vectors of up to 2^25 elements two
+    // bits below that limit (small tiles, 256 threads, more blocks in flight), larger
+    // vectors one bit below (1024 threads)
     const int kmax = (dtype == PNED_C128) ? 13 : 14;
     int K = ctx->tune_tile_bits;
-    if (K <= 0) K = kmax - 1;
+    if (K <= 0) K = default_large(ctx, colbits) ? kmax - 1 : kmax - 2;
     if (K > kmax) K = kmax;
     int L = ctx->tune_low_bits > 0 ? ctx->tune_low_bits : 3;
     PNED_TRY(s12_make_passes(ctx->s12, colbits, K, L));
@@ -735,6 +742,54 @@ int pned_ctx_destroy(pned_ctx *ctx)
 
 int64_t pned_ctx_dim(const pned_ctx *ctx) { return ctx ? ctx->dim : -1; }
 
+// New bond matrices whose terms and diagonal groups fit the current single-operator plan
+// (same rule as a batch column): replace the coefficients and diagonal weights and keep
+// the tile passes, so a sequence of operators on one cluster is planned once.  Returns 1
+// when the plan has to be rebuilt instead.
+static int refit_plan(pned_ctx *ctx, const double *xl, const int32_t *ll)
+{
+    S12Plan &pl = ctx->s12;
+    if (pl.nop != 1 || pl.passes.empty() || ctx->plan_colbits < 0 || pl.colcoef.empty()) return 1;
+    S12Plan trial = pl;
+    if (s12_set_column(trial, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), 0, xl, ll) != PNED_OK) return 1;
+    for (size_t t = 0; t < trial.terms.size(); ++t) {
+        Term &T = trial.terms[t];
+        T.c0r = trial.colcoef[t * 4 + 0];
+        T.c0i = trial.colcoef[t * 4 + 1];
+        T.c1r = trial.colcoef[t * 4 + 2];
+        T.c1i = trial.colcoef[t * 4 + 3];
+    }
+    for (DevTerm &d : trial.devterms) {
+        const Term &T = trial.terms[d.pad];
+        d.c0r = T.c0r;
+        d.c0i = T.c0i;
+        d.c1r = T.c1r;
+        d.c1i = T.c1i;
+        d.is_real = (T.c0i == 0.0 && T.c1i == 0.0) ? 1u : 0u;
+    }
+    const size_t ng = trial.diag.size();
+    for (size_t g = 0; g < ng; ++g) {
+        trial.diag[g].wr = trial.coldiag[2 * g];
+        trial.diag[g].wi = trial.coldiag[2 * g + 1];
+    }
+    trial.c0r = trial.coldiag[2 * ng];
+    trial.c0i = trial.coldiag[2 * ng + 1];
+    pl = std::move(trial);
+    if (!pl.devterms.empty())
+        PNED_HIP(hipMemcpy(ctx->d_terms, pl.devterms.data(), sizeof(DevTerm) * pl.devterms.size(),
+                           hipMemcpyHostToDevice));
+    DiagDev &dh = ctx->diag_host;
+    for (size_t g = 0; g < ng; ++g) {
+        dh.wr[g] = pl.diag[g].wr;
+        dh.wi[g] = pl.diag[g].wi;
+    }
+    dh.c0r = pl.c0r;
+    dh.c0i = pl.c0i;
+    PNED_HIP(hipMemcpy(ctx->d_diag, &dh, sizeof(DiagDev), hipMemcpyHostToDevice));
+    ctx->pd_threads = -1;   // the term-major table carries the coefficients
+    return PNED_OK;
+}
+
 int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t *llocal_ham)
 {
     if (!ctx || !xlocal_ham || !llocal_ham) return fail(PNED_E_ARG, "null argument");
@@ -742,6 +797,10 @@ int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t
     ctx->xl.assign(xlocal_ham, xlocal_ham + 2 * qq * qq * 9);
     ctx->ll.assign(llocal_ham, llocal_ham + qq * qq * 9);
     PNED_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->q == 2) {
+        const int rc = refit_plan(ctx, xlocal_ham, llocal_ham);
+        if (rc <= 0) return rc;   // refitted, or a HIP error
+    }
     if (ctx->q == 2) {
         PNED_TRY(s12_decompose(ctx->n, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), xlocal_ham, llocal_ham,
                                ctx->s12));

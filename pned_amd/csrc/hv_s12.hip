@@ -692,7 +692,8 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
     // per-pass address tables (rebuilt when the plan or the thread count changes)
     const int TPBpref = (ctx->tune_threads == 256 || ctx->tune_threads == 512 || ctx->tune_threads == 1024)
                             ? ctx->tune_threads
-                            : (ctx->s12_kernel == 1 ? 1024 : 512);
+                            : (ctx->s12_kernel == 1 && default_large(ctx, colbits) ? 1024
+                                                                                    : (ctx->s12_kernel == 1 ? 256 : 512));
     std::vector<int> tpb(npass), rr(npass);
     for (int p = 0; p < npass; ++p) {
         int T = TPBpref;

@@ -185,6 +185,8 @@ int ensure_partial(pned_ctx *ctx, size_t ndoubles);
 int ensure_result(pned_ctx *ctx, size_t ndoubles);
 int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
+// untuned tile defaults: true for vectors above 2^25 elements (dim x columns)
+bool default_large(const pned_ctx *ctx, int colbits);
 int upload_batch_coefficients(pned_ctx *ctx);
 
 // kernels (launchers), all asynchronous on ctx->stream unless stated

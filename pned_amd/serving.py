@@ -86,7 +86,9 @@ class Pool:
         cl = lattice.get_cluster(spec)
         # a structural template: generic couplings and a field give every term/group
         tmpl = [model.build_operator(cl, 20, 37.0 + 11.0 * k, field=0.2 + 0.01 * k) for k in range(slots)]
-        self.ctx = gpu.Context(tmpl[0], batch_ops=tmpl)
+        # one slot: a plain context whose bond matrices are replaced per request (the plan
+        # is refitted, not rebuilt); more: a batch context with one operator per column
+        self.ctx = gpu.Context(tmpl[0]) if slots == 1 else gpu.Context(tmpl[0], batch_ops=tmpl)
         self.dim = self.ctx.dim
         self.v1 = self.ctx.vector(slots)
         self.v0 = self.ctx.vector(slots)
@@ -110,7 +112,10 @@ class Pool:
         return any(r is not None for r in self.req)
 
     def admit(self, c: int, r: Request, now: float):
-        self.ctx.set_column_matrices(c, operator_for(r))
+        if self.slots == 1:
+            self.ctx.set_matrices(operator_for(r))
+        else:
+            self.ctx.set_column_matrices(c, operator_for(r))
         self.ctx.set_initial_column(self.v1, c, r.seed)
         self.req[c] = r
         self.mode[c] = 1
@@ -161,7 +166,10 @@ class Server:
         self.tol = tol
         self.max_steps = max_steps
         self.sequential = sequential
-        self.slots_for = slots_for or (lambda n: 1 if sequential else (8 if n <= 24 else 4))
+        # slots per pool: batching amortises launches and host round trips, which dominate
+        # up to N = 20; from N = 22 up a step is HBM-bound, a batched column costs as much
+        # as a plain one (and idle slots cost the same), so those sizes run one at a time
+        self.slots_for = slots_for or (lambda n: 1 if sequential else (8 if n <= 20 else 1))
         self.pools: Dict[str, Pool] = {}
         self.queues: Dict[str, "queue.Queue[Request]"] = {}
         self.used = 0

@@ -341,3 +341,28 @@ def test_spin1_tiled_vs_gather_n18():
     ref = np.sqrt(ctx.calc_vec_inprdct(a, a).real)
     assert diff <= 1e-12 * ref
     assert abs(p1 - p2) <= 1e-12 * abs(p2)
+
+
+def test_set_matrices_refits_plan():
+    """Replacing the bond matrices on a planned context keeps its tile passes when the
+    new operator fits them (coefficients refitted) and still matches the oracle; an
+    operator without field terms fits with those coefficients zero."""
+    _say("set_matrices refit")
+    cl = lattice.reference_cluster(16)
+    opa = model.build_operator(cl, 20, 126.0, field=0.1)
+    ctx = gpu.Context(opa)
+    x = _rand(ctx.dim, 21)
+    v1, v0 = ctx.vector(), ctx.vector()
+    v1.upload(x)
+    ctx.hv_fresh(v1, v0)
+    plan_a = ctx.plan_info()
+    for phi, h in ((37.0, 0.25), (200.0, 0.0), (300.0, 0.4)):
+        opb = model.build_operator(cl, 20, phi, field=h)
+        ctx.set_matrices(opb)
+        pr = ctx.hv_fresh(v1, v0, 0.03)
+        want = np.zeros_like(x)
+        pr_ref = oracle.hv(oracle.OracleOp(opb), x, want, eperbd=0.03)
+        got = v0.download()
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), (phi, h)
+        assert abs(pr - pr_ref) <= 1e-13 * np.linalg.norm(x) * np.linalg.norm(want)
+        assert ctx.plan_info() == plan_a, "plan rebuilt although the operator fits it"

@@ -99,3 +99,18 @@ def test_server_results_match_oracle():
         x = oracle.gaussian(op.dim, q.seed)
         itr, a, b, ene = oracle.lanczos(oracle.OracleOp(op), x, max_itr=1500, nvec=1)
         assert abs(r.e0 - ene[0]) <= 1e-10 * abs(ene[0]), (q, r.e0, ene[0])
+
+
+def test_sequential_server_matches_oracle():
+    """The one-slot (plain context, matrices replaced per request) serving path."""
+    _say("sequential server")
+    reqs = serving.make_requests(6, ["2x4", "3x3"], seed=13)
+    srv = serving.Server(hbm_budget_bytes=50e9, tol=1e-10, sequential=True)
+    res = srv.run(reqs)
+    assert len(res) == len(reqs)
+    for r, q in zip(sorted(res, key=lambda z: z.rid), sorted(reqs, key=lambda z: z.rid)):
+        assert r.rid == q.rid and r.resid < 1e-10
+        op = serving.operator_for(q)
+        x = oracle.gaussian(op.dim, q.seed)
+        itr, a, b, ene = oracle.lanczos(oracle.OracleOp(op), x, max_itr=1500, nvec=1)
+        assert abs(r.e0 - ene[0]) <= 1e-10 * abs(ene[0]), (q, r.e0, ene[0])
