@@ -48,7 +48,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1 << 27)
-    ap.add_argument("--cfg", default="N32")
+    ap.add_argument("--cfg", default="N32", help="N32 (default), N24, TPS, SERVE, LANCZOS, SPIN1")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="", help="fixed plan K,L,threads (skips the autotuner)")
     ap.add_argument("--requests", type=int, default=120, help="SERVE: number of requests")
@@ -96,6 +96,10 @@ def main():
 
     if args.cfg == "TPS":
         return bench_tps(args)
+    if args.cfg == "LANCZOS":
+        return bench_lanczos(args)
+    if args.cfg == "SPIN1":
+        return bench_spin1(args)
     if args.cfg == "SERVE":
         return bench_serve(args)
 
@@ -186,6 +190,131 @@ def main():
                      "per_launch": round(achieved * npass, 1)},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def _ground_state(ctx, gpu, v1, v0, x, seed, max_itr):
+    """Lanczos until the lowest eigenvalue converges (the reference's criterion), then
+    3x as many fixed steps so the Ritz vector's residual is at round-off, and the Ritz
+    vector itself (a second Lanczos from the same start)."""
+    ctx.set_initial_vec(v1, seed)
+    t = time.perf_counter()
+    itr, alpha, beta, ene = ctx.lnc1(v1, v0, max_itr=max_itr, nvec=1)
+    ctx.set_initial_vec(v1, seed)
+    itr, alpha, beta, ene = ctx.lnc1(v1, v0, fixed_steps=min(3 * itr, max_itr), nvec=1)
+    coef = gpu.vec12(alpha, beta, [ene[0]], 1)[0]
+    ctx.set_initial_vec(v1, seed)
+    ctx.ritz_vector(v1, v0, x, alpha, beta, coef)
+    ctx.sync()
+    dt = time.perf_counter() - t
+    res = ctx.residual(x, ene[0], v0)
+    return itr, float(ene[0]), res, dt
+
+
+def bench_lanczos(args):
+    """Plain FP64 Lanczos at N=24 (reference 24-site cluster and the 3x4 torus), MODEL 20
+    phi=54 deg, h=0, complex128: 300 fixed steps (timed), then the converged ground state
+    and its residual |Hx - E0 x| / |x|."""
+    import numpy as np  # noqa: F401
+    import torch  # noqa: F401
+
+    from pned_amd import gpu, lattice, model
+
+    rows = {}
+    for spec in (24, "3x4"):
+        cl = lattice.get_cluster(spec)
+        op = model.build_operator(cl, 20, 54.0)
+        ctx = gpu.Context(op)
+        v1, v0, x = ctx.vector(), ctx.vector(), ctx.vector()
+        ctx.set_initial_vec(v1, 1)
+        tune = ctx.autotune(v1, v0, reps=2)
+        ctx.set_initial_vec(v1, 20261202)
+        ctx.sync()
+        t = time.perf_counter()
+        itr, alpha, beta, ene = ctx.lnc1(v1, v0, fixed_steps=300, nvec=1)
+        ctx.sync()
+        t300 = time.perf_counter() - t
+        itc, e0, res, tgs = _ground_state(ctx, gpu, v1, v0, x, 20261202, 2000)
+        rows[str(spec)] = {"n_sites": cl.n, "dim": ctx.dim, "tune": tune, "ms_per_step_300": round(t300 / 300 * 1e3, 4),
+                           "E0_per_site_300": float(ene[0]) / cl.n, "converged_steps": itc,
+                           "E0_per_site": e0 / cl.n, "residual": res, "ground_state_s": round(tgs, 3)}
+        _progress(f"Lanczos {spec}: {rows[str(spec)]}")
+        for v in (v1, v0, x):
+            v.free()
+        ctx.close()
+    out = {"metric": "plain FP64 Lanczos ms per step (N=24, complex128)", "value": rows["24"]["ms_per_step_300"],
+           "unit": "ms/step", "higher_is_better": False, "config": rows}
+    print(json.dumps(out), flush=True)
+
+
+def bench_spin1(args):
+    """Spin-1 N=18 reference cluster (3^18 states), MODEL 20 phi=108 deg, complex128:
+    ground state to 1e-10 (Lanczos + Ritz vector), S^z(q) DSF at the cluster's 8 momenta
+    plus (pi, pi/sqrt 3) (300 continued-fraction steps, eta=0.05, 2000 omegas on [0,5)),
+    and a COCG solve of (H - E0 - omega - i eta) x = S^z(q)|0> at omega = 1 checked against
+    the continued fraction."""
+    import numpy as np
+    import torch  # noqa: F401
+
+    from pned_amd import gpu, lattice, model
+
+    cl = lattice.reference_cluster(18)
+    op = model.build_operator(cl, 20, 108.0, q=3)
+    ctx = gpu.Context(op)
+    v1, v0, x, w = (ctx.vector() for _ in range(4))
+    ctx.set_initial_vec(v1, 1)
+    ctx.sync()
+    t = time.perf_counter()
+    nhv = 20
+    for _ in range(nhv):
+        ctx.p_mltply_v3(v1, v0)
+    ctx.sync()
+    hv_ms = (time.perf_counter() - t) / nhv * 1e3
+    _progress(f"spin-1 N=18 H.v {hv_ms:.3f} ms ({3 * 16 * ctx.dim / (hv_ms * 1e-3) / 1e9:.0f} GB/s algorithmic)")
+    itc, e0, res, tgs = _ground_state(ctx, gpu, v1, v0, x, 20261202, 3000)
+    _progress(f"ground state: E0/N {e0 / cl.n:.12f}, {itc} steps, residual {res:.2e}, {tgs:.2f} s")
+    qs = [np.asarray(q, float) for q in cl.q_list] + [np.array([np.pi, np.pi / np.sqrt(3.0)])]
+    eta, nomega, split, steps = 0.05, 2000, 2.5e-3, 300
+    t = time.perf_counter()
+    dsf_rows = []
+    cf_at = {}
+    for k, q in enumerate(qs):
+        cn = ctx.intalv_zz(x, w, q, cl.site_vec)
+        sa, sb = ctx.dsflnc1z(w, v0, cn, steps)
+        ww, dsf, conv = ctx.renbun(sa, sb, e0, eta, nomega, split, steps, sq=cn)
+        dsf_rows.append({"q": [round(float(q[0]), 6), round(float(q[1]), 6)], "S_q": cn,
+                         "peak_omega": float(ww[int(np.argmax(dsf))]), "converged": int(conv.sum())})
+        cf_at[k] = (sa, sb, cn)
+        _progress(f"DSF q={dsf_rows[-1]['q']}: S(q)={cn:.6f}, peak at omega={dsf_rows[-1]['peak_omega']:.4f}")
+    tdsf = time.perf_counter() - t
+    _progress(f"DSF at {len(qs)} momenta in {tdsf:.2f} s")
+    # COCG at omega = 1 for the first nonzero momentum, against the continued fraction
+    k = 1
+    sa, sb, cn = cf_at[k]
+    b, xs, r, p, y = w, ctx.vector(), ctx.vector(), ctx.vector(), ctx.vector()
+    ctx.intalv_zz(x, b, qs[k], cl.site_vec)
+    omega = 1.0
+    t = time.perf_counter()
+    try:
+        it, cres = ctx.cocg(e0 + omega, -eta, b, xs, r, p, y, max_it=4000, tol=1e-10)
+    except gpu.PnedError as err:   # not converged within max_it: report what it reached
+        it, cres = 4000, float("nan")
+        _progress(f"COCG: {err}")
+    tcocg = time.perf_counter() - t
+    g_cocg = -np.vdot(b.download(), xs.download())
+    z = omega + e0 + 1j * eta
+    g = 0.0
+    for m in range(steps - 1, 0, -1):
+        g = sb[m] ** 2 / (z - sa[m] - g)
+    g_cf = 1.0 / (z - sa[0] - g)
+    rel = abs(g_cocg - g_cf) / abs(g_cf)
+    _progress(f"COCG at omega=1: {it} iterations in {tcocg:.2f} s, residual {cres:.1e}, |G_cocg - G_cf|/|G_cf| = {rel:.2e}")
+    out = {"metric": "spin-1 N=18 pipeline (ground state + 9-momentum DSF + COCG check)", "value": round(tgs + tdsf, 3),
+           "unit": "s", "higher_is_better": False,
+           "config": {"dim": ctx.dim, "hv_ms": round(hv_ms, 4), "E0_per_site": e0 / cl.n, "gs_steps": itc,
+                      "gs_residual": res, "gs_s": round(tgs, 3), "dsf_s": round(tdsf, 3), "dsf": dsf_rows,
+                      "cocg": {"omega": omega, "eta": eta, "q": dsf_rows[k]["q"], "iterations": it, "residual": cres,
+                               "s": round(tcocg, 3), "rel_diff_vs_continued_fraction": rel}}}
     print(json.dumps(out), flush=True)
 
 

@@ -808,6 +808,7 @@ int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t
         ctx->plan_dtype = -1;
     } else {
         reset_spin_tables(ctx);
+        free_spin_plan(ctx);   // the tiled passes carry the coefficients
     }
     return PNED_OK;
 }
@@ -893,6 +894,11 @@ int pned_ctx_plan_info(const pned_ctx *ctx, int *npasses, int *terms_per_pass, i
 {
     if (!ctx || !npasses) return fail(PNED_E_ARG, "null argument");
     if (ctx->q != 2) {
+        if (ctx->spin_tiled && ctx->spin_pd) {   // tiled passes: bonds per pass
+            *npasses = ctx->spin_np;
+            for (int p = 0; p < ctx->spin_np && p < max && terms_per_pass; ++p) terms_per_pass[p] = ctx->spin_nb[p];
+            return PNED_OK;
+        }
         *npasses = 1;
         if (terms_per_pass && max > 0) terms_per_pass[0] = ctx->ibond;
         return PNED_OK;

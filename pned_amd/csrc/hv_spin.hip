@@ -228,91 +228,149 @@ int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperb
 // that pass from the tile staged in LDS.  The tile is the q^K states sharing the digits
 // of the other sites; tile-local index j = sum digit_k q^k over the tile sites in order.
 // Each pass reads v1 once and read-modify-writes v0 once (pass 0 may start from zero).
+//
+// Per pass the host lists, for every (bond, output local state), the nonzero elements as
+// (partner offset in tile units, coefficient); the kernel stages that list in LDS once per
+// block.  A thread owns R tile slots j = tid + TPB r: their digit words and memory offsets
+// are computed once per launch and stay in registers, the next tile's v1 is prefetched
+// into registers while the current one is computed, and the slots are accumulated in
+// sweeps of S so only S accumulators and v0 values are live.
 // ------------------------------------------------------------------------------------
 constexpr int SP_MAX_TILE = 8192;     // states per tile (128 KB of complex128)
 constexpr int SP_MAX_BOND = 64;
 constexpr int SP_BASE_TAB = 4096;
+constexpr int SP_MAX_QQ = 25;         // q <= 5
+constexpr int SP_MAX_ENT = 4096;      // nonzero entries per pass
 
 struct SpinPassDev {
     uint64_t off[SP_MAX_TILE];        // memory offset of tile-local state j
     uint64_t blo[SP_BASE_TAB];        // tile base, low outer digits
     uint64_t bhi[SP_BASE_TAB];        // tile base, high outer digits
-    int32_t bond_t[SP_MAX_BOND];      // bond type (0-based)
     int32_t bond_pa[SP_MAX_BOND];     // tile position of the min site
     int32_t bond_pb[SP_MAX_BOND];     // tile position of the max site
-    int32_t bond_wa[SP_MAX_BOND];     // q^pa
-    int32_t bond_wb[SP_MAX_BOND];     // q^pb
+    int32_t row_start[SP_MAX_BOND * SP_MAX_QQ + 1];   // entries of (bond, istat) row
+    int32_t ent_delta[SP_MAX_ENT];    // partner - j in tile units (0 = diagonal)
+    double ent_c[2 * SP_MAX_ENT];     // coefficient (re, im)
 };
 
 struct SpinPassArgs {
     uint64_t tile_count;
     int tsz;                          // q^K
     int K, nbond, diag_first, fresh;
+    int nent;                         // entries of this pass
     int nlo_mod;                      // tile_count split: tix % nlo_mod, tix / nlo_mod
     double shift;                     // -eperbd per bond diagonal
 };
 
-template <int Q, int TPB, int R>
+template <int Q, int TPB, int R, int S>
 __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ v1, double2 *__restrict__ v0,
-                                                    SpinTables tb, const SpinPassDev *__restrict__ pd,
-                                                    SpinPassArgs a, double *__restrict__ partial)
+                                                    const SpinPassDev *__restrict__ pd, SpinPassArgs a,
+                                                    double *__restrict__ partial)
 {
-    __shared__ double2 tile[TPB * R];
-    __shared__ double red[2 * TPB];
     constexpr int QQ = Q * Q;
+    extern __shared__ unsigned char spin_smem[];
+    double2 *tile = reinterpret_cast<double2 *>(spin_smem);
+    // staged row table: starts, deltas, coefficients
+    const int nrow = a.nbond * QQ;
+    int32_t *rs = reinterpret_cast<int32_t *>(spin_smem + sizeof(double2) * (size_t)a.tsz);
+    int32_t *dl = rs + ((nrow + 1 + 3) & ~3);
+    double2 *cf = reinterpret_cast<double2 *>(dl + ((a.nent + 3) & ~3));
     const int tid = threadIdx.x;
+    for (int i = tid; i <= nrow; i += TPB) rs[i] = pd->row_start[i];
+    for (int i = tid; i < a.nent; i += TPB) {
+        dl[i] = pd->ent_delta[i];
+        cf[i] = make_double2(pd->ent_c[2 * i], pd->ent_c[2 * i + 1]);
+    }
     const bool rd0 = !(a.diag_first && a.fresh);
-    double pr_re = 0.0, pr_im = 0.0;
-    for (uint64_t tix = blockIdx.x; tix < a.tile_count; tix += gridDim.x) {
-        const uint64_t base = pd->blo[tix % (uint64_t)a.nlo_mod] + pd->bhi[tix / (uint64_t)a.nlo_mod];
+    // per-slot digit words (4 bits per tile digit) and memory offsets, fixed for the launch
+    uint32_t dw[R];
+    uint64_t offj[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int j = tid + TPB * r;
-            if (j < a.tsz) tile[j] = v1[base + pd->off[j]];
+    for (int r = 0; r < R; ++r) {
+        const int j = tid + TPB * r;
+        uint32_t w = 0;
+        int x = j;
+        for (int k = 0; k < a.K; ++k) {
+            w |= (uint32_t)(x % Q) << (4 * k);
+            x /= Q;
         }
-        __syncthreads();
+        dw[r] = w;
+        offj[r] = (j < a.tsz) ? pd->off[j] : 0;
+    }
+    double pr_re = 0.0, pr_im = 0.0;
+    double xr[R], xi[R];
+    auto tile_base = [&](uint64_t tix) {
+        return pd->blo[tix % (uint64_t)a.nlo_mod] + pd->bhi[tix / (uint64_t)a.nlo_mod];
+    };
+    auto fetch_x = [&](uint64_t base) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int j = tid + TPB * r;
-            if (j >= a.tsz) continue;
-            const uint64_t m = base + pd->off[j];
-            double2 y = make_double2(0.0, 0.0);
-            if (rd0) y = v0[m];
-            // digits of j, 4 bits each
-            uint64_t dig = 0;
-            {
-                int x = j;
-                for (int k = 0; k < a.K; ++k) {
-                    dig |= (uint64_t)(x % Q) << (4 * k);
-                    x /= Q;
+            if (tid + TPB * r < a.tsz) {
+                const double2 t = v1[base + offj[r]];
+                xr[r] = t.x;
+                xi[r] = t.y;
+            }
+        }
+    };
+    uint64_t tix = blockIdx.x;
+    if (tix < a.tile_count) fetch_x(tile_base(tix));
+    __syncthreads();   // staged table visible
+    for (; tix < a.tile_count; tix += gridDim.x) {
+        const uint64_t base = tile_base(tix);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (tid + TPB * r < a.tsz) tile[tid + TPB * r] = make_double2(xr[r], xi[r]);
+        __syncthreads();
+        const uint64_t nx = tix + gridDim.x;
+        // v0 of the first sweep before the next tile's v1 (in-order vector memory counters)
+        double yr[S], yi[S];
+        auto fetch_y = [&](int r0) {
+#pragma unroll
+            for (int u = 0; u < S; ++u) {
+                const int r = r0 + u;
+                yr[u] = 0.0;
+                yi[u] = 0.0;
+                if (rd0 && tid + TPB * r < a.tsz) {
+                    const double2 t = v0[base + offj[r]];
+                    yr[u] = t.x;
+                    yi[u] = t.y;
                 }
             }
-            double ar = 0.0, ai = 0.0;
-            for (int b = 0; b < a.nbond; ++b) {
-                const int pa = pd->bond_pa[b], pb = pd->bond_pb[b];
-                const int d1 = (int)((dig >> (4 * pa)) & 15u), d2 = (int)((dig >> (4 * pb)) & 15u);
-                const int istat = Q * d2 + d1;
-                const int bt = pd->bond_t[b];
-                const int row = bt * QQ + istat;
-                const int cnt = tb.nzcnt[row];
-                const int wa = pd->bond_wa[b], wb = pd->bond_wb[b];
-                for (int k = 0; k < cnt; ++k) {
-                    const int ml = tb.nzin[row * QQ + k];
-                    double cr = tb.nzc[2 * (row * QQ + k)], ci = tb.nzc[2 * (row * QQ + k) + 1];
-                    if (ml == istat) cr += a.shift;
-                    const int pj = j + (ml % Q - d1) * wa + (ml / Q - d2) * wb;
-                    const double2 p = tile[pj];
-                    ar = fma(cr, p.x, fma(-ci, p.y, ar));
-                    ai = fma(cr, p.y, fma(ci, p.x, ai));
+        };
+        fetch_y(0);
+        if (nx < a.tile_count) fetch_x(tile_base(nx));
+#pragma unroll
+        for (int r0 = 0; r0 < R; r0 += S) {
+            if (r0 > 0) fetch_y(r0);
+#pragma unroll
+            for (int u = 0; u < S; ++u) {
+                const int r = r0 + u;
+                const int j = tid + TPB * r;
+                if (j >= a.tsz) continue;
+                double ar = 0.0, ai = 0.0;
+                for (int b = 0; b < a.nbond; ++b) {
+                    const int d1 = (int)((dw[r] >> (4 * pd->bond_pa[b])) & 15u);
+                    const int d2 = (int)((dw[r] >> (4 * pd->bond_pb[b])) & 15u);
+                    const int row = b * QQ + Q * d2 + d1;
+                    const int e1 = rs[row + 1];
+                    for (int e = rs[row]; e < e1; ++e) {
+                        const int d = dl[e];
+                        const double2 c = cf[e];
+                        const double cr = (d == 0) ? c.x + a.shift : c.x;
+                        const double2 p = tile[j + d];
+                        ar = fma(cr, p.x, fma(-c.y, p.y, ar));
+                        ai = fma(cr, p.y, fma(c.y, p.x, ai));
+                    }
                 }
+                const double2 x = tile[j];
+                pr_re += x.x * ar + x.y * ai;
+                pr_im += x.x * ai - x.y * ar;
+                v0[base + offj[r]] = make_double2(yr[u] + ar, yi[u] + ai);
             }
-            const double2 x = tile[j];
-            pr_re += x.x * ar + x.y * ai;
-            pr_im += x.x * ai - x.y * ar;
-            v0[m] = make_double2(y.x + ar, y.y + ai);
         }
         __syncthreads();
     }
+    double *red = reinterpret_cast<double *>(spin_smem);
     red[2 * tid] = pr_re;
     red[2 * tid + 1] = pr_im;
     __syncthreads();
@@ -373,87 +431,131 @@ static std::vector<std::vector<int>> spin_cover(int n, const std::vector<int32_t
     return out;
 }
 
+// LDS bytes of one pass: tile, then the row table (starts, deltas, coefficients)
+static size_t spin_lds_bytes(int tsz, int nrow, int nent, int tpb)
+{
+    const size_t t = sizeof(double2) * (size_t)tsz + sizeof(int32_t) * (size_t)((nrow + 1 + 3) & ~3) +
+                     sizeof(int32_t) * (size_t)((nent + 3) & ~3) + sizeof(double2) * (size_t)nent;
+    return std::max(t, sizeof(double) * 2 * (size_t)tpb);
+}
+
+template <int Q, int TPB, int R>
+static void launch_spin_pass(pned_ctx *ctx, int grid, size_t lds, const pned_vec *v1, pned_vec *v0,
+                             const SpinPassDev *pd, const SpinPassArgs &a, double *part)
+{
+    constexpr int S = (R < 4) ? R : 4;
+    auto k = hv_spin_pass<Q, TPB, R, S>;
+    (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(TPB), lds, ctx->stream, (const double2 *)v1->data, (double2 *)v0->data,
+                       pd, a, part);
+}
+
 int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                          double *prdct_host)
 {
-    const int q = ctx->q, n = ctx->n;
-    SpinTables tb;
-    PNED_TRY(build_spin_tables(ctx, tb));
-    // tile sites: the largest K with q^K <= 8192
-    int K = 0;
-    {
-        int64_t p = 1;
-        while (K < n && p * q <= SP_MAX_TILE) { p *= q; ++K; }
-    }
-    if (ctx->tune_tile_bits > 0 && ctx->tune_tile_bits < K) K = ctx->tune_tile_bits;
-    const int L = std::min(2, K);
-    if (ctx->spin_pd == nullptr || ctx->spin_K != K) {
-        auto cov = spin_cover(n, ctx->site_pair, ctx->ibond, K, L);
-        if (cov.empty()) return fail(PNED_E_UNSUPPORTED, "spin-S tile planner made no progress");
-        const int np = (int)cov.size() / 2;
-        std::vector<SpinPassDev> host(np);
-        std::vector<int> nb(np), tsz(np), nlo(np), nout(np);
-        for (int p = 0; p < np; ++p) {
-            const std::vector<int> &sites = cov[2 * p], &bonds = cov[2 * p + 1];
-            SpinPassDev &d = host[p];
-            std::memset(&d, 0, sizeof(d));
-            if ((int)bonds.size() > SP_MAX_BOND) return fail(PNED_E_UNSUPPORTED, "too many bonds in a spin-S pass");
-            std::vector<uint64_t> pw(n);
-            uint64_t x = 1;
-            for (int k = 0; k < n; ++k) { pw[k] = x; x *= (uint64_t)q; }
-            const int Kp = (int)sites.size();
-            int ts = 1;
-            for (int k = 0; k < Kp; ++k) ts *= q;
-            for (int j = 0; j < ts; ++j) {
-                uint64_t o = 0;
-                int y = j;
-                for (int k = 0; k < Kp; ++k) { o += (uint64_t)(y % q) * pw[sites[k]]; y /= q; }
-                d.off[j] = o;
-            }
-            std::vector<int> outer;
-            for (int st = 0; st < n; ++st)
-                if (std::find(sites.begin(), sites.end(), st) == sites.end()) outer.push_back(st);
-            const int no = (int)outer.size();
-            int nl = 0;
-            int64_t lm = 1;
-            while (nl < no && lm * q <= SP_BASE_TAB) { lm *= q; ++nl; }
-            int64_t hm = 1;
-            for (int k = nl; k < no; ++k) hm *= q;
-            if (hm > SP_BASE_TAB) return fail(PNED_E_UNSUPPORTED, "too many outer sites for the spin-S tables");
-            for (int64_t t2 = 0; t2 < lm; ++t2) {
-                uint64_t o = 0;
-                int64_t y = t2;
-                for (int k = 0; k < nl; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
-                d.blo[t2] = o;
-            }
-            for (int64_t t2 = 0; t2 < hm; ++t2) {
-                uint64_t o = 0;
-                int64_t y = t2;
-                for (int k = nl; k < no; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
-                d.bhi[t2] = o;
-            }
-            for (size_t b = 0; b < bonds.size(); ++b) {
-                const int bi = bonds[b];
-                const int sa = ctx->site_pair[2 * bi], sb = ctx->site_pair[2 * bi + 1];
-                const int pa = (int)(std::find(sites.begin(), sites.end(), sa) - sites.begin());
-                const int pbp = (int)(std::find(sites.begin(), sites.end(), sb) - sites.begin());
-                d.bond_t[b] = ctx->btype[bi] - 1;
-                d.bond_pa[b] = pa;
-                d.bond_pb[b] = pbp;
-                int wa = 1, wb = 1;
-                for (int k = 0; k < pa; ++k) wa *= q;
-                for (int k = 0; k < pbp; ++k) wb *= q;
-                d.bond_wa[b] = wa;
-                d.bond_wb[b] = wb;
-            }
-            nb[p] = (int)bonds.size();
-            tsz[p] = ts;
-            nlo[p] = (int)lm;
-            nout[p] = no;
-            (void)nout;
+    const int q = ctx->q, n = ctx->n, qq = q * q;
+    const int L0 = (ctx->tune_low_bits > 0) ? ctx->tune_low_bits : 2;
+    if (ctx->spin_pd == nullptr) {
+        // tile sites: the largest K with q^K <= 8192 whose passes fit LDS with their tables
+        int K = 0;
+        {
+            int64_t p = 1;
+            while (K < n && p * q <= SP_MAX_TILE) { p *= q; ++K; }
         }
+        if (ctx->tune_tile_bits > 0 && ctx->tune_tile_bits < K) K = ctx->tune_tile_bits;
+        std::vector<SpinPassDev> host;
+        std::vector<int> nb, tsz, nlo, nent;
+        for (; K >= 1; --K) {
+            const int L = std::min(L0, K);
+            auto cov = spin_cover(n, ctx->site_pair, ctx->ibond, K, L);
+            if (cov.empty()) return fail(PNED_E_UNSUPPORTED, "spin-S tile planner made no progress");
+            const int np = (int)cov.size() / 2;
+            host.assign(np, SpinPassDev{});
+            nb.assign(np, 0);
+            tsz.assign(np, 0);
+            nlo.assign(np, 0);
+            nent.assign(np, 0);
+            bool fits = true;
+            std::vector<uint64_t> pw(n);
+            {
+                uint64_t x = 1;
+                for (int k = 0; k < n; ++k) { pw[k] = x; x *= (uint64_t)q; }
+            }
+            for (int p = 0; p < np && fits; ++p) {
+                const std::vector<int> &sites = cov[2 * p], &bonds = cov[2 * p + 1];
+                SpinPassDev &d = host[p];
+                if ((int)bonds.size() > SP_MAX_BOND) return fail(PNED_E_UNSUPPORTED, "too many bonds in a spin-S pass");
+                const int Kp = (int)sites.size();
+                int ts = 1;
+                for (int k = 0; k < Kp; ++k) ts *= q;
+                for (int j = 0; j < ts; ++j) {
+                    uint64_t o = 0;
+                    int y = j;
+                    for (int k = 0; k < Kp; ++k) { o += (uint64_t)(y % q) * pw[sites[k]]; y /= q; }
+                    d.off[j] = o;
+                }
+                std::vector<int> outer;
+                for (int st = 0; st < n; ++st)
+                    if (std::find(sites.begin(), sites.end(), st) == sites.end()) outer.push_back(st);
+                const int no = (int)outer.size();
+                int nl = 0;
+                int64_t lm = 1;
+                while (nl < no && lm * q <= SP_BASE_TAB) { lm *= q; ++nl; }
+                int64_t hm = 1;
+                for (int k = nl; k < no; ++k) hm *= q;
+                if (hm > SP_BASE_TAB) return fail(PNED_E_UNSUPPORTED, "too many outer sites for the spin-S tables");
+                for (int64_t t2 = 0; t2 < lm; ++t2) {
+                    uint64_t o = 0;
+                    int64_t y = t2;
+                    for (int k = 0; k < nl; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
+                    d.blo[t2] = o;
+                }
+                for (int64_t t2 = 0; t2 < hm; ++t2) {
+                    uint64_t o = 0;
+                    int64_t y = t2;
+                    for (int k = nl; k < no; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
+                    d.bhi[t2] = o;
+                }
+                // (bond, output local state) rows of nonzero elements, as the reference's
+                // H_dot_v visits them (llocal_Ham skips), partners as tile offsets
+                int ne = 0;
+                for (size_t bi = 0; bi < bonds.size(); ++bi) {
+                    const int bond = bonds[bi];
+                    const int sa = ctx->site_pair[2 * bond], sb = ctx->site_pair[2 * bond + 1];
+                    const int pa = (int)(std::find(sites.begin(), sites.end(), sa) - sites.begin());
+                    const int pbp = (int)(std::find(sites.begin(), sites.end(), sb) - sites.begin());
+                    int wa = 1, wb = 1;
+                    for (int k = 0; k < pa; ++k) wa *= q;
+                    for (int k = 0; k < pbp; ++k) wb *= q;
+                    d.bond_pa[bi] = pa;
+                    d.bond_pb[bi] = pbp;
+                    const int t = ctx->btype[bond] - 1;
+                    for (int out = 0; out < qq; ++out) {
+                        d.row_start[bi * qq + out] = ne;
+                        const int d1 = out % q, d2 = out / q;
+                        for (int in = 0; in < qq; ++in) {
+                            const size_t f = (size_t)in + (size_t)qq * out + (size_t)qq * qq * t;
+                            if (ctx->ll[f]) continue;
+                            if (ne >= SP_MAX_ENT) return fail(PNED_E_UNSUPPORTED, "spin-S pass table too large");
+                            d.ent_delta[ne] = (in % q - d1) * wa + (in / q - d2) * wb;
+                            d.ent_c[2 * ne] = ctx->xl[2 * f];
+                            d.ent_c[2 * ne + 1] = ctx->xl[2 * f + 1];
+                            ++ne;
+                        }
+                    }
+                }
+                d.row_start[bonds.size() * qq] = ne;
+                nb[p] = (int)bonds.size();
+                tsz[p] = ts;
+                nlo[p] = (int)lm;
+                nent[p] = ne;
+                fits = spin_lds_bytes(ts, (int)bonds.size() * qq, ne, 1024) <= 160 * 1024;
+            }
+            if (fits) break;
+        }
+        if (K < 1) return fail(PNED_E_UNSUPPORTED, "spin-S tables do not fit LDS");
+        const int np = (int)host.size();
         PNED_HIP(hipStreamSynchronize(ctx->stream));
-        if (ctx->spin_pd) PNED_HIP(hipFree(ctx->spin_pd));
         PNED_HIP(hipMalloc(&ctx->spin_pd, sizeof(SpinPassDev) * np));
         PNED_HIP(hipMemcpy(ctx->spin_pd, host.data(), sizeof(SpinPassDev) * np, hipMemcpyHostToDevice));
         ctx->spin_K = K;
@@ -461,6 +563,7 @@ int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double
         ctx->spin_nb = nb;
         ctx->spin_tsz = tsz;
         ctx->spin_nlo = nlo;
+        ctx->spin_nent = nent;
     }
     const int np = ctx->spin_np;
     const int grid_cap = 2048;
@@ -474,6 +577,7 @@ int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double
         a.K = 0;
         for (int t = 1; t < a.tsz; t *= q) ++a.K;
         a.nbond = ctx->spin_nb[p];
+        a.nent = ctx->spin_nent[p];
         a.diag_first = (p == 0);
         a.fresh = fresh ? 1 : 0;
         a.nlo_mod = ctx->spin_nlo[p];
@@ -481,16 +585,21 @@ int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double
         const int grid = (int)std::min<uint64_t>(a.tile_count, (uint64_t)grid_cap);
         const SpinPassDev *pd = reinterpret_cast<const SpinPassDev *>(ctx->spin_pd) + p;
         double *part = ctx->d_partial + 2 * (size_t)rows;
-        // 256 threads x 32 states covers 8192; pick R from the tile size
-        const int R = (a.tsz + 255) / 256;
-#define PNED_SPIN_CASE(QV, RV)                                                                                      \
-        if (q == QV && R <= RV) {                                                                                  \
-            hipLaunchKernelGGL((hv_spin_pass<QV, 256, RV>), dim3(grid), dim3(256), 0, ctx->stream,                  \
-                               (const double2 *)v1->data, (double2 *)v0->data, tb, pd, a, part);                   \
+        // 1024 threads for tiles above 2048 states, else 256; R slots per thread (power of 2)
+        const int tpb = (a.tsz > 2048) ? 1024 : 256;
+        int R = 1;
+        while (R * tpb < a.tsz) R *= 2;
+        const size_t lds = spin_lds_bytes(a.tsz, a.nbond * qq, a.nent, tpb);
+#define PNED_SPIN_CASE(QV, TV, RV)                                                                                   \
+        if (q == QV && tpb == TV && R == RV) {                                                                      \
+            launch_spin_pass<QV, TV, RV>(ctx, grid, lds, v1, v0, pd, a, part);                                      \
         } else
-        PNED_SPIN_CASE(3, 8) PNED_SPIN_CASE(3, 16) PNED_SPIN_CASE(3, 32)
-        PNED_SPIN_CASE(4, 16) PNED_SPIN_CASE(4, 32)
-        PNED_SPIN_CASE(5, 32)
+        PNED_SPIN_CASE(3, 1024, 8) PNED_SPIN_CASE(3, 1024, 4) PNED_SPIN_CASE(3, 256, 8) PNED_SPIN_CASE(3, 256, 4)
+        PNED_SPIN_CASE(3, 256, 2) PNED_SPIN_CASE(3, 256, 1)
+        PNED_SPIN_CASE(4, 1024, 8) PNED_SPIN_CASE(4, 1024, 4) PNED_SPIN_CASE(4, 256, 8) PNED_SPIN_CASE(4, 256, 4)
+        PNED_SPIN_CASE(4, 256, 2) PNED_SPIN_CASE(4, 256, 1)
+        PNED_SPIN_CASE(5, 1024, 8) PNED_SPIN_CASE(5, 1024, 4) PNED_SPIN_CASE(5, 256, 8) PNED_SPIN_CASE(5, 256, 4)
+        PNED_SPIN_CASE(5, 256, 2) PNED_SPIN_CASE(5, 256, 1)
         { return fail(PNED_E_UNSUPPORTED, "no tiled spin-S kernel instance for this q / tile"); }
 #undef PNED_SPIN_CASE
         PNED_HIP(hipGetLastError());
