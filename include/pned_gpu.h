@@ -83,6 +83,13 @@ int pned_ctx_autotune(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, int reps,
                       int *best_low_bits, int *best_threads, double *best_ms);
 /* spin-S kernel choice: 1 = tiled passes (default, q = 3..5), 0 = per-state gather */
 int pned_ctx_spin_kernel(pned_ctx *ctx, int tiled);
+/* S^z sector basis (parity_type = 0, mk_list12: S_one_half/Hybrid_TPS_parallel.f90:452-578):
+ * restrict the context to the states with n_down set bits, indexed as the reference's
+ * list1/list3 tables do; n_down < 0 returns to the full 2^N space.  The context's dim
+ * becomes C(N, n_down): create vectors after this call.  S=1/2 only; fails with
+ * PNED_E_UNSUPPORTED when the operator does not conserve S^z.  H.v, BLAS-1 and the Krylov
+ * drivers work in the sector; S^z(q) and mTPQ observables need the full basis. */
+int pned_ctx_sz_sector(pned_ctx *ctx, int n_down);
 /* S=1/2 pass kernel choice (no reference counterpart): 1 = term-major (default; per-term
  * selector and partner address resolved once per tile), 0 = state-major */
 int pned_ctx_s12_kernel(pned_ctx *ctx, int kind);

@@ -70,6 +70,10 @@ def lib():
         L.orc_check1.argtypes = [i32, i32, i32, _ip, _ip, _zp, _bp, _zp, _zp, _zp, _zp, i32]
         L.orc_check1.restype = i32
         L.orc_fill_range.argtypes = [u64, i64, i64, C.c_void_p]
+        L.orc_mk_list12.argtypes = [i32, i32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_mk_list12.restype = i64
+        L.orc_hv_sector.argtypes = [i32, i32, _ip, _ip, _zp, _bp, f64, i32, i64, _up, _up, _up, _zp, _zp, _zp]
+        L.orc_hv_sector.restype = i32
         L.orc_num_threads.restype = i32
         _lib = L
     return _lib
@@ -96,6 +100,30 @@ def hv(oop: OracleOp, v1: np.ndarray, v0: np.ndarray, eperbd: float = 0.0) -> co
     """p_mltply_v3 (Hybrid_TPS_parallel.f90:190-263): v0 += (H - ibond*eperbd) v1; returns prdct."""
     pr = np.zeros(1, dtype=np.complex128)
     lib().orc_hv(*oop.args(), float(eperbd), v1, v0, pr)
+    return complex(pr[0])
+
+
+def sector_tables(n: int, tot_sz: int):
+    """mk_list12 (Hybrid_TPS_parallel.f90:452-578) at np=1: (list1, la, lb) for the states
+    with tot_sz set bits; index(s) = la[s & right] + lb[s >> ihf] - 1 (0-based)."""
+    ihf0, ihf = n // 2, (n + 1) // 2
+    ldim = lib().orc_mk_list12(n, tot_sz, None, None, None)
+    list1 = np.zeros(ldim, dtype=np.uint32)
+    la = np.zeros(1 << ihf, dtype=np.uint32)
+    lb = np.zeros(1 << ihf0, dtype=np.uint32)
+    lib().orc_mk_list12(n, tot_sz, list1.ctypes.data, la.ctypes.data, lb.ctypes.data)
+    return list1, la, lb
+
+
+def hv_sector(oop: OracleOp, tot_sz: int, v1: np.ndarray, v0: np.ndarray, eperbd: float = 0.0) -> complex:
+    """H_dot_v over the S^z sector basis (parity_type = 0): v0 += (H - ibond*eperbd) v1."""
+    assert oop.q == 2
+    list1, la, lb = sector_tables(oop.n, tot_sz)
+    pr = np.zeros(1, dtype=np.complex128)
+    rc = lib().orc_hv_sector(oop.n, oop.ibond, oop.site_pair, oop.btype, oop.xl, oop.mk, float(eperbd), int(tot_sz),
+                             len(list1), list1, la, lb, v1, v0, pr)
+    if rc != 0:
+        raise ValueError("the operator does not conserve S^z")
     return complex(pr[0])
 
 

@@ -78,6 +78,83 @@ void orc_hv(int n, int q, int ibond, const int *site_pair, const int *btype,
     orc_hv_rows(n, q, ibond, site_pair, btype, xl, mk, eperbd, v1, v0, prdct, 0, ipow64(q, n));
 }
 
+/* ------------------------------------------------------------------------------ */
+/* S^z sector basis (parity_type = 0): mk_list12, H/parallel.f90:452-578, at np=1   */
+/* (no rank bits: psz = 0) and dsf_comp != 'xx' (is = 0).  The b_bit state bits     */
+/* split into a right half of ihf = (b_bit+1)/2 bits and a left half of             */
+/* ihf0 = b_bit/2 bits; states with popcount == tot_sz are enumerated left-major,   */
+/* right-minor.  list1[i] = state of 0-based index i; la[ia] = 1-based rank of ia   */
+/* among right halves of its popcount (list3(1,ia)); lb[ib] = index offset of the   */
+/* left half ib's block (list3(2,ib)).  Index (1-based) of state s =                */
+/* la[s & right] + lb[s >> ihf].  Returns ldim; list1 may be NULL to count only.    */
+/* ------------------------------------------------------------------------------ */
+int64_t orc_mk_list12(int n, int tot_sz, uint32_t *list1, uint32_t *la, uint32_t *lb)
+{
+    const int ihf0 = n / 2, ihf = (n + 1) / 2;
+    int64_t icnt = 0, ja = 0, jb = 0;
+    int64_t ibpatn = 0;
+    for (int64_t i2 = 0; i2 < ((int64_t)1 << ihf0); ++i2) {
+        for (int64_t i1 = 0; i1 < ((int64_t)1 << ihf); ++i1) {
+            const int isz = __builtin_popcountll((unsigned long long)i1) + __builtin_popcountll((unsigned long long)i2);
+            if (isz != tot_sz) continue;
+            ++icnt;
+            if (i2 == ibpatn) {
+                ++ja;
+            } else {
+                ibpatn = i2;
+                ja = 1;
+                jb = icnt - 1;
+            }
+            if (list1) list1[icnt - 1] = (uint32_t)((i2 << ihf) + i1);
+            if (la) la[i1] = (uint32_t)ja;
+            if (lb) lb[i2] = (uint32_t)jb;
+        }
+    }
+    return icnt;
+}
+
+/* H_dot_v over the sector basis (H/parallel.f90:305-442 with list_s = list1 and    */
+/* list_lib = list3): per (bond, input local state) element, every output index l0  */
+/* decodes its state, forms the partner state and looks its index up in the two    */
+/* half tables.  S=1/2 only.  Returns -1 if an element not skipped by llocal_Ham    */
+/* leaves the sector (the operator does not conserve S^z), else 0.                  */
+int orc_hv_sector(int n, int ibond, const int *site_pair, const int *btype, const zc *xl, const unsigned char *mk,
+                  double eperbd, int tot_sz, int64_t ldim, const uint32_t *list1, const uint32_t *la,
+                  const uint32_t *lb, const zc *v1, zc *v0, zc *prdct)
+{
+    const int ihf = (n + 1) / 2;
+    const uint32_t right = (1u << ihf) - 1u;
+    double pr_re = 0.0, pr_im = 0.0;
+    int bad = 0;
+    for (int b = 0; b < ibond; ++b) {
+        const int s1 = site_pair[2 * b], s2 = site_pair[2 * b + 1];
+        const int t = btype[b] - 1;
+        for (int ml = 0; ml < 4; ++ml) {
+            const int l1 = ml % 2, l2 = ml / 2;
+            #pragma omp parallel for schedule(static) reduction(+:pr_re,pr_im) reduction(|:bad)
+            for (int64_t l0 = 0; l0 < ldim; ++l0) {
+                const uint32_t st = list1[l0];
+                const int d1 = (int)((st >> s1) & 1u), d2 = (int)((st >> s2) & 1u);
+                const int istat = 2 * d2 + d1;
+                if (mk[(size_t)t * 16 + (size_t)ml * 4 + istat]) continue;
+                const int64_t pst = (int64_t)st + (int64_t)(l1 - d1) * ((int64_t)1 << s1) +
+                                    (int64_t)(l2 - d2) * ((int64_t)1 << s2);
+                if (__builtin_popcountll((unsigned long long)pst) != tot_sz) { bad = 1; continue; }
+                const int64_t it = (int64_t)la[(uint32_t)pst & right] + (int64_t)lb[(uint32_t)pst >> ihf] - 1;
+                zc f = xl[(size_t)t * 16 + (size_t)ml * 4 + istat];
+                if (ml == istat) f -= eperbd;
+                const zc w = f * v1[it];
+                v0[l0] += w;
+                const zc p = conj(v1[l0]) * w;
+                pr_re += creal(p);
+                pr_im += cimag(p);
+            }
+        }
+    }
+    *prdct = pr_re + I * pr_im;
+    return bad ? -1 : 0;
+}
+
 /* calc_vec_inprdct  H/parallel.f90:104-126: sum conj(x) y */
 zc orc_dot(int64_t d, const zc *x, const zc *y)
 {

@@ -611,6 +611,8 @@ static int hv_dispatch(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double e
     if (v1->dim != v0->dim || v1->nvec != v0->nvec || v1->dtype != v0->dtype)
         return fail(PNED_E_ARG, "v1 and v0 differ in shape or type");
     if (v1 == v0) return fail(PNED_E_ARG, "v1 and v0 must be different vectors");
+    if (v1->dim != ctx->dim) return fail(PNED_E_ARG, "vector dimension differs from the context's basis");
+    if (ctx->sector_k >= 0) return launch_hv_sector(ctx, v1, v0, eperbd, fresh, prdct);
     if (ctx->q == 2) return launch_hv_s12(ctx, v1, v0, eperbd, fresh, prdct);
     return launch_hv_spin(ctx, v1, v0, eperbd, fresh, prdct);
 }
@@ -725,6 +727,7 @@ int pned_ctx_destroy(pned_ctx *ctx)
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
     if (ctx->pd_dev) (void)hipFree(ctx->pd_dev);
     if (ctx->d_tm) (void)hipFree(ctx->d_tm);
+    free_sector(ctx);
     if (ctx->d_colcoef) (void)hipFree(ctx->d_colcoef);
     if (ctx->d_coldiag) (void)hipFree(ctx->d_coldiag);
     if (ctx->d_partial) (void)hipFree(ctx->d_partial);
@@ -799,13 +802,14 @@ int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t
     PNED_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->q == 2) {
         const int rc = refit_plan(ctx, xlocal_ham, llocal_ham);
-        if (rc <= 0) return rc;   // refitted, or a HIP error
-    }
-    if (ctx->q == 2) {
-        PNED_TRY(s12_decompose(ctx->n, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), xlocal_ham, llocal_ham,
-                               ctx->s12));
-        ctx->plan_colbits = -1;
-        ctx->plan_dtype = -1;
+        if (rc < 0) return rc;   // a HIP error
+        if (rc > 0) {
+            PNED_TRY(s12_decompose(ctx->n, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), xlocal_ham, llocal_ham,
+                                   ctx->s12));
+            ctx->plan_colbits = -1;
+            ctx->plan_dtype = -1;
+        }
+        if (ctx->sector_k >= 0) PNED_TRY(refresh_sector_terms(ctx));
     } else {
         reset_spin_tables(ctx);
         free_spin_plan(ctx);   // the tiled passes carry the coefficients

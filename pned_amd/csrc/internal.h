@@ -156,6 +156,13 @@ struct pned_ctx {
     pned::TmTerm *d_tm = nullptr;    // per-pass tile-coordinate terms (term-major kernel)
     std::vector<int> tm_ncat;        // [pass][4] terms per term-major category
     int s12_kernel = 1;              // 1: term-major pass kernel, 0: state-major pass kernel
+    // S^z sector basis (parity_type = 0); sector_k < 0: full 2^N space
+    int sector_k = -1;
+    int sector_nterm = 0;
+    uint32_t *d_list1 = nullptr;     // index -> state
+    uint32_t *d_srank = nullptr;     // rank of a right half among its popcount class
+    uint32_t *d_soffset = nullptr;   // first index of a left half's block
+    void *d_sterms = nullptr;        // two-bit flip terms
     double *d_colcoef = nullptr;     // batch contexts: per-column term coefficients
     double *d_coldiag = nullptr;     // batch contexts: per-column diagonal weights
     int pd_threads = -1;
@@ -185,6 +192,11 @@ int ensure_partial(pned_ctx *ctx, size_t ndoubles);
 int ensure_result(pned_ctx *ctx, size_t ndoubles);
 int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
+// S^z sector basis: H.v and table release
+int launch_hv_sector(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                     double *prdct_host);
+void free_sector(pned_ctx *ctx);
+int refresh_sector_terms(pned_ctx *ctx);   // after the bond matrices change
 // untuned tile defaults: true for vectors above 2^25 elements (dim x columns)
 bool default_large(const pned_ctx *ctx, int colbits);
 int upload_batch_coefficients(pned_ctx *ctx);

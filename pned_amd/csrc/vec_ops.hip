@@ -821,6 +821,7 @@ int gaussian_fill(pned_ctx *ctx, pned_vec *v, uint64_t seed)
 
 int sz_q_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, const double *site_vec)
 {
+    if (ctx->sector_k >= 0) return fail(PNED_E_UNSUPPORTED, "observables need the full 2^N basis (sector basis active)");
     PNED_TRY(same_shape(x, v));
     if (x->nvec != 1) return fail(PNED_E_UNSUPPORTED, "S^z(q) excitation takes one column");
     if (ctx->n > 32) return fail(PNED_E_UNSUPPORTED, "at most 32 sites");
@@ -884,6 +885,7 @@ static int build_obs_args(pned_ctx *ctx, ObsArgs &a)
 // out layout per column: [vg.v1 (2)] [v1 Wzz v1 (2)] [vg Wzz v1 (2)] [sq entries (2 each)]
 int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, double *out)
 {
+    if (ctx->sector_k >= 0) return fail(PNED_E_UNSUPPORTED, "observables need the full 2^N basis (sector basis active)");
     PNED_TRY(same_shape(vg, v1));
     PNED_TRY(check_pow2_cols(v1->nvec));
     ObsArgs a;
@@ -923,6 +925,7 @@ int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, doubl
 // out per column: the 10 + 2*nsq values of k_tps_fused
 int tps_fused_reduce(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, const pned_vec *w, double *out)
 {
+    if (ctx->sector_k >= 0) return fail(PNED_E_UNSUPPORTED, "observables need the full 2^N basis (sector basis active)");
     PNED_TRY(same_shape(up, u));
     PNED_TRY(same_shape(u, w));
     PNED_TRY(check_pow2_cols(u->nvec));

@@ -53,6 +53,7 @@ def lib():
     L.pned_ctx_plan_info.argtypes = [P, C.POINTER(i32), _ip, i32]
     L.pned_ctx_spin_kernel.argtypes = [P, i32]
     L.pned_ctx_s12_kernel.argtypes = [P, i32]
+    L.pned_ctx_sz_sector.argtypes = [P, i32]
     L.pned_ctx_autotune.argtypes = [P, P, P, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                     C.POINTER(f64)]
     L.pned_ctx_sync.argtypes = [P]
@@ -219,6 +220,12 @@ class Context:
 
     def spin_kernel(self, tiled: bool):
         check(lib().pned_ctx_spin_kernel(self.h, 1 if tiled else 0))
+
+    def sz_sector(self, n_down: int):
+        """Restrict to the S^z sector with n_down set bits (n_down < 0: full space);
+        vectors must be created afterwards (the dimension changes)."""
+        check(lib().pned_ctx_sz_sector(self.h, n_down))
+        self.dim = int(lib().pned_ctx_dim(self.h))
 
     def s12_kernel(self, kind: int):
         """S=1/2 pass kernel: 1 term-major (default), 0 state-major."""

@@ -366,3 +366,57 @@ def test_set_matrices_refits_plan():
         assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), (phi, h)
         assert abs(pr - pr_ref) <= 1e-13 * np.linalg.norm(x) * np.linalg.norm(want)
         assert ctx.plan_info() == plan_a, "plan rebuilt although the operator fits it"
+
+
+@pytest.mark.parametrize("spec,k", [(12, 6), (16, 8), (16, 5), ("2x3", 5), (18, 9)])
+def test_sector_hv_parity(spec, k):
+    """S^z sector basis (parity_type = 0): H.v and <v|H|v> against the oracle's
+    list-table H_dot_v on the same sector vectors."""
+    _say("sector hv", spec, k)
+    cl = lattice.get_cluster(spec)
+    op = model.build_operator(cl, 20, 0.0)
+    ctx = gpu.Context(op)
+    ctx.sz_sector(k)
+    list1, _, _ = oracle.sector_tables(cl.n, k)
+    assert ctx.dim == len(list1)
+    x = _rand(ctx.dim, 31)
+    y = _rand(ctx.dim, 32)
+    v1, v0 = ctx.vector(), ctx.vector()
+    v1.upload(x)
+    v0.upload(y)
+    pr = ctx.p_mltply_v3(v1, v0, 0.05)
+    want = y.copy()
+    pr_ref = oracle.hv_sector(oracle.OracleOp(op), k, x, want, eperbd=0.05)
+    assert np.linalg.norm(v0.download() - want) <= 1e-12 * np.linalg.norm(want)
+    assert abs(pr - pr_ref) <= 1e-13 * np.linalg.norm(x) * np.linalg.norm(want - y)
+
+
+def test_sector_lanczos_ground_state():
+    """The Heisenberg ground state of the 16-site cluster lies in the S^z = 0 sector:
+    Lanczos in that sector reaches the oracle's full-space ground energy."""
+    _say("sector lanczos")
+    cl = lattice.reference_cluster(16)
+    op = model.build_operator(cl, 20, 0.0)
+    ctx = gpu.Context(op)
+    ctx.sz_sector(8)
+    v1, v0 = ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 7)
+    itr, alpha, beta, ene = ctx.lnc1(v1, v0, max_itr=1024, nvec=2)
+    x = oracle.gaussian(2 ** cl.n, 7)
+    _, _, _, e_full = oracle.lanczos(oracle.OracleOp(op), x, max_itr=1024, nvec=2)
+    assert abs(ene[0] - e_full[0]) <= 1e-10 * abs(e_full[0])
+
+
+def test_sector_rejects_non_conserving_operator():
+    _say("sector reject")
+    op = model.build_operator(lattice.reference_cluster(12), 20, 54.0)
+    ctx = gpu.Context(op)
+    with pytest.raises(gpu.PnedError):
+        ctx.sz_sector(6)
+    # back to the full space: vectors of the other basis are refused
+    op0 = model.build_operator(lattice.reference_cluster(12), 20, 0.0)
+    ctx0 = gpu.Context(op0)
+    a, b = ctx0.vector(), ctx0.vector()
+    ctx0.sz_sector(6)
+    with pytest.raises(gpu.PnedError):
+        ctx0.p_mltply_v3(a, b)

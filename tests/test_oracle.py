@@ -307,3 +307,50 @@ def test_sz_q_and_cf_closed_forms():
     nd = oracle.normalize_dsf(dsf, 2.5e-3, 0.3)
     integral = np.sum(0.5 * 2.5e-3 * (nd[1:] + nd[:-1]))
     assert abs(integral - 0.3) < 1e-12
+
+
+@pytest.mark.parametrize("n,k", [(8, 4), (8, 3), (9, 4), (12, 6), (12, 0), (12, 12), (13, 5)])
+def test_sector_tables_enumerate_and_index(n, k):
+    """mk_list12: list1 is every state with k set bits in increasing order (left half
+    major, right half minor is numeric order), and la/lb index each of them back."""
+    list1, la, lb = oracle.sector_tables(n, k)
+    want = np.array([s for s in range(1 << n) if bin(s).count("1") == k], dtype=np.uint32)
+    assert np.array_equal(list1, want)
+    ihf = (n + 1) // 2
+    idx = la[list1 & ((1 << ihf) - 1)].astype(np.int64) + lb[list1 >> ihf].astype(np.int64) - 1
+    assert np.array_equal(idx, np.arange(len(list1)))
+
+
+@pytest.mark.parametrize("spec,k", [(8, 4), (8, 3), (12, 6), ("2x3", 5)])
+def test_sector_hv_is_the_projected_dense_operator(spec, k):
+    """The sector H.v equals the dense Heisenberg H restricted to the k-down sector
+    (phi = 0: J S.S), with the eperbd shift on every bond diagonal."""
+    cl = lattice.get_cluster(spec)
+    op = model.build_operator(cl, 20, 0.0)
+    J, K = dense.model20_couplings(0.0)
+    H = dense.kh_dense(cl, J, K, K, K)
+    H = H.toarray() if hasattr(H, "toarray") else np.asarray(H)
+    list1, _, _ = oracle.sector_tables(cl.n, k)
+    Hs = H[np.ix_(list1.astype(np.int64), list1.astype(np.int64))]
+    rng = np.random.default_rng(k)
+    x = rng.standard_normal(len(list1)) + 1j * rng.standard_normal(len(list1))
+    y0 = rng.standard_normal(len(list1)) + 0j
+    y = y0.copy()
+    ep = 0.04
+    pr = oracle.hv_sector(oracle.OracleOp(op), k, x, y, eperbd=ep)
+    want = y0 + Hs @ x - cl.ibond * ep * x
+    assert np.linalg.norm(y - want) <= 1e-12 * np.linalg.norm(want)
+    assert abs(pr - np.vdot(x, Hs @ x - cl.ibond * ep * x)) <= 1e-12 * np.linalg.norm(x) ** 2 * np.linalg.norm(Hs, 2)
+    # the sector of the complement count is the spin-flipped image with the same spectrum
+    w_sec = np.linalg.eigvalsh(Hs)
+    w_full = np.linalg.eigvalsh(H)
+    assert np.all(np.min(np.abs(w_full[None, :] - w_sec[:, None]), axis=1) < 1e-9)
+
+
+def test_sector_hv_rejects_non_conserving_operator():
+    cl = lattice.reference_cluster(8)
+    op = model.build_operator(cl, 20, 54.0)   # Kitaev terms flip two aligned spins
+    list1, _, _ = oracle.sector_tables(cl.n, 4)
+    x = np.ones(len(list1), complex)
+    with pytest.raises(ValueError):
+        oracle.hv_sector(oracle.OracleOp(op), 4, x, np.zeros_like(x))
