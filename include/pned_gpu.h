@@ -248,6 +248,16 @@ int pned_renbun(pned_ctx *ctx, int i_range, int s_maxitr, double split, double e
                 const double *salpha, const double *sbeta, double sq, double *w, double *dsf,
                 int *conv);
 
+/* ---- post-processing (host FP64) ---- */
+/* cTPQ evaluation of mTPQ runs: the reference's evaluation program
+ * (S_one_half/Hybrid_TPS_eval_mpi.f90:236-446, one process).  Inputs in the reference's
+ * Fortran order: beta(i,seed) at beta[i + num_data1*seed] (alpha likewise, unused),
+ * coef(i,m,seed) at coef[i + num_data1*(m + num_data2*seed)] (0-based; columns 4 and 5
+ * are read).  Temperatures log-spaced on [t_min, t_max] with num_div intervals;
+ * out[4*(num_div+1)] = per temperature {T, C = dE/dT, variance term, dC/dT}. */
+int pned_ctpq_eval(int num_data1, int num_data2, int seed_num, const double *alpha, const double *beta,
+                   const double *coef, double t_min, double t_max, int num_div, double *out);
+
 #ifdef __cplusplus
 }
 #endif

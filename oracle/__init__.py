@@ -74,6 +74,9 @@ def lib():
         L.orc_mk_list12.restype = i64
         L.orc_hv_sector.argtypes = [i32, i32, _ip, _ip, _zp, _bp, f64, i32, i64, _up, _up, _up, _zp, _zp, _zp]
         L.orc_hv_sector.restype = i32
+        L.orc_ctpq_sample.argtypes = [i32, f64, _dp, _dp, _dp]
+        L.orc_ctpq_sample.restype = f64
+        L.orc_ctpq.argtypes = [i32, i32, i32, _dp, _dp, _dp, f64, f64, i32, _dp]
         L.orc_num_threads.restype = i32
         _lib = L
     return _lib
@@ -125,6 +128,28 @@ def hv_sector(oop: OracleOp, tot_sz: int, v1: np.ndarray, v0: np.ndarray, eperbd
     if rc != 0:
         raise ValueError("the operator does not conserve S^z")
     return complex(pr[0])
+
+
+def ctpq_sample(beta, c4, c5, bt: float) -> float:
+    """calc_fac + eval_q (Hybrid_TPS_eval_mpi.f90:236-303) for one mTPQ sample."""
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    return lib().orc_ctpq_sample(len(b), float(bt), b, np.ascontiguousarray(c4, dtype=np.float64),
+                                 np.ascontiguousarray(c5, dtype=np.float64))
+
+
+def ctpq(alpha, beta, coef, t_min: float, t_max: float, num_div: int) -> np.ndarray:
+    """The cTPQ program (Hybrid_TPS_eval_mpi.f90:383-446).  alpha, beta: (seeds, nd1);
+    coef: (seeds, nd2, nd1), i.e. the reference's Fortran arrays.  Returns
+    (num_div+1, 4): T, C, variance, dC/dT."""
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.ascontiguousarray(beta, dtype=np.float64)
+    c = np.ascontiguousarray(coef, dtype=np.float64)
+    seeds, nd1 = b.shape
+    nd2 = c.shape[1]
+    out = np.zeros(4 * (num_div + 1))
+    lib().orc_ctpq(nd1, nd2, seeds, a.reshape(-1), b.reshape(-1), c.reshape(-1), float(t_min), float(t_max),
+                   int(num_div), out)
+    return out.reshape(num_div + 1, 4)
 
 
 def hv_rows(oop: OracleOp, v1, v0, lo: int, hi: int, eperbd: float = 0.0) -> complex:

@@ -155,6 +155,102 @@ int orc_hv_sector(int n, int ibond, const int *site_pair, const int *btype, cons
     return bad ? -1 : 0;
 }
 
+/* ------------------------------------------------------------------------------ */
+/* cTPQ post-processing, H/eval_mpi.f90:236-446 (program main + calc_fac, eval_q,   */
+/* lsort, calc_ave).  Arrays in the reference's Fortran order: alpha(i,seed) at      */
+/* i + nd1*seed, beta likewise, coef(i,m,seed) at i + nd1*(m + nd2*seed), 0-based.  */
+/* ------------------------------------------------------------------------------ */
+static void ctpq_calc_fac(int nd1, double bt, const double *beta_s, double *fac1, double *fac2)
+{
+    double y = 0.0, maxfac = -1e200;
+    for (int i = 0; i < nd1; ++i) { fac1[i] = 0.0; fac2[i] = 0.0; }
+    for (int i = 1; i <= 2 * (nd1 - 1); ++i) {
+        const int m = ((i + 1) >> 1) + 1;                 /* 1-based */
+        const double z = log(bt / (double)(2 * i) * beta_s[m - 1]);
+        y += z;
+        fac1[m - 1] = y;
+        if (y > maxfac) maxfac = y;
+    }
+    for (int i = 1; i <= nd1 - 1; ++i) {
+        fac1[i - 1] = exp(fac1[i - 1] - maxfac);
+        fac2[i - 1] = fac1[i - 1] * bt * beta_s[i] / (double)(2 * i - 1);
+    }
+    fac1[nd1 - 1] = exp(fac1[nd1 - 1] - maxfac);
+    fac2[nd1 - 1] = 0.0;
+}
+
+/* lsort: the reference's exchange sort, largest first */
+static void ctpq_lsort_desc(int n, double *v)
+{
+    for (int j = 0; j < n - 1; ++j)
+        for (int i = j; i < n; ++i)
+            if (v[i] > v[j]) { const double t = v[i]; v[i] = v[j]; v[j] = t; }
+}
+
+double orc_ctpq_sample(int nd1, double bt, const double *beta_s, const double *c4, const double *c5)
+{
+    double *fac1 = malloc(sizeof(double) * nd1), *fac2 = malloc(sizeof(double) * nd1);
+    double *t1 = malloc(sizeof(double) * nd1), *t2 = malloc(sizeof(double) * nd1);
+    ctpq_calc_fac(nd1, bt, beta_s, fac1, fac2);
+    double z = 0.0;
+    for (int i = 0; i < nd1 - 1; ++i) {
+        t1[i] = fac1[i] * c4[i];
+        t2[i] = fac2[i] * c5[i];
+        z += fac1[i] + fac2[i];
+    }
+    ctpq_lsort_desc(nd1 - 1, t1);
+    ctpq_lsort_desc(nd1 - 1, t2);
+    double x = 0.0, y = 0.0;
+    for (int i = 0; i < nd1 - 1; ++i) x += t1[i];
+    for (int i = 0; i < nd1 - 1; ++i) y += t2[i];
+    free(fac1); free(fac2); free(t1); free(t2);
+    return (x + y) / z;
+}
+
+/* out[4*(num_div+1)]: per temperature T, C = dE/dT, Var(C) proxy, dC/dT (the program's columns) */
+void orc_ctpq(int nd1, int nd2, int seed_num, const double *alpha, const double *beta, const double *coef,
+              double t_min, double t_max, int num_div, double *out)
+{
+    (void)alpha;
+    const double log_t_max = log(t_max), log_t_min = log(t_min);
+    const double delta = (log_t_max - log_t_min) / (double)num_div;
+    double t_delta = 1e-4;
+    if (t_min < t_delta) t_delta = t_min * 0.9;
+    double *tmp = malloc(sizeof(double) * seed_num);
+    for (int i1 = 0; i1 <= num_div; ++i1) {
+        const double t0 = exp(log_t_min + (double)i1 * delta);
+        double dat[9];
+        for (int i2 = 0; i2 <= 2; ++i2) {
+            double bt = 1.0 / t0;
+            if (i2 == 0) bt = 1.0 / (t0 - t_delta);
+            if (i2 == 2) bt = 1.0 / (t0 + t_delta);
+            for (int sd = 0; sd < seed_num; ++sd) {
+                const double *bs = beta + (size_t)nd1 * sd;
+                const double *c4 = coef + (size_t)nd1 * (3 + (size_t)nd2 * sd);
+                const double *c5 = coef + (size_t)nd1 * (4 + (size_t)nd2 * sd);
+                tmp[sd] = orc_ctpq_sample(nd1, bt, bs, c4, c5);
+            }
+            double ave = 0.0, var = 0.0;
+            for (int sd = 0; sd < seed_num; ++sd) { ave += tmp[sd]; var += tmp[sd] * tmp[sd]; }
+            const double r1 = ave / (double)seed_num;
+            dat[i2 * 3 + 0] = bt;
+            dat[i2 * 3 + 1] = r1;
+            dat[i2 * 3 + 2] = var / (double)seed_num - r1 * r1;
+        }
+        const double d_e = 0.5 * (dat[7] - dat[1]);
+        const double spec = d_e / t_delta;
+        const double var_spec = 0.5 * (dat[2] + dat[8]);
+        const double spec_d1 = (dat[4] - dat[1]) / t_delta;
+        const double spec_d2 = (dat[7] - dat[4]) / t_delta;
+        const double del_c = (spec_d2 - spec_d1) / t_delta;
+        out[4 * i1 + 0] = 1.0 / dat[3];
+        out[4 * i1 + 1] = spec;
+        out[4 * i1 + 2] = var_spec;
+        out[4 * i1 + 3] = del_c;
+    }
+    free(tmp);
+}
+
 /* calc_vec_inprdct  H/parallel.f90:104-126: sum conj(x) y */
 zc orc_dot(int64_t d, const zc *x, const zc *y)
 {

@@ -54,6 +54,7 @@ def lib():
     L.pned_ctx_spin_kernel.argtypes = [P, i32]
     L.pned_ctx_s12_kernel.argtypes = [P, i32]
     L.pned_ctx_sz_sector.argtypes = [P, i32]
+    L.pned_ctpq_eval.argtypes = [i32, i32, i32, _dp, _dp, _dp, C.c_double, C.c_double, i32, _dp]
     L.pned_ctx_autotune.argtypes = [P, P, P, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                     C.POINTER(f64)]
     L.pned_ctx_sync.argtypes = [P]
@@ -420,6 +421,29 @@ def s12_plan(op, colbits=0, tile_bits=12, low_bits=4):
     return {"passes": passes, "tmask": tt[:, 0].astype(np.int64), "selmask": tt[:, 1].astype(np.int64),
             "c0": tt[:, 2] + 1j * tt[:, 3], "c1": tt[:, 4] + 1j * tt[:, 5], "gmask": gmask[:T].astype(np.int64),
             "diag": diag[: 5 * G].reshape(G, 5), "c0diag": diag[5 * G] + 1j * diag[5 * G + 1]}
+
+
+def tps_arrays(alpha, beta, coef, nvec, ncoef):
+    """Reshape `Context.tps` / `tps_fused` outputs (step-major, columns interleaved) into
+    the reference's per-sample arrays: alpha, beta (samples, steps), coef (samples,
+    ncoef, steps) -- what SalphaSbeta files hold per seed."""
+    a = np.asarray(alpha, dtype=float).reshape(-1, nvec).T.copy()
+    b = np.asarray(beta, dtype=float).reshape(-1, nvec).T.copy()
+    c = np.asarray(coef, dtype=float).reshape(-1, nvec, ncoef).transpose(1, 2, 0).copy()
+    return a, b, c
+
+
+def ctpq(alpha, beta, coef, t_min, t_max, num_div):
+    """cTPQ evaluation (pned_ctpq_eval): alpha, beta (samples, steps), coef (samples,
+    ncoef, steps).  Returns (num_div+1, 4): T, C, variance term, dC/dT."""
+    a = np.ascontiguousarray(alpha, dtype=float)
+    b = np.ascontiguousarray(beta, dtype=float)
+    c = np.ascontiguousarray(coef, dtype=float)
+    seeds, nd1 = b.shape
+    out = np.zeros(4 * (num_div + 1))
+    check(lib().pned_ctpq_eval(nd1, c.shape[1], seeds, a.reshape(-1), b.reshape(-1), c.reshape(-1),
+                               float(t_min), float(t_max), int(num_div), out))
+    return out.reshape(num_div + 1, 4)
 
 
 def bisec(alpha, beta, ne, eps=1e-25):

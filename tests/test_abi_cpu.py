@@ -48,3 +48,25 @@ def test_host_tridiagonal_solvers_match_oracle():
     assert np.array_equal(E, oracle.bisec(a, b, 8))
     V = gpu.vec12(a, b, E, 3)
     assert np.allclose(V, oracle.vec12(a, b, E, 3), atol=1e-14)
+
+
+def test_ctpq_product_matches_oracle():
+    """pned_ctpq_eval (host code of the library) against the oracle's restatement on the
+    oracle's own mTPQ output for 3 samples of the 8-site cluster."""
+    import oracle
+    from pned_amd import lattice, model
+    cl = lattice.reference_cluster(8)
+    op = model.build_operator(cl, 20, 90.0, tps=True)
+    steps, seeds = 120, 3
+    A, B, Cf = [], [], []
+    for sd in range(seeds):
+        a, b, c = oracle.tps(oracle.OracleOp(op), cl.site_vec, np.zeros((0, 2)), op.shift_L, steps,
+                             oracle.gaussian(2 ** cl.n, 11 + sd))
+        A.append(a)
+        B.append(b)
+        Cf.append(c.T)
+    A, B, Cf = np.array(A), np.array(B), np.array(Cf)
+    got = gpu.ctpq(A, B, Cf, 0.05, 5.0, 40)
+    want = oracle.ctpq(A, B, Cf, 0.05, 5.0, 40)
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-12)
+    assert np.all(np.diff(got[:, 0]) > 0)

@@ -420,3 +420,29 @@ def test_sector_rejects_non_conserving_operator():
     ctx0.sz_sector(6)
     with pytest.raises(gpu.PnedError):
         ctx0.p_mltply_v3(a, b)
+
+
+def test_ctpq_from_gpu_mtpq():
+    """cTPQ post-processing of a batched mTPQ run on the GPU (4 samples, fused complex128)
+    against the oracle's mTPQ + cTPQ restatements of the same samples."""
+    _say("ctpq from gpu mtpq")
+    cl = lattice.reference_cluster(12)
+    op = model.build_operator(cl, 20, 90.0, tps=True)
+    ctx = gpu.Context(op)
+    ctx.tps_set_observables(cl.site_vec, cl.sq_momenta())
+    nvec, steps = 4, 150
+    v1, va, vb = (ctx.vector(nvec, gpu.C128) for _ in range(3))
+    ctx.set_initial_vec(v1, 41)
+    X = v1.download()
+    al, be, co = ctx.tps_fused(v1, va, vb, op.shift_L, steps)
+    A, B, Cf = gpu.tps_arrays(al, be, co, nvec, co.shape[-1])
+    got = gpu.ctpq(A, B, Cf, 0.1, 10.0, 30)
+    RA, RB, RC = [], [], []
+    for c in range(nvec):
+        a_ref, b_ref, c_ref = oracle.tps(oracle.OracleOp(op), cl.site_vec, cl.sq_momenta(), op.shift_L, steps,
+                                         np.ascontiguousarray(X[:, c]))
+        RA.append(a_ref)
+        RB.append(b_ref)
+        RC.append(c_ref.T)
+    want = oracle.ctpq(np.array(RA), np.array(RB), np.array(RC), 0.1, 10.0, 30)
+    assert np.allclose(got[:, :2], want[:, :2], rtol=1e-7, atol=1e-9)

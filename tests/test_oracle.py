@@ -354,3 +354,33 @@ def test_sector_hv_rejects_non_conserving_operator():
     x = np.ones(len(list1), complex)
     with pytest.raises(ValueError):
         oracle.hv_sector(oracle.OracleOp(op), 4, x, np.zeros_like(x))
+
+
+def test_ctpq_three_step_expansion():
+    """calc_fac/eval_q written out by hand for three mTPQ steps: the even weights are
+    1 and (bt^2/8) beta_2^2, the odd ones bt beta_2 and (bt^2/8) beta_2^2 bt beta_3 / 3;
+    beta_1 and the last step's columns do not enter."""
+    rng = np.random.default_rng(2)
+    beta = rng.random(3) + 0.5
+    c4 = rng.standard_normal(3)
+    c5 = rng.standard_normal(3)
+    for bt in (0.3, 1.7, 6.0):
+        w2 = bt * bt / 8.0 * beta[1] ** 2
+        u1 = bt * beta[1]
+        u2 = w2 * bt * beta[2] / 3.0
+        want = (c4[0] + w2 * c4[1] + u1 * c5[0] + u2 * c5[1]) / (1.0 + w2 + u1 + u2)
+        assert abs(oracle.ctpq_sample(beta, c4, c5, bt) - want) <= 1e-14 * max(1.0, abs(want))
+
+
+def test_ctpq_constant_norm_series():
+    """With every norm equal to b the even weights sum to cosh x and the odd ones to
+    2 sinh x (x = bt b / 2); constant columns come back unchanged."""
+    nd1, b = 400, 3.0
+    beta = np.full(nd1, b)
+    for bt in (0.5, 2.0, 5.0):
+        x = bt * b / 2.0
+        c4 = np.full(nd1, 1.7)
+        assert abs(oracle.ctpq_sample(beta, c4, c4, bt) - 1.7) < 1e-13
+        got = oracle.ctpq_sample(beta, c4, np.zeros(nd1), bt)
+        want = 1.7 * np.cosh(x) / (np.cosh(x) + 2.0 * np.sinh(x))
+        assert abs(got - want) < 1e-12
