@@ -258,6 +258,34 @@ int pned_renbun(pned_ctx *ctx, int i_range, int s_maxitr, double split, double e
 int pned_ctpq_eval(int num_data1, int num_data2, int seed_num, const double *alpha, const double *beta,
                    const double *coef, double t_min, double t_max, int num_div, double *out);
 
+/* ---- result and restart files (Fortran unformatted sequential records) ---- */
+/* Records carry 4-byte length markers; records above the subrecord limit (default
+ * 2^31-9 bytes) are split into subrecords with negated markers as gfortran documents.
+ * Diagnostic (no reference counterpart): set the limit (0 = default) to exercise the
+ * split at small sizes. */
+int pned_io_subrecord_limit(uint64_t bytes);
+/* dump_alpha_beta_coef / read_alpha_beta_coef (S_one_half/Hybrid_TPS_TPS.f90:507-583):
+ * num_data1, num_data2, alpha(num_data1), beta(num_data1), coef(num_data1,num_data2) in
+ * Fortran order.  read: with alpha = beta = coef = NULL returns the sizes only. */
+int pned_dump_alpha_beta_coef(const char *path, int num_data1, int num_data2, const double *alpha,
+                              const double *beta, const double *coef);
+int pned_read_alpha_beta_coef(const char *path, int *num_data1, int *num_data2, double *alpha, double *beta,
+                              double *coef, int max_data1, int max_data2);
+/* DSF continued-fraction coefficients (S_one_half/Hybrid_TPS_DSF.f90:353-443 dump_alpha_beta
+ * / read_alpha_beta2): sq, salpha(n_alpha), sbeta(n_beta) */
+int pned_dump_dsf_alpha_beta(const char *path, double sq, int n_alpha, const double *salpha, int n_beta,
+                             const double *sbeta);
+int pned_read_dsf_alpha_beta(const char *path, double *sq, int n_alpha, double *salpha, int n_beta,
+                             double *sbeta);
+/* vector files (vec_q*, vec_me*: Hybrid_TPS_DSF.f90:374-379, Hybrid_TPS_TPS.f90:530-537):
+ * an integer record, then one record per complex128 vector, streamed from / to device
+ * memory through a pinned staging buffer */
+int pned_dump_vectors(const char *path, int indx, int nvec, const pned_vec *const *vecs);
+int pned_read_vectors(const char *path, int *indx, int nvec, pned_vec *const *vecs);
+/* readwrite_vec_data (S_one_half/Hybrid_TPS_output.f90:12-36): e, then the vector */
+int pned_basis_write(const char *path, double e, const pned_vec *x);
+int pned_basis_read(const char *path, double *e, pned_vec *x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,301 @@
+// io.cpp -- the reference's result and restart files (Fortran unformatted sequential).
+//
+// Every file the reference writes with form='unformatted', access='sequential' is a list
+// of records; a record is its bytes between two 4-byte length markers.  Records longer
+// than a subrecord limit (2^31 - 9 bytes, the largest a 4-byte marker describes) are
+// split into subrecords as gfortran documents: the leading marker of a subrecord that is
+// followed by another one is negated, the trailing marker of a subrecord that follows
+// another one is negated.  The reader also accepts one record of up to 4 GiB with
+// unsigned markers (what amdflang writes).
+//
+// Files (S_one_half/Hybrid_TPS_*.f90):
+//   SalphaSbeta_me*seed*<comp>  TPS.f90:507-538   num_data1, num_data2, alpha, beta, coef
+//   SalphaSbeta_q*me*<comp>     DSF.f90:353-372   sq, salpha, sbeta
+//   vec_q*me*<comp>, vec_me*    DSF.f90:374-379   indx, v1, v2 [, v3]
+//   basisme*                    output.f90:12-36  e, x
+// Device vectors are streamed through a pinned host buffer, never held whole on the host.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace pned {
+namespace {
+
+size_t g_sub_limit = 2147483639;   // 2^31 - 9
+
+struct File {
+    FILE *f = nullptr;
+    ~File()
+    {
+        if (f) fclose(f);
+    }
+};
+
+int put_marker(FILE *f, int32_t m)
+{
+    return fwrite(&m, 4, 1, f) == 1 ? PNED_OK : fail(PNED_E_ARG, "write failed");
+}
+
+// writes one record of `total` bytes whose data come from fill(buf, offset, n)
+template <typename Fill>
+int write_record(FILE *f, size_t total, Fill fill)
+{
+    const size_t lim = g_sub_limit;
+    size_t done = 0;
+    std::vector<unsigned char> buf;
+    const size_t nsub = total == 0 ? 1 : (total + lim - 1) / lim;
+    for (size_t k = 0; k < nsub; ++k) {
+        const size_t len = std::min(lim, total - done);
+        const int32_t m = (int32_t)len;
+        PNED_TRY(put_marker(f, k + 1 < nsub ? -m : m));
+        PNED_TRY(fill(f, done, len));
+        PNED_TRY(put_marker(f, k > 0 ? -m : m));
+        done += len;
+    }
+    return PNED_OK;
+}
+
+int write_host(FILE *f, const void *p, size_t bytes)
+{
+    const unsigned char *c = (const unsigned char *)p;
+    return write_record(f, bytes, [&](FILE *ff, size_t off, size_t n) -> int {
+        return fwrite(c + off, 1, n, ff) == n ? PNED_OK : fail(PNED_E_ARG, "write failed");
+    });
+}
+
+// reads one record of exactly `bytes`, handing the data to take(offset, ptr, n)
+template <typename Take>
+int read_record(FILE *f, size_t bytes, Take take)
+{
+    size_t got = 0;
+    std::vector<unsigned char> buf(1 << 20);
+    bool first = true;
+    for (;;) {
+        int32_t lead;
+        if (fread(&lead, 4, 1, f) != 1) return fail(PNED_E_ARG, "unexpected end of file");
+        size_t len;
+        bool more;
+        if (lead >= 0) {
+            len = (size_t)lead;
+            more = false;
+        } else if (first && (uint32_t)lead == bytes) {   // one record with unsigned markers
+            len = (size_t)(uint32_t)lead;
+            more = false;
+        } else {
+            len = (size_t)(-(int64_t)lead);
+            more = true;
+        }
+        if (got + len > bytes) return fail(PNED_E_ARG, "record longer than expected");
+        size_t left = len;
+        while (left > 0) {
+            const size_t n = std::min(left, buf.size());
+            if (fread(buf.data(), 1, n, f) != n) return fail(PNED_E_ARG, "unexpected end of file");
+            PNED_TRY(take(got, buf.data(), n));
+            got += n;
+            left -= n;
+        }
+        int32_t trail;
+        if (fread(&trail, 4, 1, f) != 1) return fail(PNED_E_ARG, "unexpected end of file");
+        first = false;
+        if (!more) break;
+    }
+    if (got != bytes) return fail(PNED_E_ARG, "record shorter than expected");
+    return PNED_OK;
+}
+
+int read_host(FILE *f, void *p, size_t bytes)
+{
+    unsigned char *c = (unsigned char *)p;
+    return read_record(f, bytes, [&](size_t off, const unsigned char *src, size_t n) -> int {
+        std::memcpy(c + off, src, n);
+        return PNED_OK;
+    });
+}
+
+constexpr size_t STAGE = (size_t)64 << 20;
+
+int write_vec(FILE *f, const pned_vec *v)
+{
+    if (v->dtype != PNED_C128 || v->nvec != 1)
+        return fail(PNED_E_UNSUPPORTED, "vector files hold one complex128 column");
+    PNED_HIP(hipSetDevice(v->ctx->device));
+    PNED_HIP(hipStreamSynchronize(v->ctx->stream));
+    void *pin = nullptr;
+    PNED_HIP(hipHostMalloc(&pin, STAGE, hipHostMallocDefault));
+    const unsigned char *dev = (const unsigned char *)v->data;
+    const int rc = write_record(f, v->bytes, [&](FILE *ff, size_t off, size_t n) -> int {
+        size_t done = 0;
+        while (done < n) {
+            const size_t m = std::min(STAGE, n - done);
+            if (hipMemcpy(pin, dev + off + done, m, hipMemcpyDeviceToHost) != hipSuccess)
+                return fail(PNED_E_HIP, "device to host copy failed");
+            if (fwrite(pin, 1, m, ff) != m) return fail(PNED_E_ARG, "write failed");
+            done += m;
+        }
+        return PNED_OK;
+    });
+    (void)hipHostFree(pin);
+    return rc;
+}
+
+int read_vec(FILE *f, pned_vec *v)
+{
+    if (v->dtype != PNED_C128 || v->nvec != 1)
+        return fail(PNED_E_UNSUPPORTED, "vector files hold one complex128 column");
+    PNED_HIP(hipSetDevice(v->ctx->device));
+    PNED_HIP(hipStreamSynchronize(v->ctx->stream));
+    void *pin = nullptr;
+    PNED_HIP(hipHostMalloc(&pin, STAGE, hipHostMallocDefault));
+    unsigned char *dev = (unsigned char *)v->data;
+    size_t fill = 0, base = 0;   // staged bytes, device offset of the stage
+    int rc = read_record(f, v->bytes, [&](size_t off, const unsigned char *src, size_t n) -> int {
+        size_t done = 0;
+        while (done < n) {
+            if (fill == 0) base = off + done;
+            const size_t m = std::min(STAGE - fill, n - done);
+            std::memcpy((unsigned char *)pin + fill, src + done, m);
+            fill += m;
+            done += m;
+            if (fill == STAGE) {
+                if (hipMemcpy(dev + base, pin, fill, hipMemcpyHostToDevice) != hipSuccess)
+                    return fail(PNED_E_HIP, "host to device copy failed");
+                fill = 0;
+            }
+        }
+        return PNED_OK;
+    });
+    if (rc == PNED_OK && fill > 0 && hipMemcpy(dev + base, pin, fill, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(PNED_E_HIP, "host to device copy failed");
+    (void)hipHostFree(pin);
+    return rc;
+}
+
+int open_file(File &ff, const char *path, const char *mode)
+{
+    if (!path) return fail(PNED_E_ARG, "null path");
+    ff.f = fopen(path, mode);
+    if (!ff.f) return fail(PNED_E_ARG, std::string("cannot open ") + path);
+    return PNED_OK;
+}
+
+}  // namespace
+}  // namespace pned
+
+using namespace pned;
+
+extern "C" {
+
+int pned_io_subrecord_limit(uint64_t bytes)
+{
+    if (bytes != 0 && (bytes < 8 || bytes > 2147483639ull))
+        return fail(PNED_E_ARG, "subrecord limit in 8 .. 2^31-9 (0 = default)");
+    g_sub_limit = bytes ? (size_t)bytes : 2147483639;
+    return PNED_OK;
+}
+
+int pned_dump_alpha_beta_coef(const char *path, int num_data1, int num_data2, const double *alpha,
+                              const double *beta, const double *coef)
+{
+    if (num_data1 < 1 || num_data2 < 1 || !alpha || !beta || !coef) return fail(PNED_E_ARG, "bad argument");
+    File ff;
+    PNED_TRY(open_file(ff, path, "wb"));
+    const int32_t n1 = num_data1, n2 = num_data2;
+    PNED_TRY(write_host(ff.f, &n1, 4));
+    PNED_TRY(write_host(ff.f, &n2, 4));
+    PNED_TRY(write_host(ff.f, alpha, sizeof(double) * (size_t)num_data1));
+    PNED_TRY(write_host(ff.f, beta, sizeof(double) * (size_t)num_data1));
+    PNED_TRY(write_host(ff.f, coef, sizeof(double) * (size_t)num_data1 * num_data2));
+    return PNED_OK;
+}
+
+int pned_read_alpha_beta_coef(const char *path, int *num_data1, int *num_data2, double *alpha, double *beta,
+                              double *coef, int max_data1, int max_data2)
+{
+    if (!num_data1 || !num_data2) return fail(PNED_E_ARG, "null argument");
+    File ff;
+    PNED_TRY(open_file(ff, path, "rb"));
+    int32_t n1, n2;
+    PNED_TRY(read_host(ff.f, &n1, 4));
+    PNED_TRY(read_host(ff.f, &n2, 4));
+    *num_data1 = n1;
+    *num_data2 = n2;
+    if (!alpha && !beta && !coef) return PNED_OK;   // sizes only
+    if (!alpha || !beta || !coef || n1 > max_data1 || n2 > max_data2 || n1 < 1 || n2 < 1)
+        return fail(PNED_E_ARG, "arrays missing or smaller than the file's num_data1 x num_data2");
+    PNED_TRY(read_host(ff.f, alpha, sizeof(double) * (size_t)n1));
+    PNED_TRY(read_host(ff.f, beta, sizeof(double) * (size_t)n1));
+    PNED_TRY(read_host(ff.f, coef, sizeof(double) * (size_t)n1 * n2));
+    return PNED_OK;
+}
+
+int pned_dump_dsf_alpha_beta(const char *path, double sq, int n_alpha, const double *salpha, int n_beta,
+                             const double *sbeta)
+{
+    if (n_alpha < 1 || n_beta < 1 || !salpha || !sbeta) return fail(PNED_E_ARG, "bad argument");
+    File ff;
+    PNED_TRY(open_file(ff, path, "wb"));
+    PNED_TRY(write_host(ff.f, &sq, sizeof(double)));
+    PNED_TRY(write_host(ff.f, salpha, sizeof(double) * (size_t)n_alpha));
+    PNED_TRY(write_host(ff.f, sbeta, sizeof(double) * (size_t)n_beta));
+    return PNED_OK;
+}
+
+int pned_read_dsf_alpha_beta(const char *path, double *sq, int n_alpha, double *salpha, int n_beta, double *sbeta)
+{
+    if (!sq || n_alpha < 1 || n_beta < 1 || !salpha || !sbeta) return fail(PNED_E_ARG, "bad argument");
+    File ff;
+    PNED_TRY(open_file(ff, path, "rb"));
+    PNED_TRY(read_host(ff.f, sq, sizeof(double)));
+    PNED_TRY(read_host(ff.f, salpha, sizeof(double) * (size_t)n_alpha));
+    PNED_TRY(read_host(ff.f, sbeta, sizeof(double) * (size_t)n_beta));
+    return PNED_OK;
+}
+
+int pned_dump_vectors(const char *path, int indx, int nvec, const pned_vec *const *vecs)
+{
+    if (nvec < 1 || !vecs) return fail(PNED_E_ARG, "bad argument");
+    for (int k = 0; k < nvec; ++k)
+        if (!vecs[k]) return fail(PNED_E_ARG, "null vector");
+    File ff;
+    PNED_TRY(open_file(ff, path, "wb"));
+    const int32_t ix = indx;
+    PNED_TRY(write_host(ff.f, &ix, 4));
+    for (int k = 0; k < nvec; ++k) PNED_TRY(write_vec(ff.f, vecs[k]));
+    return PNED_OK;
+}
+
+int pned_read_vectors(const char *path, int *indx, int nvec, pned_vec *const *vecs)
+{
+    if (!indx || nvec < 1 || !vecs) return fail(PNED_E_ARG, "bad argument");
+    for (int k = 0; k < nvec; ++k)
+        if (!vecs[k]) return fail(PNED_E_ARG, "null vector");
+    File ff;
+    PNED_TRY(open_file(ff, path, "rb"));
+    int32_t ix;
+    PNED_TRY(read_host(ff.f, &ix, 4));
+    *indx = ix;
+    for (int k = 0; k < nvec; ++k) PNED_TRY(read_vec(ff.f, vecs[k]));
+    return PNED_OK;
+}
+
+int pned_basis_write(const char *path, double e, const pned_vec *x)
+{
+    if (!x) return fail(PNED_E_ARG, "null vector");
+    File ff;
+    PNED_TRY(open_file(ff, path, "wb"));
+    PNED_TRY(write_host(ff.f, &e, sizeof(double)));
+    return write_vec(ff.f, x);
+}
+
+int pned_basis_read(const char *path, double *e, pned_vec *x)
+{
+    if (!e || !x) return fail(PNED_E_ARG, "null argument");
+    File ff;
+    PNED_TRY(open_file(ff, path, "rb"));
+    PNED_TRY(read_host(ff.f, e, sizeof(double)));
+    return read_vec(ff.f, x);
+}
+
+}  // extern "C"

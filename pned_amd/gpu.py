@@ -54,6 +54,16 @@ def lib():
     L.pned_ctx_spin_kernel.argtypes = [P, i32]
     L.pned_ctx_s12_kernel.argtypes = [P, i32]
     L.pned_ctx_sz_sector.argtypes = [P, i32]
+    L.pned_io_subrecord_limit.argtypes = [C.c_uint64]
+    L.pned_dump_alpha_beta_coef.argtypes = [C.c_char_p, i32, i32, _dp, _dp, _dp]
+    L.pned_read_alpha_beta_coef.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.c_void_p, C.c_void_p,
+                                            C.c_void_p, i32, i32]
+    L.pned_dump_dsf_alpha_beta.argtypes = [C.c_char_p, C.c_double, i32, _dp, i32, _dp]
+    L.pned_read_dsf_alpha_beta.argtypes = [C.c_char_p, C.POINTER(C.c_double), i32, _dp, i32, _dp]
+    L.pned_dump_vectors.argtypes = [C.c_char_p, i32, i32, C.POINTER(P)]
+    L.pned_read_vectors.argtypes = [C.c_char_p, C.POINTER(i32), i32, C.POINTER(P)]
+    L.pned_basis_write.argtypes = [C.c_char_p, C.c_double, P]
+    L.pned_basis_read.argtypes = [C.c_char_p, C.POINTER(C.c_double), P]
     L.pned_ctpq_eval.argtypes = [i32, i32, i32, _dp, _dp, _dp, C.c_double, C.c_double, i32, _dp]
     L.pned_ctx_autotune.argtypes = [P, P, P, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                     C.POINTER(f64)]
@@ -221,6 +231,25 @@ class Context:
 
     def spin_kernel(self, tiled: bool):
         check(lib().pned_ctx_spin_kernel(self.h, 1 if tiled else 0))
+
+    def dump_vectors(self, path, indx: int, vecs):
+        """vec_q*/vec_me* file: indx, then each (device) vector as one record."""
+        arr = (C.c_void_p * len(vecs))(*[v.h for v in vecs])
+        check(lib().pned_dump_vectors(str(path).encode(), int(indx), len(vecs), arr))
+
+    def read_vectors(self, path, vecs) -> int:
+        arr = (C.c_void_p * len(vecs))(*[v.h for v in vecs])
+        ix = C.c_int()
+        check(lib().pned_read_vectors(str(path).encode(), C.byref(ix), len(vecs), arr))
+        return ix.value
+
+    def basis_write(self, path, e: float, x: "Vector"):
+        check(lib().pned_basis_write(str(path).encode(), float(e), x.h))
+
+    def basis_read(self, path, x: "Vector") -> float:
+        e = C.c_double()
+        check(lib().pned_basis_read(str(path).encode(), C.byref(e), x.h))
+        return e.value
 
     def sz_sector(self, n_down: int):
         """Restrict to the S^z sector with n_down set bits (n_down < 0: full space);
@@ -444,6 +473,43 @@ def ctpq(alpha, beta, coef, t_min, t_max, num_div):
     check(lib().pned_ctpq_eval(nd1, c.shape[1], seeds, a.reshape(-1), b.reshape(-1), c.reshape(-1),
                                float(t_min), float(t_max), int(num_div), out))
     return out.reshape(num_div + 1, 4)
+
+
+def io_subrecord_limit(nbytes: int = 0):
+    """Largest subrecord the file writers emit (0 = 2^31-9, the 4-byte marker limit)."""
+    check(lib().pned_io_subrecord_limit(int(nbytes)))
+
+
+def dump_alpha_beta_coef(path, alpha, beta, coef):
+    """SalphaSbeta_me*seed* file of an mTPQ sample: alpha, beta (nd1,), coef (nd1, nd2)."""
+    a = np.ascontiguousarray(alpha, dtype=float)
+    b = np.ascontiguousarray(beta, dtype=float)
+    c = np.asarray(coef, dtype=float)
+    check(lib().pned_dump_alpha_beta_coef(str(path).encode(), len(a), c.shape[1], a, b, np.ascontiguousarray(c.T)))
+
+
+def read_alpha_beta_coef(path):
+    n1, n2 = C.c_int(), C.c_int()
+    check(lib().pned_read_alpha_beta_coef(str(path).encode(), C.byref(n1), C.byref(n2), None, None, None, 0, 0))
+    a, b = np.zeros(n1.value), np.zeros(n1.value)
+    c = np.zeros((n2.value, n1.value))
+    check(lib().pned_read_alpha_beta_coef(str(path).encode(), C.byref(n1), C.byref(n2), a.ctypes.data,
+                                          b.ctypes.data, c.ctypes.data, n1.value, n2.value))
+    return a, b, c.T.copy()
+
+
+def dump_dsf_alpha_beta(path, sq, salpha, sbeta):
+    """SalphaSbeta_q*me* file of a DSF run: sq, salpha, sbeta."""
+    a = np.ascontiguousarray(salpha, dtype=float)
+    b = np.ascontiguousarray(sbeta, dtype=float)
+    check(lib().pned_dump_dsf_alpha_beta(str(path).encode(), float(sq), len(a), a, len(b), b))
+
+
+def read_dsf_alpha_beta(path, n_alpha, n_beta):
+    sq = C.c_double()
+    a, b = np.zeros(n_alpha), np.zeros(n_beta)
+    check(lib().pned_read_dsf_alpha_beta(str(path).encode(), C.byref(sq), n_alpha, a, n_beta, b))
+    return sq.value, a, b
 
 
 def bisec(alpha, beta, ne, eps=1e-25):
