@@ -272,9 +272,10 @@ int pned_dump_alpha_beta_coef(const char *path, int num_data1, int num_data2, co
 int pned_read_alpha_beta_coef(const char *path, int *num_data1, int *num_data2, double *alpha, double *beta,
                               double *coef, int max_data1, int max_data2);
 /* DSF continued-fraction coefficients (S_one_half/Hybrid_TPS_DSF.f90:353-443 dump_alpha_beta
- * / read_alpha_beta2): sq, salpha(n_alpha), sbeta(n_beta) */
-int pned_dump_dsf_alpha_beta(const char *path, double sq, int n_alpha, const double *salpha, int n_beta,
-                             const double *sbeta);
+ * / read_alpha_beta2): sq (complex: sq[0] real, sq[1] imaginary part), salpha(n_alpha),
+ * sbeta(n_beta) */
+int pned_dump_dsf_alpha_beta(const char *path, const double *sq, int n_alpha, const double *salpha,
+                             int n_beta, const double *sbeta);
 int pned_read_dsf_alpha_beta(const char *path, double *sq, int n_alpha, double *salpha, int n_beta,
                              double *sbeta);
 /* vector files (vec_q*, vec_me*: Hybrid_TPS_DSF.f90:374-379, Hybrid_TPS_TPS.f90:530-537):

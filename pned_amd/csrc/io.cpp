@@ -230,13 +230,13 @@ int pned_read_alpha_beta_coef(const char *path, int *num_data1, int *num_data2, 
     return PNED_OK;
 }
 
-int pned_dump_dsf_alpha_beta(const char *path, double sq, int n_alpha, const double *salpha, int n_beta,
-                             const double *sbeta)
+int pned_dump_dsf_alpha_beta(const char *path, const double *sq, int n_alpha, const double *salpha,
+                             int n_beta, const double *sbeta)
 {
-    if (n_alpha < 1 || n_beta < 1 || !salpha || !sbeta) return fail(PNED_E_ARG, "bad argument");
+    if (!sq || n_alpha < 1 || n_beta < 1 || !salpha || !sbeta) return fail(PNED_E_ARG, "bad argument");
     File ff;
     PNED_TRY(open_file(ff, path, "wb"));
-    PNED_TRY(write_host(ff.f, &sq, sizeof(double)));
+    PNED_TRY(write_host(ff.f, sq, 2 * sizeof(double)));   // complex(kind(0d0)) sq
     PNED_TRY(write_host(ff.f, salpha, sizeof(double) * (size_t)n_alpha));
     PNED_TRY(write_host(ff.f, sbeta, sizeof(double) * (size_t)n_beta));
     return PNED_OK;
@@ -247,7 +247,7 @@ int pned_read_dsf_alpha_beta(const char *path, double *sq, int n_alpha, double *
     if (!sq || n_alpha < 1 || n_beta < 1 || !salpha || !sbeta) return fail(PNED_E_ARG, "bad argument");
     File ff;
     PNED_TRY(open_file(ff, path, "rb"));
-    PNED_TRY(read_host(ff.f, sq, sizeof(double)));
+    PNED_TRY(read_host(ff.f, sq, 2 * sizeof(double)));
     PNED_TRY(read_host(ff.f, salpha, sizeof(double) * (size_t)n_alpha));
     PNED_TRY(read_host(ff.f, sbeta, sizeof(double) * (size_t)n_beta));
     return PNED_OK;

@@ -172,6 +172,92 @@ module pned_gpu
       integer(c_int), intent(out) :: conv(*)
       integer(c_int) :: rc
     end function
+    function pned_ctx_sz_sector(ctx, n_down) bind(C, name="pned_ctx_sz_sector") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n_down
+      integer(c_int) :: rc
+    end function
+    function pned_cocg(ctx, e0, eta, b, x, r, p, y, max_it, tol, iters, resid) bind(C, name="pned_cocg") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, b, x, r, p, y
+      real(c_double), value :: e0, eta, tol
+      integer(c_int), value :: max_it
+      integer(c_int), intent(out) :: iters
+      real(c_double), intent(out) :: resid
+      integer(c_int) :: rc
+    end function
+    function pned_ctpq_eval(nd1, nd2, seed_num, alpha, beta, coef, t_min, t_max, num_div, out) &
+        bind(C, name="pned_ctpq_eval") result(rc)
+      import :: c_int, c_double
+      integer(c_int), value :: nd1, nd2, seed_num, num_div
+      real(c_double), intent(in) :: alpha(*), beta(*), coef(*)
+      real(c_double), value :: t_min, t_max
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: rc
+    end function
+    function pned_dump_alpha_beta_coef(path, nd1, nd2, alpha, beta, coef) &
+        bind(C, name="pned_dump_alpha_beta_coef") result(rc)
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: nd1, nd2
+      real(c_double), intent(in) :: alpha(*), beta(*), coef(*)
+      integer(c_int) :: rc
+    end function
+    function pned_read_alpha_beta_coef(path, nd1, nd2, alpha, beta, coef, max1, max2) &
+        bind(C, name="pned_read_alpha_beta_coef") result(rc)
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: nd1, nd2
+      real(c_double), intent(out) :: alpha(*), beta(*), coef(*)
+      integer(c_int), value :: max1, max2
+      integer(c_int) :: rc
+    end function
+    function pned_dump_dsf_alpha_beta(path, sq, n_alpha, salpha, n_beta, sbeta) &
+        bind(C, name="pned_dump_dsf_alpha_beta") result(rc)
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      real(c_double), intent(in) :: sq(2), salpha(*), sbeta(*)
+      integer(c_int), value :: n_alpha, n_beta
+      integer(c_int) :: rc
+    end function
+    function pned_read_dsf_alpha_beta(path, sq, n_alpha, salpha, n_beta, sbeta) &
+        bind(C, name="pned_read_dsf_alpha_beta") result(rc)
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      real(c_double), intent(out) :: sq(2), salpha(*), sbeta(*)
+      integer(c_int), value :: n_alpha, n_beta
+      integer(c_int) :: rc
+    end function
+    function pned_dump_vectors(path, indx, nvec, vecs) bind(C, name="pned_dump_vectors") result(rc)
+      import :: c_int, c_ptr, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: indx, nvec
+      type(c_ptr), intent(in) :: vecs(*)
+      integer(c_int) :: rc
+    end function
+    function pned_read_vectors(path, indx, nvec, vecs) bind(C, name="pned_read_vectors") result(rc)
+      import :: c_int, c_ptr, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: indx
+      integer(c_int), value :: nvec
+      type(c_ptr), intent(in) :: vecs(*)
+      integer(c_int) :: rc
+    end function
+    function pned_basis_write(path, e, x) bind(C, name="pned_basis_write") result(rc)
+      import :: c_int, c_ptr, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      real(c_double), value :: e
+      type(c_ptr), value :: x
+      integer(c_int) :: rc
+    end function
+    function pned_basis_read(path, e, x) bind(C, name="pned_basis_read") result(rc)
+      import :: c_int, c_ptr, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      real(c_double), intent(out) :: e
+      type(c_ptr), value :: x
+      integer(c_int) :: rc
+    end function
   end interface
 contains
   !! print the library's message and stop, as the reference does on its errors
@@ -271,5 +357,82 @@ contains
     call pned_check(pned_lnc1(pned_context, v1, v0, int(max_itr, c_int), int(nvec2, c_int), 0_c_int, &
                     alpha, beta, ene, it), 'lnc1')
     itr = it
+  end subroutine
+
+  !! restrict the context to the S^z sector with n_down down spins (parity_type = 0;
+  !! the index tables of mk_list12, Hybrid_TPS_parallel.f90:452-578)
+  subroutine pned_set_sz_sector(n_down)
+    integer, intent(in) :: n_down
+    call pned_check(pned_ctx_sz_sector(pned_context, int(n_down, c_int)), 'sz_sector')
+  end subroutine
+
+  !! readwrite_vec_data (Hybrid_TPS_output.f90:11-35): file "basisme<ind>", e then x;
+  !! dump = .true. writes, .false. reads
+  subroutine readwrite_vec_data(ind, dump, e, x)
+    integer, intent(in) :: ind
+    logical, intent(in) :: dump
+    double precision, intent(inout) :: e
+    type(c_ptr), intent(in) :: x
+    character(len=32) :: fname
+    write(fname, '(A,I5.5)') 'basisme', ind
+    if (dump) then
+      call pned_check(pned_basis_write(trim(fname)//c_null_char, e, x), 'readwrite_vec_data')
+    else
+      call pned_check(pned_basis_read(trim(fname)//c_null_char, e, x), 'readwrite_vec_data')
+    end if
+  end subroutine
+
+  !! dump_alpha_beta_coef (Hybrid_TPS_TPS.f90:507-538): the mTPQ norms and observables of
+  !! one sample, file SalphaSbeta_me<pme>seed<indx1><comp>
+  subroutine dump_alpha_beta_coef(comp, indx1, pme, alpha, beta, coef)
+    character(len=2), intent(in) :: comp
+    integer, intent(in) :: indx1, pme
+    double precision, intent(in) :: alpha(:), beta(:), coef(:,:)
+    character(len=128) :: fname
+    write(fname, '(a,i5.5,a,i5.5,a)') 'SalphaSbeta_me', pme, 'seed', indx1, trim(comp)
+    call pned_check(pned_dump_alpha_beta_coef(trim(fname)//c_null_char, int(ubound(alpha, 1), c_int), &
+                    int(ubound(coef, 2), c_int), alpha, beta, coef), 'dump_alpha_beta_coef')
+  end subroutine
+
+  !! dump_alpha_beta (Hybrid_TPS_DSF.f90:353-379): continued-fraction file
+  !! SalphaSbeta_q<iq>me<pme><comp> and the vector file vec_q<iq>me<pme><comp>
+  subroutine dump_alpha_beta(iq, pme, indx, sq, comp, salpha, sbeta, v1, v2)
+    integer, intent(in) :: iq, pme, indx
+    complex(kind(0d0)), intent(in) :: sq
+    character(len=2), intent(in) :: comp
+    double precision, intent(in) :: salpha(:), sbeta(:)
+    type(c_ptr), intent(in) :: v1, v2
+    character(len=128) :: fname
+    real(c_double) :: s(2)
+    type(c_ptr) :: vv(2)
+    s = [dble(sq), dimag(sq)]
+    write(fname, '(a,i5.5,a,i5.5,a)') 'SalphaSbeta_q', iq, 'me', pme, trim(comp)
+    call pned_check(pned_dump_dsf_alpha_beta(trim(fname)//c_null_char, s, int(size(salpha), c_int), salpha, &
+                    int(size(sbeta), c_int), sbeta), 'dump_alpha_beta')
+    write(fname, '(a,i5.5,a,i5.5,a)') 'vec_q', iq, 'me', pme, trim(comp)
+    vv = [v1, v2]
+    call pned_check(pned_dump_vectors(trim(fname)//c_null_char, int(indx, c_int), 2_c_int, vv), 'dump_alpha_beta')
+  end subroutine
+
+  !! read_alpha_beta2 (Hybrid_TPS_DSF.f90:382-419): the restart counterpart of dump_alpha_beta
+  subroutine read_alpha_beta2(iq, pme, indx, sq, comp, salpha, sbeta, v1, v2)
+    integer, intent(in) :: iq, pme
+    integer, intent(inout) :: indx
+    complex(kind(0d0)), intent(inout) :: sq
+    character(len=2), intent(in) :: comp
+    double precision, intent(inout) :: salpha(:), sbeta(:)
+    type(c_ptr), intent(in) :: v1, v2
+    character(len=128) :: fname
+    real(c_double) :: s(2)
+    integer(c_int) :: ix
+    type(c_ptr) :: vv(2)
+    write(fname, '(a,i5.5,a,i5.5,a)') 'SalphaSbeta_q', iq, 'me', pme, trim(comp)
+    call pned_check(pned_read_dsf_alpha_beta(trim(fname)//c_null_char, s, int(size(salpha), c_int), salpha, &
+                    int(size(sbeta), c_int), sbeta), 'read_alpha_beta2')
+    sq = dcmplx(s(1), s(2))
+    write(fname, '(a,i5.5,a,i5.5,a)') 'vec_q', iq, 'me', pme, trim(comp)
+    vv = [v1, v2]
+    call pned_check(pned_read_vectors(trim(fname)//c_null_char, ix, 2_c_int, vv), 'read_alpha_beta2')
+    indx = ix
   end subroutine
 end module pned_parallelized_diag

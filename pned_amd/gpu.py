@@ -58,8 +58,8 @@ def lib():
     L.pned_dump_alpha_beta_coef.argtypes = [C.c_char_p, i32, i32, _dp, _dp, _dp]
     L.pned_read_alpha_beta_coef.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.c_void_p, C.c_void_p,
                                             C.c_void_p, i32, i32]
-    L.pned_dump_dsf_alpha_beta.argtypes = [C.c_char_p, C.c_double, i32, _dp, i32, _dp]
-    L.pned_read_dsf_alpha_beta.argtypes = [C.c_char_p, C.POINTER(C.c_double), i32, _dp, i32, _dp]
+    L.pned_dump_dsf_alpha_beta.argtypes = [C.c_char_p, _dp, i32, _dp, i32, _dp]
+    L.pned_read_dsf_alpha_beta.argtypes = [C.c_char_p, _dp, i32, _dp, i32, _dp]
     L.pned_dump_vectors.argtypes = [C.c_char_p, i32, i32, C.POINTER(P)]
     L.pned_read_vectors.argtypes = [C.c_char_p, C.POINTER(i32), i32, C.POINTER(P)]
     L.pned_basis_write.argtypes = [C.c_char_p, C.c_double, P]
@@ -499,17 +499,19 @@ def read_alpha_beta_coef(path):
 
 
 def dump_dsf_alpha_beta(path, sq, salpha, sbeta):
-    """SalphaSbeta_q*me* file of a DSF run: sq, salpha, sbeta."""
+    """SalphaSbeta_q*me* file of a DSF run: sq (complex), salpha, sbeta."""
     a = np.ascontiguousarray(salpha, dtype=float)
     b = np.ascontiguousarray(sbeta, dtype=float)
-    check(lib().pned_dump_dsf_alpha_beta(str(path).encode(), float(sq), len(a), a, len(b), b))
+    s = np.array([complex(sq).real, complex(sq).imag])
+    check(lib().pned_dump_dsf_alpha_beta(str(path).encode(), s, len(a), a, len(b), b))
 
 
 def read_dsf_alpha_beta(path, n_alpha, n_beta):
-    sq = C.c_double()
+    """Returns (sq as a complex number, salpha, sbeta)."""
+    s = np.zeros(2)
     a, b = np.zeros(n_alpha), np.zeros(n_beta)
-    check(lib().pned_read_dsf_alpha_beta(str(path).encode(), C.byref(sq), n_alpha, a, n_beta, b))
-    return sq.value, a, b
+    check(lib().pned_read_dsf_alpha_beta(str(path).encode(), s, n_alpha, a, n_beta, b))
+    return complex(s[0], s[1]), a, b
 
 
 def bisec(alpha, beta, ne, eps=1e-25):

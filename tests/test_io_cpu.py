@@ -72,10 +72,13 @@ def test_dsf_alpha_beta_round_trip(tmp_path):
     r = np.random.default_rng(3)
     sa, sb = r.standard_normal(30), r.random(31)
     f = tmp_path / "SalphaSbeta_q00001me00000zz"
-    gpu.dump_dsf_alpha_beta(f, 0.375, sa, sb)
-    assert [x[0] for x in _records(f)] == [8, 240, 248]
+    gpu.dump_dsf_alpha_beta(f, 0.375 - 0.125j, sa, sb)
+    # sq is complex(kind(0d0)) in the reference (Hybrid_TPS_DSF.f90:358): a 16-byte record
+    recs = _records(f)
+    assert [x[0] for x in recs] == [16, 240, 248]
+    assert recs[0][2] == np.array([0.375, -0.125]).tobytes()
     sq, sa2, sb2 = gpu.read_dsf_alpha_beta(f, 30, 31)
-    assert sq == 0.375 and np.array_equal(sa2, sa) and np.array_equal(sb2, sb)
+    assert sq == 0.375 - 0.125j and np.array_equal(sa2, sa) and np.array_equal(sb2, sb)
 
 
 FORTRAN_READ = """
@@ -83,7 +86,8 @@ program rd
   implicit none
   integer :: num_data1, num_data2
   double precision, allocatable :: ta(:), tb(:), tc(:,:)
-  double precision :: sq, sa(30), sb(31)
+  complex(kind(0d0)) :: sq
+  double precision :: sa(30), sb(31)
   open(11, file='SalphaSbeta_me00000seed00001tp', form='unformatted', access='sequential', status='old')
   read(11) num_data1
   read(11) num_data2
@@ -99,7 +103,7 @@ program rd
   read(12) sa
   read(12) sb
   close(12)
-  write(*,'(ES25.17)') sq, sa(7), sb(31)
+  write(*,'(ES25.17)') dble(sq), dimag(sq), sa(7), sb(31)
 end program rd
 """
 
@@ -108,6 +112,7 @@ program wr
   implicit none
   integer :: i, k
   double precision :: alpha(9), beta(9), coef(9,8)
+  complex(kind(0d0)) :: sq
   do i = 1, 9
     alpha(i) = 0.5d0 * i
     beta(i) = 1d0 / i
@@ -122,6 +127,12 @@ program wr
   write(11) beta
   write(11) coef
   close(11)
+  sq = dcmplx(1.5d0, -0.25d0)
+  open(12, file='SalphaSbeta_q00002me00000zz', form='unformatted', access='sequential', status='unknown')
+  write(12) sq
+  write(12) alpha
+  write(12) beta
+  close(12)
 end program wr
 """
 
@@ -132,7 +143,7 @@ def test_fortran_program_reads_and_writes_the_same_files(tmp_path):
     gpu.dump_alpha_beta_coef(tmp_path / "SalphaSbeta_me00000seed00001tp", a, b, c)
     r = np.random.default_rng(5)
     sa, sb = r.standard_normal(30), r.random(31)
-    gpu.dump_dsf_alpha_beta(tmp_path / "SalphaSbeta_q00001me00000zz", -2.25, sa, sb)
+    gpu.dump_dsf_alpha_beta(tmp_path / "SalphaSbeta_q00001me00000zz", -2.25 + 0.5j, sa, sb)
     for name, src in (("rd", FORTRAN_READ), ("wr", FORTRAN_WRITE)):
         (tmp_path / f"{name}.f90").write_text(src)
         subprocess.run([FC, f"{name}.f90", "-o", name], cwd=tmp_path, check=True, capture_output=True)
@@ -141,9 +152,58 @@ def test_fortran_program_reads_and_writes_the_same_files(tmp_path):
     vals = [float(x) for x in out[2:]]
     assert vals[0] == a[2] and vals[1] == b[4] and vals[2] == c[3, 5]
     assert abs(vals[3] - c.sum()) <= 1e-12 * np.abs(c).sum()
-    assert vals[4] == -2.25 and vals[5] == sa[6] and vals[6] == sb[30]
+    assert vals[4:6] == [-2.25, 0.5] and vals[6] == sa[6] and vals[7] == sb[30]
     subprocess.run(["./wr"], cwd=tmp_path, check=True, capture_output=True)
     a2, b2, c2 = gpu.read_alpha_beta_coef(tmp_path / "SalphaSbeta_me00000seed00002tp")
     i = np.arange(1, 10)
     assert np.array_equal(a2, 0.5 * i) and np.array_equal(b2, 1.0 / i)
     assert np.array_equal(c2, i[:, None] + 100.0 * np.arange(1, 9)[None, :])
+    sq, sa3, sb3 = gpu.read_dsf_alpha_beta(tmp_path / "SalphaSbeta_q00002me00000zz", 9, 9)
+    assert sq == 1.5 - 0.25j and np.array_equal(sa3, 0.5 * i) and np.array_equal(sb3, 1.0 / i)
+
+
+FORTRAN_BINDINGS = """
+program bind
+  use iso_c_binding
+  use pned_gpu
+  use pned_parallelized_diag
+  implicit none
+  integer :: i, k
+  double precision :: alpha(12), beta(12), coef(12,9), out(4*5)
+  do i = 1, 12
+    alpha(i) = -0.1d0 * i
+    beta(i) = 1d0 + 0.01d0 * i
+    do k = 1, 9
+      coef(i,k) = -0.5d0 + 0.02d0 * i - 0.001d0 * k * i
+    end do
+  end do
+  call dump_alpha_beta_coef('tp', 1, 0, alpha, beta, coef)
+  call pned_check(pned_ctpq_eval(12_c_int, 9_c_int, 1_c_int, alpha, beta, coef, 0.5d0, 4d0, 4_c_int, out), 'ctpq')
+  write(*,'(ES25.17)') out
+end program bind
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(FC), reason="amdflang not available")
+def test_fortran_module_bindings(tmp_path):
+    """The ISO_C_BINDING layer's file and cTPQ entry points, called from Fortran: the file
+    it writes is the one the library's own reader returns, and the cTPQ columns equal the
+    library's through the Python binding."""
+    here = os.path.dirname(gpu.LIB_PATH)
+    mod = os.path.join(here, "fortran", "_build")
+    if not os.path.exists(os.path.join(mod, "libpned_fortran.a")):
+        pytest.skip("Fortran layer not built")
+    (tmp_path / "bind.f90").write_text(FORTRAN_BINDINGS)
+    subprocess.run([FC, "-I", mod, "bind.f90", os.path.join(mod, "libpned_fortran.a"), "-L", here,
+                    "-lpned_gpu", f"-Wl,-rpath,{here}", "-o", "bind"], cwd=tmp_path, check=True,
+                   capture_output=True)
+    out = subprocess.run(["./bind"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout
+    got = np.array([float(x) for x in out.split()]).reshape(5, 4)
+    i = np.arange(1, 13, dtype=float)
+    k = np.arange(1, 10, dtype=float)
+    alpha, beta = -0.1 * i, 1.0 + 0.01 * i
+    coef = -0.5 + 0.02 * i[:, None] - 0.001 * k[None, :] * i[:, None]
+    a2, b2, c2 = gpu.read_alpha_beta_coef(tmp_path / "SalphaSbeta_me00000seed00001tp")
+    assert np.array_equal(a2, alpha) and np.array_equal(b2, beta) and np.array_equal(c2, coef)
+    want = gpu.ctpq(alpha[None, :], beta[None, :], coef.T[None, :, :], 0.5, 4.0, 4)
+    np.testing.assert_allclose(got, want, rtol=1e-15, atol=0)
