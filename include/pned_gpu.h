@@ -238,6 +238,15 @@ int pned_tps_fused(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, doub
  * w = (d - 1/2) for S=1/2 (bit value d) or (d - S) for spin S; cnorm = ||.||^2 / N. */
 int pned_intalv_zz(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double q[2],
                    const double *site_vec, double *cnorm);
+/* intalv_xy (S_one_half/Hybrid_TPS_DSF.f90:89-193, S_arbitrary/Hybrid_TPS9_DSF.f90:91-286):
+ * v = normalised sum_j e^{i q.r_j} S^{ipm}_j x, cnorm = ||.||^2 / N; ipm = +1 (the
+ * reference's S+: digit j + 1) or -1.  Full space (parity_type != 0): x and v of one
+ * cluster and spin (x may belong to another context); spin S > 1/2 applies the ladder
+ * factor sqrt((S -/+ m)(S +/- m + 1)) of the source digit, the S=1/2 tree sets or clears
+ * bit j with factor 1.  S=1/2 sector basis (parity_type = 0): x in a context with k set
+ * bits, ctx and v in one with k + ipm. */
+int pned_intalv_xy(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double q[2],
+                   const double *site_vec, int ipm, double *cnorm);
 /* parallelized_dsflnc1z (Hybrid_TPS_DSF.f90:195-261): s_maxitr-1 Lanczos steps from the
  * normalised excitation in v1; salpha[s_maxitr], sbeta[s_maxitr+1] (sbeta[0] = cnorm). */
 int pned_dsflnc1z(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double cnorm, int s_maxitr,

@@ -59,6 +59,8 @@ def lib():
                                             _zp, _zp, _zp, _zp, _zp, i32, i32, i32]
         L.orc_inverse_iteration.restype = i32
         L.orc_sz_q.argtypes = [i32, i32, _dp, _dp, _zp, _zp]
+        L.orc_intalv_xy.argtypes = [i32, _dp, _dp, i64, _up, _zp, i64, _up, _up, _zp]
+        L.orc_intalv_pm.argtypes = [i32, i32, _dp, _dp, i32, _zp, _zp]
         L.orc_sq_apply.argtypes = [i32, _dp, _dp, i32, _zp, _zp]
         L.orc_zz_bond_apply.argtypes = [i32, i32, _ip, _ip, _zp, _zp]
         L.orc_tps.argtypes = [i32, i32, _ip, _ip, _ip, _zp, _bp, _dp, i32, _dp, f64, i32,
@@ -224,6 +226,36 @@ def sz_q(n, q, site_vec, qv, x):
     lib().orc_sz_q(n, q, np.ascontiguousarray(site_vec, dtype=np.float64).reshape(-1),
                    np.asarray(qv, dtype=np.float64), x, v)
     return v
+
+
+def intalv_xy(n, site_vec, qv, tot_sz, x):
+    """intalv_xy (Hybrid_TPS_DSF.f90:89-193): x in the sector with tot_sz set bits ->
+    normalised S^+(q) x in the sector with tot_sz + 1, and cnorm = ||S^+(q) x||^2 / n
+    (the reference's tiny-norm floor 1e-25 included)."""
+    listt, _, _ = sector_tables(n, tot_sz)
+    _, la, lb = sector_tables(n, tot_sz + 1)
+    ldim = lib().orc_mk_list12(n, tot_sz + 1, None, None, None)
+    v = np.zeros(ldim, dtype=np.complex128)
+    lib().orc_intalv_xy(n, np.ascontiguousarray(site_vec, dtype=np.float64).reshape(-1),
+                        np.asarray(qv, dtype=np.float64), len(listt), listt,
+                        np.ascontiguousarray(x, dtype=np.complex128), ldim, la, lb, v)
+    cn = float(np.vdot(v, v).real)
+    if cn < 1e-25:
+        cn = 1e-25
+    return v / np.sqrt(cn), cn / n
+
+
+def intalv_pm(n, q, site_vec, qv, ipm, x):
+    """intalv_xy over the full q^n space (parity_type != 0): normalised
+    sum_j e^{i q.r_j} S^{ipm}_j x (spin-S ladder factors; q = 2: bit j set or cleared)
+    and cnorm = ||.||^2 / n."""
+    v = np.zeros(q ** n, dtype=np.complex128)
+    lib().orc_intalv_pm(n, q, np.ascontiguousarray(site_vec, dtype=np.float64).reshape(-1),
+                        np.asarray(qv, dtype=np.float64), int(ipm), np.ascontiguousarray(x, dtype=np.complex128), v)
+    cn = float(np.vdot(v, v).real)
+    if cn < 1e-25:
+        cn = 1e-25
+    return v / np.sqrt(cn), cn / n
 
 
 def tps(oop: OracleOp, site_vec, sq_q, lmax, nsteps, v_start):

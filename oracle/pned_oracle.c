@@ -583,6 +583,56 @@ void orc_sz_q(int n, int q, const double *site_vec /* n x 2 */, const double qv[
     free(ph);
 }
 
+/* intalv_xy  H/DSF.f90:89-193 (S=1/2, one process): v = sum_j e^{i q.r_j} S^+_j x from
+ * the sector with tot_sz set bits (x indexed by that sector's list1) to the sector with
+ * tot_sz + 1 (v indexed by the target's la/lb tables).  Site-outer loop over the source
+ * list, as the reference scatters; S^+_j sets bit j.  Not normalised here. */
+void orc_intalv_xy(int n, const double *site_vec, const double qv[2], int64_t pdim, const uint32_t *listt,
+                   const zc *x, int64_t ldim, const uint32_t *la, const uint32_t *lb, zc *v)
+{
+    const int ihf = (n + 1) / 2;
+    const uint32_t right = (1u << ihf) - 1u;
+    for (int64_t l = 0; l < ldim; ++l) v[l] = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const double qn = qv[0] * site_vec[2 * j] + qv[1] * site_vec[2 * j + 1];
+        const double c = cos(qn), s = sin(qn);
+        for (int64_t i = 0; i < pdim; ++i) {
+            const uint32_t st = listt[i];
+            if ((st >> j) & 1u) continue;
+            const uint32_t t = st + (1u << j);
+            const int64_t it = (int64_t)la[t & right] + (int64_t)lb[t >> ihf] - 1;
+            v[it] += (c * creal(x[i]) - s * cimag(x[i])) + I * (s * creal(x[i]) + c * cimag(x[i]));
+        }
+    }
+}
+
+/* intalv_xy in the full q^n space (parity_type != 0; S/DSF.f90:91-286 local_operation at
+ * one process, H/DSF.f90:89-193 with listt = every state): site-outer scatter of
+ * fac * e^{i q.r_j} x(i) onto the state whose digit j is moved by ipm.  q >= 3: the
+ * spin-S ladder factor of the source digit d (m = d - S): sqrt((S-m)(S+m+1)) for ipm = +1,
+ * sqrt((S+m)(S-m+1)) for ipm = -1.  q = 2 (the S=1/2 tree): bit j set (+1) or cleared
+ * (-1), factor 1.  Not normalised here. */
+void orc_intalv_pm(int n, int q, const double *site_vec, const double qv[2], int ipm, const zc *x, zc *v)
+{
+    const int64_t d = ipow64(q, n);
+    const double ss = 0.5 * (q - 1);
+    for (int64_t i = 0; i < d; ++i) v[i] = 0.0;
+    int64_t base = 1;
+    for (int j = 0; j < n; ++j, base *= q) {
+        const double qn = qv[0] * site_vec[2 * j] + qv[1] * site_vec[2 * j + 1];
+        const double c = cos(qn), s = sin(qn);
+        for (int64_t i = 0; i < d; ++i) {
+            const int is = (int)((i / base) % q);
+            if (is + ipm < 0 || is + ipm > q - 1) continue;
+            const double sm = (double)is - ss;
+            const double fac = (q == 2) ? 1.0 : (ipm == 1 ? sqrt((ss - sm) * (ss + sm + 1.0))
+                                                           : sqrt((ss + sm) * (ss - sm + 1.0)));
+            const int64_t it = i + (int64_t)ipm * base;
+            v[it] += fac * ((c * creal(x[i]) - s * cimag(x[i])) + I * (s * creal(x[i]) + c * cimag(x[i])));
+        }
+    }
+}
+
 /* calc_sq_zz / calc_sq_pp  H/TPS.f90:146-251 (S=1/2): mode 0: (s_j-1/2)(s_0-1/2) e^{iqr_j};
  * mode 1: [s_j != s_0]/4 e^{iqr_j}. */
 void orc_sq_apply(int n, const double *site_vec, const double qv[2], int mode, const zc *x, zc *v)

@@ -13,6 +13,7 @@
 // context checks at set-up that every other term vanishes (the operator conserves S^z).
 // One thread per output index gathers its partners through the two half tables.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "internal.h"
@@ -94,6 +95,67 @@ __global__ __launch_bounds__(256) void hv_sector_kernel(const double2 *__restric
         partial[2 * blockIdx.x] = red[0];
         partial[2 * blockIdx.x + 1] = red[1];
     }
+}
+
+// intalv_xy (S_one_half/Hybrid_TPS_DSF.f90:89-193) as a gather: one thread per target
+// index l (sector k + ipm) sums e^{i q.r_j} x(source index of t with bit j flipped) over
+// the bits j of its state t that ipm has just set (ipm = +1) or cleared (-1), in ascending
+// j -- per element, the order in which the reference's site-outer scatter adds them.
+struct XyArgs {
+    int n, ipm;
+    double phr[32], phi[32];
+};
+
+__global__ __launch_bounds__(256) void k_intalv_xy(const double2 *__restrict__ x, double2 *__restrict__ v,
+                                                   const uint32_t *__restrict__ list1_dst,
+                                                   const uint32_t *__restrict__ rank,
+                                                   const uint32_t *__restrict__ offset_src, XyArgs a, int ihf,
+                                                   uint64_t dim)
+{
+    const uint32_t right = (1u << ihf) - 1u;
+    const uint64_t stride = (uint64_t)blockDim.x * gridDim.x;
+    for (uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; l < dim; l += stride) {
+        const uint32_t t = list1_dst[l];
+        double ar = 0.0, ai = 0.0;
+        for (int j = 0; j < a.n; ++j) {
+            if ((int)((t >> j) & 1u) != (a.ipm > 0 ? 1 : 0)) continue;
+            const uint32_t s = t ^ (1u << j);
+            const double2 y = x[(uint64_t)rank[s & right] + (uint64_t)offset_src[s >> ihf]];
+            ar += a.phr[j] * y.x - a.phi[j] * y.y;
+            ai += a.phi[j] * y.x + a.phr[j] * y.y;
+        }
+        v[l] = make_double2(ar, ai);
+    }
+}
+
+int intalv_xy_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, const double *site_vec,
+                    int ipm)
+{
+    const pned_ctx *src = x->ctx;
+    if (v->ctx != ctx) return fail(PNED_E_ARG, "v must belong to the target context");
+    if (ctx->q != 2 || !src || src->q != 2 || src->n != ctx->n)
+        return fail(PNED_E_ARG, "S^+(q): two S=1/2 contexts of the same cluster");
+    if (ctx->sector_k < 0 || src->sector_k < 0 || ctx->sector_k != src->sector_k + ipm)
+        return fail(PNED_E_ARG, "S^+/-(q): x in the sector with k set bits, v in the one with k + ipm");
+    if (src->device != ctx->device) return fail(PNED_E_ARG, "S^+(q): both contexts on one device");
+    if (x->nvec != 1 || v->nvec != 1 || x->dtype != PNED_C128 || v->dtype != PNED_C128)
+        return fail(PNED_E_UNSUPPORTED, "S^+(q) takes one complex128 column");
+    XyArgs a{};
+    a.n = ctx->n;
+    a.ipm = ipm;
+    for (int j = 0; j < ctx->n; ++j) {
+        const double qn = qv[0] * site_vec[2 * j] + qv[1] * site_vec[2 * j + 1];
+        a.phr[j] = std::cos(qn);
+        a.phi[j] = std::sin(qn);
+    }
+    // x was written on the source context's stream
+    PNED_HIP(hipStreamSynchronize(src->stream));
+    const uint64_t dim = (uint64_t)ctx->dim;
+    const int grid = (int)std::min<uint64_t>((dim + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_intalv_xy, dim3(grid), dim3(256), 0, ctx->stream, (const double2 *)x->data,
+                       (double2 *)v->data, ctx->d_list1, src->d_srank, src->d_soffset, a, (ctx->n + 1) / 2, dim);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
 }
 
 __global__ void sector_reduce(const double *__restrict__ partial, int nrows, double *__restrict__ out)

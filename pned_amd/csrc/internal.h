@@ -196,6 +196,12 @@ int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
 int launch_hv_sector(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                      double *prdct_host);
 void free_sector(pned_ctx *ctx);
+// intalv_xy, S=1/2 sectors: v (sector k + ipm of ctx) = sum_j e^{i q.r_j} (x with bit j
+// set (ipm = +1) or cleared (-1)); x in sector k of x->ctx
+int intalv_xy_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, const double *site_vec,
+                    int ipm);
+// intalv_xy over the full q^N space: v = sum_j e^{i q.r_j} S^{ipm}_j x (ladder factors)
+int intalv_pm_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, const double *site_vec, int ipm);
 int refresh_sector_terms(pned_ctx *ctx);   // after the bond matrices change
 // untuned tile defaults: true for vectors above 2^25 elements (dim x columns)
 bool default_large(const pned_ctx *ctx, int colbits);

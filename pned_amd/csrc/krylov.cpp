@@ -478,6 +478,26 @@ int pned_intalv_zz(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double q
     return PNED_OK;
 }
 
+int pned_intalv_xy(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double q[2], const double *site_vec,
+                   int ipm, double *cnorm)
+{
+    if (!ctx || !x || !v || !q || !site_vec || !cnorm || !x->ctx) return fail(PNED_E_ARG, "null argument");
+    if (ipm != 1 && ipm != -1) return fail(PNED_E_ARG, "ipm is +1 (S+) or -1 (S-)");
+    if (v->ctx != ctx) return fail(PNED_E_ARG, "v must belong to the target context");
+    if (ctx->sector_k >= 0 || x->ctx->sector_k >= 0)
+        PNED_TRY(intalv_xy_apply(ctx, x, v, q, site_vec, ipm));
+    else
+        PNED_TRY(intalv_pm_apply(ctx, x, v, q, site_vec, ipm));
+    double nn[2];
+    PNED_TRY(pned_calc_vec_inprdct(ctx, v, v, nn));
+    double cn = nn[0];
+    if (cabs2(nn) < 1e-25) cn = 1e-25;   // 'vec norm is too small' (Hybrid_TPS_DSF.f90:174-179)
+    const double s[2] = {1.0 / std::sqrt(cn), 0.0};
+    PNED_TRY(pned_scale(ctx, v, s));
+    *cnorm = cn / (double)ctx->n;
+    return PNED_OK;
+}
+
 // parallelized_dsflnc1z (S_one_half/Hybrid_TPS_DSF.f90:195-261)
 int pned_dsflnc1z(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double cnorm, int s_maxitr, double *salpha,
                   double *sbeta)

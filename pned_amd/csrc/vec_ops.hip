@@ -350,6 +350,44 @@ __global__ void k_sz_q(const typename V2<Real>::T *__restrict__ x, typename V2<R
     }
 }
 
+// ---- excitation S^{+/-}(q) x over the full q^N space (intalv_xy at parity_type != 0:
+// S_arbitrary/Hybrid_TPS9_DSF.f90:91-286, S_one_half/Hybrid_TPS_DSF.f90:89-193) ----------------
+// One thread per target element t gathers, over the sites j in ascending order (the order
+// the reference's site-outer scatter adds them to one element), fac(d_s) e^{i q.r_j} x(s)
+// where s has digit j moved back by ipm and d_s is that source digit.
+struct PmArgs {
+    int n, q, ipm;
+    double phr[32], phi[32];
+    double fac[16];        // ladder factor by source digit (1 for the S=1/2 tree)
+    uint64_t pw[32];       // q^j
+};
+
+__global__ __launch_bounds__(256) void k_intalv_pm(const double2 *__restrict__ x, double2 *__restrict__ v, PmArgs a,
+                                                   uint64_t dim)
+{
+    const uint64_t stride = (uint64_t)blockDim.x * gridDim.x;
+    for (uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; l < dim; l += stride) {
+        double ar = 0.0, ai = 0.0;
+        uint32_t rem = (uint32_t)l;   // q >= 3: dim < 2^32 (checked on the host)
+        for (int j = 0; j < a.n; ++j) {
+            int dt;
+            if (a.q == 2) {
+                dt = (int)((l >> j) & 1u);
+            } else {
+                dt = (int)(rem % (uint32_t)a.q);
+                rem /= (uint32_t)a.q;
+            }
+            const int ds = dt - a.ipm;
+            if (ds < 0 || ds >= a.q) continue;
+            const double2 y = x[a.ipm > 0 ? l - a.pw[j] : l + a.pw[j]];
+            const double rr = a.phr[j] * y.x - a.phi[j] * y.y, ri = a.phi[j] * y.x + a.phr[j] * y.y;
+            ar += a.fac[ds] * rr;
+            ai += a.fac[ds] * ri;
+        }
+        v[l] = make_double2(ar, ai);
+    }
+}
+
 // ---- mTPQ observables of eval_phys_val (Hybrid_TPS_TPS.f90:83-322), fused ------------------
 // Per column: out[0..1] = <vg|v1>, out[2..3] = <v1|Wzz|v1>, out[4..5] = <vg|Wzz|v1>,
 // then for each requested structure-factor entry k: out[6+2k..] = <a_k|W_k|v1>.
@@ -618,6 +656,41 @@ int same_shape(const pned_vec *a, const pned_vec *b)
 }
 
 }  // namespace
+
+int intalv_pm_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, const double *site_vec, int ipm)
+{
+    if (ctx->sector_k >= 0 || x->ctx->sector_k >= 0)
+        return fail(PNED_E_ARG, "S^+/-(q): both vectors in the full space, or both in sector contexts");
+    if (x->ctx->n != ctx->n || x->ctx->q != ctx->q || x->dim != v->dim)
+        return fail(PNED_E_ARG, "S^+/-(q): x and v of the same cluster and spin");
+    if (x->nvec != 1 || v->nvec != 1 || x->dtype != PNED_C128 || v->dtype != PNED_C128)
+        return fail(PNED_E_UNSUPPORTED, "S^+/-(q) takes one complex128 column");
+    if (ctx->n > 32 || ctx->q > 16 || (ctx->q > 2 && (uint64_t)ctx->dim > 0xFFFFFFFFull))
+        return fail(PNED_E_UNSUPPORTED, "S^+/-(q): at most 32 sites and 2^32 states for spin S > 1/2");
+    PmArgs a{};
+    a.n = ctx->n;
+    a.q = ctx->q;
+    a.ipm = ipm;
+    const double ss = 0.5 * (ctx->q - 1);
+    for (int d = 0; d < ctx->q; ++d) {
+        const double sm = (double)d - ss;
+        a.fac[d] = (ctx->q == 2) ? 1.0 : (ipm > 0 ? std::sqrt((ss - sm) * (ss + sm + 1.0))
+                                                 : std::sqrt((ss + sm) * (ss - sm + 1.0)));
+    }
+    uint64_t p = 1;
+    for (int j = 0; j < ctx->n; ++j, p *= (uint64_t)ctx->q) {
+        const double qn = qv[0] * site_vec[2 * j] + qv[1] * site_vec[2 * j + 1];
+        a.phr[j] = std::cos(qn);
+        a.phi[j] = std::sin(qn);
+        a.pw[j] = p;
+    }
+    if (x->ctx != ctx) PNED_HIP(hipStreamSynchronize(x->ctx->stream));   // x written on its own stream
+    const int grid = (int)std::min<uint64_t>(((uint64_t)v->dim + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_intalv_pm, dim3(grid), dim3(256), 0, ctx->stream, (const double2 *)x->data,
+                       (double2 *)v->data, a, (uint64_t)v->dim);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
 
 int dot(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out)
 {

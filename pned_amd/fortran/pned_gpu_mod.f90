@@ -153,6 +153,14 @@ module pned_gpu
       real(c_double), intent(out) :: cnorm
       integer(c_int) :: rc
     end function
+    function pned_intalv_xy(ctx, x, v, q, site_vec, ipm, cnorm) bind(C, name="pned_intalv_xy") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, x, v
+      real(c_double), intent(in) :: q(2), site_vec(*)
+      integer(c_int), value :: ipm
+      real(c_double), intent(out) :: cnorm
+      integer(c_int) :: rc
+    end function
     function pned_dsflnc1z(ctx, v1, v0, cnorm, s_maxitr, salpha, sbeta) bind(C, name="pned_dsflnc1z") result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx, v1, v0

@@ -100,6 +100,7 @@ def lib():
     L.pned_tps.argtypes = [P, P, P, P, P, f64, i32, _dp, _dp, _dp]
     L.pned_tps_fused.argtypes = [P, P, P, P, f64, i32, _dp, _dp, _dp]
     L.pned_intalv_zz.argtypes = [P, P, P, _dp, _dp, C.POINTER(f64)]
+    L.pned_intalv_xy.argtypes = [P, P, P, _dp, _dp, i32, C.POINTER(f64)]
     L.pned_dsflnc1z.argtypes = [P, P, P, f64, i32, _dp, _dp]
     L.pned_renbun.argtypes = [P, i32, i32, f64, f64, f64, _dp, _dp, f64, _dp, _dp, _ip]
     L.pned_ctx_create_batch.argtypes = [i32, i32, i32, _ip, _ip, i32, _dp, _ip, i32, C.POINTER(P)]
@@ -404,6 +405,16 @@ class Context:
         cn = C.c_double()
         check(lib().pned_intalv_zz(self.h, x.h, v.h, np.asarray(q, dtype=float),
                                    np.ascontiguousarray(np.asarray(site_vec, float).reshape(-1)), C.byref(cn)))
+        return cn.value
+
+    def intalv_xy(self, x: Vector, v: Vector, q, site_vec, ipm: int = 1) -> float:
+        """intalv_xy: v (of this context) = normalised S^{ipm}(q) x; returns cnorm.  Full
+        space: x of a context of the same cluster and spin.  S=1/2 sectors: x of a context
+        with k set bits, this one with k + ipm."""
+        cn = C.c_double()
+        check(lib().pned_intalv_xy(self.h, x.h, v.h, np.asarray(q, dtype=float),
+                                   np.ascontiguousarray(np.asarray(site_vec, float).reshape(-1)), int(ipm),
+                                   C.byref(cn)))
         return cn.value
 
     def dsflnc1z(self, v1: Vector, v0: Vector, cnorm: float, s_maxitr: int):

@@ -391,6 +391,84 @@ def test_sector_hv_parity(spec, k):
     assert abs(pr - pr_ref) <= 1e-13 * np.linalg.norm(x) * np.linalg.norm(want - y)
 
 
+@pytest.mark.parametrize("spec,k", [(12, 5), (16, 7), ("2x3", 6)])
+def test_intalv_xy_and_sector_dsf(spec, k):
+    """S^+(q) (intalv_xy) from a k-set-bit sector context to a (k+1) one against the
+    oracle, then the DSF Lanczos in the target sector against the oracle's full-space
+    Lanczos of the embedded vector (H conserves S^z, so the two runs are the same
+    Lanczos)."""
+    _say("intalv_xy + sector dsf", spec, k)
+    cl = lattice.get_cluster(spec)
+    op = model.build_operator(cl, 20, 0.0)
+    src, dst = gpu.Context(op), gpu.Context(op)
+    src.sz_sector(k)
+    dst.sz_sector(k + 1)
+    x = _rand(src.dim, 41)
+    xv = src.vector()
+    xv.upload(x)
+    w, v0 = dst.vector(), dst.vector()
+    q = cl.q_list[min(2, len(cl.q_list) - 1)]
+    cn = dst.intalv_xy(xv, w, q, cl.site_vec)
+    v_ref, cn_ref = oracle.intalv_xy(cl.n, cl.site_vec, q, k, x)
+    assert abs(cn - cn_ref) <= 1e-13 * cn_ref
+    assert np.linalg.norm(w.download() - v_ref) <= 1e-13
+    with pytest.raises(gpu.PnedError):   # x must come from the sector with one bit fewer
+        src.intalv_xy(xv, src.vector(), q, cl.site_vec)
+    sa, sb = dst.dsflnc1z(w, v0, cn, 30)
+    lst, _, _ = oracle.sector_tables(cl.n, k + 1)
+    vf = np.zeros(1 << cl.n, complex)
+    vf[lst.astype(np.int64)] = v_ref
+    sa_ref, sb_ref = oracle.dsf_lanczos(oracle.OracleOp(op), vf, cn_ref, 30)
+    assert np.allclose(sa[:20], sa_ref[:20], rtol=1e-11)
+    assert np.allclose(sb[:20], sb_ref[:20], rtol=1e-11)
+
+
+@pytest.mark.parametrize("spec,q,ipm", [(12, 2, 1), (16, 2, -1), (8, 3, 1), (12, 3, -1), (8, 5, 1)])
+def test_intalv_pm_full_space(spec, q, ipm):
+    """Full-space S^{+/-}(q) (parity_type != 0) against the oracle, for the S=1/2 tree and
+    spin S > 1/2 with ladder factors; for spin 1 also the DSF Lanczos that follows."""
+    _say("intalv_pm full space", spec, q, ipm)
+    cl = lattice.get_cluster(spec)
+    op = model.build_operator(cl, 20, 108.0, q=q) if q > 2 else model.build_operator(cl, 20, 54.0, field=0.1)
+    ctx = gpu.Context(op)
+    x = _rand(ctx.dim, 51)
+    xv, w, v0 = ctx.vector(), ctx.vector(), ctx.vector()
+    xv.upload(x)
+    qq = cl.q_list[min(1, len(cl.q_list) - 1)]
+    cn = ctx.intalv_xy(xv, w, qq, cl.site_vec, ipm)
+    v_ref, cn_ref = oracle.intalv_pm(cl.n, q, cl.site_vec, qq, ipm, x)
+    assert abs(cn - cn_ref) <= 1e-13 * cn_ref
+    assert np.linalg.norm(w.download() - v_ref) <= 1e-13
+    if q == 3:
+        sa, sb = ctx.dsflnc1z(w, v0, cn, 30)
+        sa_ref, sb_ref = oracle.dsf_lanczos(oracle.OracleOp(op), v_ref, cn_ref, 30)
+        assert np.allclose(sa[:20], sa_ref[:20], rtol=1e-11)
+        assert np.allclose(sb[:20], sb_ref[:20], rtol=1e-11)
+
+
+def test_intalv_xy_sector_lowering_matches_full_space():
+    """S=1/2 sectors with ipm = -1 (k -> k-1) equal the full-space bit-clearing sum of the
+    embedded vector, restricted to the target sector."""
+    _say("intalv_xy sector ipm=-1")
+    cl = lattice.get_cluster(16)
+    op = model.build_operator(cl, 20, 0.0)
+    src, dst = gpu.Context(op), gpu.Context(op)
+    src.sz_sector(8)
+    dst.sz_sector(7)
+    x = _rand(src.dim, 61)
+    xv, w = src.vector(), dst.vector()
+    xv.upload(x)
+    qq = cl.q_list[2]
+    cn = dst.intalv_xy(xv, w, qq, cl.site_vec, -1)
+    s_src, _, _ = oracle.sector_tables(cl.n, 8)
+    s_dst, _, _ = oracle.sector_tables(cl.n, 7)
+    xf = np.zeros(1 << cl.n, complex)
+    xf[s_src.astype(np.int64)] = x
+    vf, cnf = oracle.intalv_pm(cl.n, 2, cl.site_vec, qq, -1, xf)
+    assert abs(cn - cnf) <= 1e-13 * cnf
+    assert np.linalg.norm(w.download() - vf[s_dst.astype(np.int64)]) <= 1e-13
+
+
 def test_sector_lanczos_ground_state():
     """The Heisenberg ground state of the 16-site cluster lies in the S^z = 0 sector:
     Lanczos in that sector reaches the oracle's full-space ground energy."""

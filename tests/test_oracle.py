@@ -347,6 +347,58 @@ def test_sector_hv_is_the_projected_dense_operator(spec, k):
     assert np.all(np.min(np.abs(w_full[None, :] - w_sec[:, None]), axis=1) < 1e-9)
 
 
+@pytest.mark.parametrize("spec,k", [(8, 3), (8, 4), (8, 0), (12, 5), ("2x3", 6)])
+def test_intalv_xy_is_the_dense_bit_setting_sum(spec, k):
+    """intalv_xy equals sum_j e^{i q.r_j} |1><0|_j (the reference's ipm = +1: the bit of
+    site j goes from 0 to 1) built from dense single-site operators on the full space,
+    restricted to the sector with k + 1 set bits and normalised; cnorm = ||.||^2 / N."""
+    cl = lattice.get_cluster(spec)
+    n = cl.n
+    src, _, _ = oracle.sector_tables(n, k)
+    dst, _, _ = oracle.sector_tables(n, k + 1)
+    rng = np.random.default_rng(7 + k)
+    x = rng.standard_normal(len(src)) + 1j * rng.standard_normal(len(src))
+    xf = np.zeros(1 << n, complex)
+    xf[src.astype(np.int64)] = x
+    q = (0.7, -1.3)
+    raise_bit = np.array([[0, 0], [1, 0]], dtype=complex)   # |1><0| in the bit basis
+    vf = np.zeros(1 << n, complex)
+    for j in range(n):
+        ph = np.exp(1j * (q[0] * cl.site_vec[j][0] + q[1] * cl.site_vec[j][1]))
+        vf += ph * (dense.embed(raise_bit, j, n, 2) @ xf)
+    outside = np.ones(1 << n, bool)
+    outside[dst.astype(np.int64)] = False
+    assert np.all(vf[outside] == 0)
+    want = vf[dst.astype(np.int64)]
+    v, cn = oracle.intalv_xy(n, cl.site_vec, q, k, x)
+    assert abs(cn - np.vdot(want, want).real / n) <= 1e-13 * cn
+    assert np.linalg.norm(v - want / np.linalg.norm(want)) <= 1e-13
+
+
+@pytest.mark.parametrize("spec,q,ipm", [(8, 3, 1), (8, 3, -1), (8, 4, 1), (8, 5, -1), (12, 2, 1), (12, 2, -1)])
+def test_intalv_pm_full_space_is_the_dense_ladder_sum(spec, q, ipm):
+    """Full-space intalv_xy equals sum_j e^{i q.r_j} S^{+/-}_j from the textbook spin-S
+    ladder matrices (digit basis m = d - S); for q = 2 the S=1/2 tree's bit set / clear."""
+    cl = lattice.get_cluster(spec) if isinstance(spec, str) else lattice.reference_cluster(spec)
+    n = cl.n
+    rng = np.random.default_rng(3 + q)
+    x = rng.standard_normal(q ** n) + 1j * rng.standard_normal(q ** n)
+    if q == 2:
+        step = np.array([[0, 0], [1, 0]], dtype=complex) if ipm == 1 else np.array([[0, 1], [0, 0]], dtype=complex)
+    else:
+        sx, sy, _ = dense.site_ops(q)
+        splus = sx + 1j * sy            # raises the digit: sp[d+1, d] = sqrt((S-m)(S+m+1))
+        step = splus if ipm == 1 else splus.conj().T
+    qv = (0.4, 1.1)
+    want = np.zeros_like(x)
+    for j in range(n):
+        ph = np.exp(1j * (qv[0] * cl.site_vec[j][0] + qv[1] * cl.site_vec[j][1]))
+        want += ph * (dense.embed(step, j, n, q) @ x)
+    v, cn = oracle.intalv_pm(n, q, cl.site_vec, qv, ipm, x)
+    assert abs(cn - np.vdot(want, want).real / n) <= 1e-12 * cn
+    assert np.linalg.norm(v - want / np.linalg.norm(want)) <= 1e-12
+
+
 def test_sector_hv_rejects_non_conserving_operator():
     cl = lattice.reference_cluster(8)
     op = model.build_operator(cl, 20, 54.0)   # Kitaev terms flip two aligned spins
