@@ -295,6 +295,32 @@ int pned_read_vectors(const char *path, int *indx, int nvec, pned_vec *const *ve
 /* readwrite_vec_data (S_one_half/Hybrid_TPS_output.f90:12-36): e, then the vector */
 int pned_basis_write(const char *path, double e, const pned_vec *x);
 int pned_basis_read(const char *path, double *e, pned_vec *x);
+/* interval_dump / interval_dump_read (S_one_half/Hybrid_TPS_output.f90:50-91, 147-201): the
+ * checkpoint vecdump_me*: data_size, vec_size, then one record per given real array (the
+ * reference's optional sa, sb, sc, se in that order, real_counts doubles each) and per
+ * given vector (its optional vec0 .. vec4, streamed from / to device memory). */
+int pned_interval_dump(const char *path, int32_t data_size, int32_t vec_size, int nreal,
+                       const double *const *reals, const int64_t *real_counts, int nvec,
+                       const pned_vec *const *vecs);
+int pned_interval_dump_read(const char *path, int32_t *data_size, int32_t *vec_size, int nreal,
+                            double *const *reals, const int64_t *real_counts, int nvec,
+                            pned_vec *const *vecs);
+/* the checkpoint's .info file (interval_dump / check_interval_dump, Hybrid_TPS_output.f90:
+ * 93-146, 160-176): DUMP_NAME (character(128), blank padded), np, position(npos) (logical:
+ * written as true_value -- 1 for gfortran/flang, -1 for Intel Fortran's default -- read as
+ * non-zero), max_itr, dsf_comp (character(2)), s_maxitr, n, ibond, nvec, total_sz,
+ * itr_counter(12).  read: dump_name gets the trimmed name (>= 129 bytes), dsf_comp 3 bytes. */
+int pned_dump_info_write(const char *path, const char *dump_name, int32_t np, int npos, const int32_t *position,
+                         int32_t true_value, int32_t max_itr, const char *dsf_comp, int32_t s_maxitr,
+                         int32_t n, int32_t ibond, int32_t nvec, int32_t total_sz, const int32_t *itr_counter);
+int pned_dump_info_read(const char *path, char *dump_name, int32_t *np, int npos, int32_t *position,
+                        int32_t *max_itr, char *dsf_comp, int32_t *s_maxitr, int32_t *n, int32_t *ibond,
+                        int32_t *nvec, int32_t *total_sz, int32_t *itr_counter);
+/* write_dsf_res (Hybrid_TPS_output.f90:243-262): one momentum's block of dsf_res_<comp>.dat*
+ * text -- '#' line with sq (F25.18), ll rows (2F18.10, 2F25.16, I5) of q, w, dsf, conv, four
+ * list-directed blank lines; append = 1 adds to the file (the reference keeps it open). */
+int pned_write_dsf_res(const char *path, int append, double sq, const double q[2], int ll, const double *w,
+                       const double *dsf, const int32_t *conv);
 
 #ifdef __cplusplus
 }

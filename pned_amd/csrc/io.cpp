@@ -13,9 +13,17 @@
 //   SalphaSbeta_q*me*<comp>     DSF.f90:353-372   sq, salpha, sbeta
 //   vec_q*me*<comp>, vec_me*    DSF.f90:374-379   indx, v1, v2 [, v3]
 //   basisme*                    output.f90:12-36  e, x
+//   vecdump_me* (interval_dump) output.f90:147-201 data_size, vec_size, sa, sb, sc, se,
+//                                                  vec0..vec4 (the ones a caller dumps)
+//   vecdump_me*.info            output.f90:160-176 DUMP_NAME, np, position, max_itr,
+//                                                  dsf_comp, s_maxitr, n, ibond, nvec,
+//                                                  total_sz, itr_counter(1..12)
+//   dsf_res_<comp>.dat*         output.f90:243-262 text: sq, then q, w, dsf, conv rows
 // Device vectors are streamed through a pinned host buffer, never held whole on the host.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -296,6 +304,140 @@ int pned_basis_read(const char *path, double *e, pned_vec *x)
     PNED_TRY(open_file(ff, path, "rb"));
     PNED_TRY(read_host(ff.f, e, sizeof(double)));
     return read_vec(ff.f, x);
+}
+
+int pned_interval_dump(const char *path, int32_t data_size, int32_t vec_size, int nreal, const double *const *reals,
+                       const int64_t *real_counts, int nvec, const pned_vec *const *vecs)
+{
+    if (nreal < 0 || nvec < 0 || (nreal > 0 && (!reals || !real_counts)) || (nvec > 0 && !vecs))
+        return fail(PNED_E_ARG, "bad argument");
+    for (int k = 0; k < nreal; ++k)
+        if (!reals[k] || real_counts[k] < 0) return fail(PNED_E_ARG, "bad real array");
+    for (int k = 0; k < nvec; ++k)
+        if (!vecs[k]) return fail(PNED_E_ARG, "null vector");
+    File ff;
+    PNED_TRY(open_file(ff, path, "wb"));
+    PNED_TRY(write_host(ff.f, &data_size, 4));
+    PNED_TRY(write_host(ff.f, &vec_size, 4));
+    for (int k = 0; k < nreal; ++k) PNED_TRY(write_host(ff.f, reals[k], sizeof(double) * (size_t)real_counts[k]));
+    for (int k = 0; k < nvec; ++k) PNED_TRY(write_vec(ff.f, vecs[k]));
+    return PNED_OK;
+}
+
+int pned_interval_dump_read(const char *path, int32_t *data_size, int32_t *vec_size, int nreal, double *const *reals,
+                            const int64_t *real_counts, int nvec, pned_vec *const *vecs)
+{
+    if (!data_size || !vec_size || nreal < 0 || nvec < 0 || (nreal > 0 && (!reals || !real_counts)) ||
+        (nvec > 0 && !vecs))
+        return fail(PNED_E_ARG, "bad argument");
+    for (int k = 0; k < nreal; ++k)
+        if (!reals[k] || real_counts[k] < 0) return fail(PNED_E_ARG, "bad real array");
+    for (int k = 0; k < nvec; ++k)
+        if (!vecs[k]) return fail(PNED_E_ARG, "null vector");
+    File ff;
+    PNED_TRY(open_file(ff, path, "rb"));
+    PNED_TRY(read_host(ff.f, data_size, 4));
+    PNED_TRY(read_host(ff.f, vec_size, 4));
+    for (int k = 0; k < nreal; ++k) PNED_TRY(read_host(ff.f, reals[k], sizeof(double) * (size_t)real_counts[k]));
+    for (int k = 0; k < nvec; ++k) PNED_TRY(read_vec(ff.f, vecs[k]));
+    return PNED_OK;
+}
+
+int pned_dump_info_write(const char *path, const char *dump_name, int32_t np, int npos, const int32_t *position,
+                         int32_t true_value, int32_t max_itr, const char *dsf_comp, int32_t s_maxitr, int32_t n,
+                         int32_t ibond, int32_t nvec, int32_t total_sz, const int32_t *itr_counter)
+{
+    if (!dump_name || npos < 0 || (npos > 0 && !position) || !dsf_comp || !itr_counter)
+        return fail(PNED_E_ARG, "bad argument");
+    char name[128], comp[2];
+    std::memset(name, ' ', sizeof(name));                 // character(len=128), blank padded
+    std::memcpy(name, dump_name, std::min(std::strlen(dump_name), sizeof(name)));
+    std::memset(comp, ' ', sizeof(comp));
+    std::memcpy(comp, dsf_comp, std::min(std::strlen(dsf_comp), sizeof(comp)));
+    std::vector<int32_t> pos((size_t)npos);
+    for (int k = 0; k < npos; ++k) pos[k] = position[k] ? true_value : 0;   // logical (default kind)
+    File ff;
+    PNED_TRY(open_file(ff, path, "wb"));
+    PNED_TRY(write_host(ff.f, name, sizeof(name)));
+    PNED_TRY(write_host(ff.f, &np, 4));
+    PNED_TRY(write_host(ff.f, pos.data(), 4 * (size_t)npos));
+    PNED_TRY(write_host(ff.f, &max_itr, 4));
+    PNED_TRY(write_host(ff.f, comp, sizeof(comp)));
+    for (int32_t v : {s_maxitr, n, ibond, nvec, total_sz}) PNED_TRY(write_host(ff.f, &v, 4));
+    for (int i = 0; i < 12; ++i) PNED_TRY(write_host(ff.f, &itr_counter[i], 4));
+    return PNED_OK;
+}
+
+int pned_dump_info_read(const char *path, char *dump_name, int32_t *np, int npos, int32_t *position,
+                        int32_t *max_itr, char *dsf_comp, int32_t *s_maxitr, int32_t *n, int32_t *ibond,
+                        int32_t *nvec, int32_t *total_sz, int32_t *itr_counter)
+{
+    if (!dump_name || !np || npos < 0 || (npos > 0 && !position) || !max_itr || !dsf_comp || !s_maxitr || !n ||
+        !ibond || !nvec || !total_sz || !itr_counter)
+        return fail(PNED_E_ARG, "bad argument");
+    char name[128], comp[2];
+    std::vector<int32_t> pos((size_t)npos);
+    File ff;
+    PNED_TRY(open_file(ff, path, "rb"));
+    PNED_TRY(read_host(ff.f, name, sizeof(name)));
+    PNED_TRY(read_host(ff.f, np, 4));
+    PNED_TRY(read_host(ff.f, pos.data(), 4 * (size_t)npos));
+    PNED_TRY(read_host(ff.f, max_itr, 4));
+    PNED_TRY(read_host(ff.f, comp, sizeof(comp)));
+    for (int32_t *v : {s_maxitr, n, ibond, nvec, total_sz}) PNED_TRY(read_host(ff.f, v, 4));
+    for (int i = 0; i < 12; ++i) PNED_TRY(read_host(ff.f, &itr_counter[i], 4));
+    int len = (int)sizeof(name);
+    while (len > 0 && name[len - 1] == ' ') --len;        // trim(DUMP_NAME)
+    std::memcpy(dump_name, name, (size_t)len);
+    dump_name[len] = '\0';
+    dsf_comp[0] = comp[0];
+    dsf_comp[1] = comp[1];
+    dsf_comp[2] = '\0';
+    for (int k = 0; k < npos; ++k) position[k] = pos[k] != 0;   // any non-zero value is .true.
+    return PNED_OK;
+}
+
+// Fortran F<w>.<d> / I<w> edit: right-justified in w columns, asterisks when it does not fit
+static void put_f(std::string &out, double v, int w, int d)
+{
+    char buf[512];
+    const int len = std::snprintf(buf, sizeof(buf), "%*.*f", w, d, v);
+    if (len < 0 || len > w)
+        out.append((size_t)w, '*');
+    else
+        out.append(buf, (size_t)len);
+}
+
+static void put_i(std::string &out, long v, int w)
+{
+    char buf[64];
+    const int len = std::snprintf(buf, sizeof(buf), "%*ld", w, v);
+    if (len < 0 || len > w)
+        out.append((size_t)w, '*');
+    else
+        out.append(buf, (size_t)len);
+}
+
+int pned_write_dsf_res(const char *path, int append, double sq, const double q[2], int ll, const double *w,
+                       const double *dsf, const int32_t *conv)
+{
+    if (!q || ll < 0 || (ll > 0 && (!w || !dsf || !conv))) return fail(PNED_E_ARG, "bad argument");
+    std::string out = "#        ";
+    put_f(out, sq, 25, 18);
+    out += '\n';
+    for (int k = 0; k < ll; ++k) {
+        put_f(out, q[0], 18, 10);
+        put_f(out, q[1], 18, 10);
+        put_f(out, w[k], 25, 16);
+        put_f(out, dsf[k], 25, 16);
+        put_i(out, conv[k], 5);
+        out += '\n';
+    }
+    for (int k = 0; k < 4; ++k) out += " \n";           // write(j,*) '' four times
+    File ff;
+    PNED_TRY(open_file(ff, path, append ? "ab" : "wb"));
+    if (fwrite(out.data(), 1, out.size(), ff.f) != out.size()) return fail(PNED_E_ARG, "write failed");
+    return PNED_OK;
 }
 
 }  // extern "C"

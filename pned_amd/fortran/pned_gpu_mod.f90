@@ -252,6 +252,42 @@ module pned_gpu
       type(c_ptr), intent(in) :: vecs(*)
       integer(c_int) :: rc
     end function
+    function pned_interval_dump(path, data_size, vec_size, nreal, reals, real_counts, nvec, vecs) &
+        bind(C, name="pned_interval_dump") result(rc)
+      import :: c_int, c_ptr, c_char, c_int64_t
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: data_size, vec_size, nreal, nvec
+      type(c_ptr), intent(in) :: reals(*), vecs(*)
+      integer(c_int64_t), intent(in) :: real_counts(*)
+      integer(c_int) :: rc
+    end function
+    function pned_interval_dump_read(path, data_size, vec_size, nreal, reals, real_counts, nvec, vecs) &
+        bind(C, name="pned_interval_dump_read") result(rc)
+      import :: c_int, c_ptr, c_char, c_int64_t
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), intent(out) :: data_size, vec_size
+      integer(c_int), value :: nreal, nvec
+      type(c_ptr), intent(in) :: reals(*), vecs(*)
+      integer(c_int64_t), intent(in) :: real_counts(*)
+      integer(c_int) :: rc
+    end function
+    function pned_dump_info_write(path, dump_name, np, npos, position, true_value, max_itr, dsf_comp, s_maxitr, &
+                                  n, ibond, nvec, total_sz, itr_counter) bind(C, name="pned_dump_info_write") result(rc)
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: path(*), dump_name(*), dsf_comp(*)
+      integer(c_int), value :: np, npos, true_value, max_itr, s_maxitr, n, ibond, nvec, total_sz
+      integer(c_int), intent(in) :: position(*), itr_counter(12)
+      integer(c_int) :: rc
+    end function
+    function pned_write_dsf_res(path, append, sq, q, ll, w, dsf, conv) bind(C, name="pned_write_dsf_res") result(rc)
+      import :: c_int, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: append, ll
+      real(c_double), value :: sq
+      real(c_double), intent(in) :: q(2), w(*), dsf(*)
+      integer(c_int), intent(in) :: conv(*)
+      integer(c_int) :: rc
+    end function
     function pned_basis_write(path, e, x) bind(C, name="pned_basis_write") result(rc)
       import :: c_int, c_ptr, c_double, c_char
       character(kind=c_char), intent(in) :: path(*)

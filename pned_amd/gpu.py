@@ -64,6 +64,16 @@ def lib():
     L.pned_read_vectors.argtypes = [C.c_char_p, C.POINTER(i32), i32, C.POINTER(P)]
     L.pned_basis_write.argtypes = [C.c_char_p, C.c_double, P]
     L.pned_basis_read.argtypes = [C.c_char_p, C.POINTER(C.c_double), P]
+    L.pned_interval_dump.argtypes = [C.c_char_p, i32, i32, i32, C.POINTER(C.c_void_p), C.POINTER(C.c_int64), i32,
+                                     C.POINTER(P)]
+    L.pned_interval_dump_read.argtypes = [C.c_char_p, C.POINTER(i32), C.POINTER(i32), i32, C.POINTER(C.c_void_p),
+                                          C.POINTER(C.c_int64), i32, C.POINTER(P)]
+    L.pned_dump_info_write.argtypes = [C.c_char_p, C.c_char_p, i32, i32, _ip, i32, i32, C.c_char_p, i32, i32, i32,
+                                       i32, i32, _ip]
+    L.pned_dump_info_read.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(i32), i32, _ip, C.POINTER(i32), C.c_char_p,
+                                      C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                      C.POINTER(i32), _ip]
+    L.pned_write_dsf_res.argtypes = [C.c_char_p, i32, f64, _dp, i32, _dp, _dp, _ip]
     L.pned_ctpq_eval.argtypes = [i32, i32, i32, _dp, _dp, _dp, C.c_double, C.c_double, i32, _dp]
     L.pned_ctx_autotune.argtypes = [P, P, P, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                                     C.POINTER(f64)]
@@ -251,6 +261,27 @@ class Context:
         e = C.c_double()
         check(lib().pned_basis_read(str(path).encode(), C.byref(e), x.h))
         return e.value
+
+    def interval_dump(self, path, data_size: int, vec_size: int, reals=(), vecs=()):
+        """Checkpoint vecdump_me* (interval_dump): data_size, vec_size, the given real
+        arrays (sa, sb, sc, se that the caller dumps, in that order), the given vectors."""
+        arrs = [np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, order="F")) for a in reals]
+        ptrs = (C.c_void_p * max(len(arrs), 1))(*[a.ctypes.data for a in arrs])
+        cnts = (C.c_int64 * max(len(arrs), 1))(*[a.size for a in arrs])
+        vv = (C.c_void_p * max(len(vecs), 1))(*[v.h for v in vecs])
+        check(lib().pned_interval_dump(str(path).encode(), int(data_size), int(vec_size), len(arrs), ptrs, cnts,
+                                       len(vecs), vv))
+
+    def interval_dump_read(self, path, real_counts=(), vecs=()):
+        """interval_dump_read: returns (data_size, vec_size, [real arrays]); vectors filled."""
+        arrs = [np.zeros(int(c)) for c in real_counts]
+        ptrs = (C.c_void_p * max(len(arrs), 1))(*[a.ctypes.data for a in arrs])
+        cnts = (C.c_int64 * max(len(arrs), 1))(*[a.size for a in arrs])
+        vv = (C.c_void_p * max(len(vecs), 1))(*[v.h for v in vecs])
+        ds, vs = C.c_int(), C.c_int()
+        check(lib().pned_interval_dump_read(str(path).encode(), C.byref(ds), C.byref(vs), len(arrs), ptrs, cnts,
+                                            len(vecs), vv))
+        return ds.value, vs.value, arrs
 
     def sz_sector(self, n_down: int):
         """Restrict to the S^z sector with n_down set bits (n_down < 0: full space);
@@ -523,6 +554,39 @@ def read_dsf_alpha_beta(path, n_alpha, n_beta):
     a, b = np.zeros(n_alpha), np.zeros(n_beta)
     check(lib().pned_read_dsf_alpha_beta(str(path).encode(), s, n_alpha, a, n_beta, b))
     return complex(s[0], s[1]), a, b
+
+
+def dump_info_write(path, dump_name, np_, position, max_itr, dsf_comp, s_maxitr, n, ibond, nvec, total_sz,
+                    itr_counter, true_value=1):
+    """The checkpoint's .info file (interval_dump); logicals written as true_value."""
+    pos = np.ascontiguousarray(np.asarray(position, dtype=np.int32))
+    itc = np.ascontiguousarray(np.asarray(itr_counter, dtype=np.int32))
+    assert itc.size == 12
+    check(lib().pned_dump_info_write(str(path).encode(), dump_name.encode(), np_, pos.size, pos, true_value, max_itr,
+                                     dsf_comp.encode(), s_maxitr, n, ibond, nvec, total_sz, itc))
+
+
+def dump_info_read(path, npos=7):
+    """check_interval_dump's read of the .info file: a dict of its fields."""
+    name = C.create_string_buffer(129)
+    comp = C.create_string_buffer(3)
+    pos = np.zeros(npos, dtype=np.int32)
+    itc = np.zeros(12, dtype=np.int32)
+    v = [C.c_int() for _ in range(7)]
+    check(lib().pned_dump_info_read(str(path).encode(), name, C.byref(v[0]), npos, pos, C.byref(v[1]), comp,
+                                    C.byref(v[2]), C.byref(v[3]), C.byref(v[4]), C.byref(v[5]), C.byref(v[6]), itc))
+    return {"dump_name": name.value.decode(), "np": v[0].value, "position": [bool(p) for p in pos],
+            "max_itr": v[1].value, "dsf_comp": comp.value.decode(), "s_maxitr": v[2].value, "n": v[3].value,
+            "ibond": v[4].value, "nvec": v[5].value, "total_sz": v[6].value, "itr_counter": itc.tolist()}
+
+
+def write_dsf_res(path, sq, q, w, dsf, conv, append=True):
+    """One momentum's block of the dsf_res_<comp>.dat* text file (write_dsf_res)."""
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    d = np.ascontiguousarray(dsf, dtype=np.float64)
+    c = np.ascontiguousarray(conv, dtype=np.int32)
+    check(lib().pned_write_dsf_res(str(path).encode(), 1 if append else 0, float(sq), np.asarray(q, dtype=float),
+                                   len(w), w, d, c))
 
 
 def bisec(alpha, beta, ne, eps=1e-25):

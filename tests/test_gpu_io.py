@@ -83,3 +83,26 @@ def test_fortran_reads_gpu_vector_file(tmp_path):
     v = [float(x) for x in out[1:]]
     xa, xb = a.download(), b.download()
     assert v[0] == xa[0].real and v[1] == xa[0].imag and v[2] == xb[-1].real and v[3] == xb[-1].imag
+
+
+def test_interval_dump_round_trip(tmp_path):
+    """Checkpoint as the TPS restart writes it (sa, sb, sc, se, vec1, vec2, vec3) and read
+    back into fresh device vectors; the real arrays keep Fortran order."""
+    _say("interval_dump round trip")
+    op = model.build_operator(lattice.reference_cluster(12), 20, 54.0)
+    ctx = gpu.Context(op)
+    vs = [ctx.vector() for _ in range(3)]
+    for k, v in enumerate(vs):
+        ctx.set_initial_vec(v, 10 + k)
+    want = [v.download() for v in vs]
+    r = np.random.default_rng(4)
+    sa, sb, sc, se = r.standard_normal(20), r.standard_normal(20), r.standard_normal((20, 9)), r.standard_normal(8)
+    f = tmp_path / "vecdump_me00000"
+    ctx.interval_dump(f, ctx.dim, 20, reals=[sa, sb, sc, se], vecs=vs)
+    back = [ctx.vector() for _ in range(3)]
+    ds, vsz, arrs = ctx.interval_dump_read(f, real_counts=[20, 20, 180, 8], vecs=back)
+    assert (ds, vsz) == (ctx.dim, 20)
+    assert np.array_equal(arrs[0], sa) and np.array_equal(arrs[1], sb) and np.array_equal(arrs[3], se)
+    assert np.array_equal(arrs[2].reshape(9, 20).T, sc)
+    for v, w in zip(back, want):
+        assert np.array_equal(v.download(), w)

@@ -207,3 +207,152 @@ def test_fortran_module_bindings(tmp_path):
     assert np.array_equal(a2, alpha) and np.array_equal(b2, beta) and np.array_equal(c2, coef)
     want = gpu.ctpq(alpha[None, :], beta[None, :], coef.T[None, :, :], 0.5, 4.0, 4)
     np.testing.assert_allclose(got, want, rtol=1e-15, atol=0)
+
+
+FORTRAN_DSF_RES = """
+program res
+  implicit none
+  double precision :: q(2), w(3), dsf(3), sq
+  integer :: conv(3), k, i, j
+  q = (/ 0.5d0, -1.25d0 /)
+  w = (/ 0.0d0, 2.5d-3, 4.0d0 /)
+  dsf = (/ 1.5d0, -0.0d0, 123456789.25d0 /)
+  conv = (/ 1, 0, 1 /)
+  sq = 0.1234567890123456789d0
+  j = 100
+  open(j, file='dsf_res_zz.datcl00000lme00000', status='unknown')
+  do i = 1, 2
+    write(j,'(A,F25.18)') '#        ', sq
+    do k = 1, 3
+      write(j,'(2F18.10,2F25.16,I5)') q(1), q(2), w(k), dsf(k), conv(k)
+    end do
+    write(j,*) ''
+    write(j,*) ''
+    write(j,*) ''
+    write(j,*) ''
+    q(1) = q(1) * 2d0
+  end do
+  close(j)
+end program res
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(FC), reason="amdflang not available")
+def test_dsf_res_text_matches_fortran(tmp_path):
+    """write_dsf_res's text, byte for byte as a Fortran program with the reference's format
+    statements writes it (two momenta appended)."""
+    (tmp_path / "res.f90").write_text(FORTRAN_DSF_RES)
+    subprocess.run([FC, "res.f90", "-o", "res"], cwd=tmp_path, check=True, capture_output=True)
+    subprocess.run(["./res"], cwd=tmp_path, check=True, capture_output=True)
+    want = (tmp_path / "dsf_res_zz.datcl00000lme00000").read_bytes()
+    f = tmp_path / "mine"
+    w, d, c = [0.0, 2.5e-3, 4.0], [1.5, -0.0, 123456789.25], [1, 0, 1]
+    gpu.write_dsf_res(f, 0.1234567890123456789, (0.5, -1.25), w, d, c, append=False)
+    gpu.write_dsf_res(f, 0.1234567890123456789, (1.0, -1.25), w, d, c, append=True)
+    assert f.read_bytes() == want
+    assert b"*" * 25 in want   # a value wider than F25.16 prints as asterisks in both
+
+
+FORTRAN_INFO = """
+program info
+  implicit none
+  character(len=128) :: dump_name
+  integer :: t_np, t_max_itr, t_s_maxitr, t_n, t_ibond, t_nvec, t_total_sz, i, k
+  character(len=2) :: t_dsf_comp
+  logical :: position(7)
+  integer :: itr(12)
+  double precision :: sa(5), sb(5), sc(5,3), se(8)
+  integer :: data_size, vec_size
+  open(11, file='vecdump_me00000.info', form='unformatted', access='sequential', status='old')
+  read(11) dump_name
+  read(11) t_np
+  read(11) position
+  read(11) t_max_itr
+  read(11) t_dsf_comp
+  read(11) t_s_maxitr
+  read(11) t_n
+  read(11) t_ibond
+  read(11) t_nvec
+  read(11) t_total_sz
+  do i = 1, 12
+    read(11) k
+    itr(i) = k
+  end do
+  close(11)
+  write(*,'(A)') trim(dump_name)
+  write(*,'(8I8)') t_np, t_max_itr, t_s_maxitr, t_n, t_ibond, t_nvec, t_total_sz, count(position)
+  write(*,'(7L2)') position
+  write(*,'(A)') t_dsf_comp
+  write(*,'(12I6)') itr
+  open(12, file='vecdump_me00000', form='unformatted', access='sequential', status='old')
+  read(12) data_size
+  read(12) vec_size
+  read(12) sa
+  read(12) sb
+  read(12) sc
+  read(12) se
+  close(12)
+  write(*,'(2I8)') data_size, vec_size
+  write(*,'(4ES25.17)') sa(2), sb(5), sc(4,2), se(8)
+  ! the same statements interval_dump uses, for the library to read back
+  position = (/ .false., .true., .false., .true., .false., .false., .true. /)
+  open(13, file='fortran_dump.info', form='unformatted', access='sequential', status='unknown')
+  write(13) dump_name
+  write(13) 2
+  write(13) position
+  write(13) 300
+  write(13) 'xx'
+  write(13) 150
+  write(13) 24
+  write(13) 36
+  write(13) 8
+  write(13) 12
+  do i = 1, 12
+    write(13) 10 * i
+  end do
+  close(13)
+end program info
+"""
+
+
+@pytest.mark.skipif(not os.path.exists(FC), reason="amdflang not available")
+def test_interval_dump_files_against_fortran(tmp_path):
+    """The checkpoint's .info and dump files: the library writes, a Fortran program reads
+    them with check_interval_dump's and interval_dump_read's statements; and the library
+    reads the .info that interval_dump's write statements produce."""
+    pos = [1, 0, 0, 1, 1, 0, 0]
+    itc = list(range(101, 113))
+    gpu.dump_info_write(tmp_path / "vecdump_me00000.info", "vecdump_me00000", 1, pos, 1024, "zz", 300, 32, 48, 8,
+                        16, itc, true_value=1)
+    r = np.random.default_rng(9)
+    sa, sb, sc, se = r.standard_normal(5), r.standard_normal(5), r.standard_normal((5, 3)), r.standard_normal(8)
+    # no device vectors in this CPU test (their records are tested on the GPU)
+    lib = gpu.lib()
+    arrs = [np.ascontiguousarray(a.reshape(-1, order="F")) for a in (sa, sb, sc, se)]
+    ptrs = (gpu.C.c_void_p * 4)(*[a.ctypes.data for a in arrs])
+    cnts = (gpu.C.c_int64 * 4)(*[a.size for a in arrs])
+    gpu.check(lib.pned_interval_dump(str(tmp_path / "vecdump_me00000").encode(), 4096, 300, 4, ptrs, cnts, 0,
+                                     (gpu.C.c_void_p * 1)()))
+    (tmp_path / "info.f90").write_text(FORTRAN_INFO)
+    subprocess.run([FC, "info.f90", "-o", "info"], cwd=tmp_path, check=True, capture_output=True)
+    out = subprocess.run(["./info"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout.splitlines()
+    assert out[0] == "vecdump_me00000"
+    assert [int(x) for x in out[1].split()] == [1, 1024, 300, 32, 48, 8, 16, 3]
+    assert out[2].split() == ["T", "F", "F", "T", "T", "F", "F"]
+    assert out[3] == "zz"
+    assert [int(x) for x in out[4].split()] == itc
+    assert [int(x) for x in out[5].split()] == [4096, 300]
+    got = [float(x) for x in out[6].split()]
+    assert got == [sa[1], sb[4], sc[3, 1], se[7]]
+    info = gpu.dump_info_read(tmp_path / "fortran_dump.info")
+    assert info == {"dump_name": "vecdump_me00000", "np": 2,
+                    "position": [False, True, False, True, False, False, True], "max_itr": 300,
+                    "dsf_comp": "xx", "s_maxitr": 150, "n": 24, "ibond": 36, "nvec": 8, "total_sz": 12,
+                    "itr_counter": [10 * i for i in range(1, 13)]}
+    # Intel Fortran's default .true. (-1) reads back as .true. as well
+    gpu.dump_info_write(tmp_path / "intel.info", "vecdump_me00003", 4, [1, 0, 1], 10, "zz", 5, 8, 12, 1, 4, itc,
+                        true_value=-1)
+    raw = (tmp_path / "intel.info").read_bytes()
+    pos_at = (4 + 128 + 4) + (4 + 4 + 4) + 4   # name record, np record, position marker
+    assert struct.unpack_from("<3i", raw, pos_at) == (-1, 0, -1)
+    assert gpu.dump_info_read(tmp_path / "intel.info", npos=3)["position"] == [True, False, True]
