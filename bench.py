@@ -283,7 +283,10 @@ def bench_spin1(args):
         sa, sb = ctx.dsflnc1z(w, v0, cn, steps)
         ww, dsf, conv = ctx.renbun(sa, sb, e0, eta, nomega, split, steps, sq=cn)
         dsf_rows.append({"q": [round(float(q[0]), 6), round(float(q[1]), 6)], "S_q": cn,
-                         "peak_omega": float(ww[int(np.argmax(dsf))]), "converged": int(conv.sum())})
+                         "peak_omega": float(ww[int(np.argmax(dsf))]),
+                         # conv[j]: the fraction depth at which omega_j converged (0: not within the steps)
+                         "omega_converged": int(np.count_nonzero(conv)),
+                         "mean_depth": round(float(conv[conv > 0].mean()), 2) if np.any(conv > 0) else None})
         cf_at[k] = (sa, sb, cn)
         _progress(f"DSF q={dsf_rows[-1]['q']}: S(q)={cn:.6f}, peak at omega={dsf_rows[-1]['peak_omega']:.4f}")
     tdsf = time.perf_counter() - t
