@@ -48,7 +48,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1 << 27)
-    ap.add_argument("--cfg", default="N32", help="N32 (default), N24, TPS, SERVE, LANCZOS, SPIN1")
+    ap.add_argument("--cfg", default="N32", help="N32 (default), N24, TPS, SERVE, LANCZOS, SPIN1, SPINHV")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="", help="fixed plan K,L,threads (skips the autotuner)")
     ap.add_argument("--requests", type=int, default=120, help="SERVE: number of requests")
@@ -96,6 +96,8 @@ def main():
 
     if args.cfg == "TPS":
         return bench_tps(args)
+    if args.cfg == "SPINHV":
+        return bench_spin_hv(args)
     if args.cfg == "LANCZOS":
         return bench_lanczos(args)
     if args.cfg == "SPIN1":
@@ -209,6 +211,38 @@ def _ground_state(ctx, gpu, v1, v0, x, seed, max_itr):
     dt = time.perf_counter() - t
     res = ctx.residual(x, ene[0], v0)
     return itr, float(ene[0]), res, dt
+
+
+def bench_spin_hv(args):
+    """The spin-S H.v alone on the spin-1 workload (reference N=18 cluster, 3^18 states,
+    MODEL 20 phi=108 deg, complex128): warmup + timed H.v (v0 += H v1), algorithmic bytes
+    3 * 16 * 3^18 per pass-equivalent stream set."""
+    import torch  # noqa: F401
+
+    from pned_amd import gpu, lattice, model
+
+    cl = lattice.reference_cluster(18)
+    op = model.build_operator(cl, 20, 108.0, q=3)
+    ctx = gpu.Context(op)
+    v1, v0 = ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 1)
+    ctx.hv_fresh(v1, v0)
+    for _ in range(args.warmup):
+        ctx.p_mltply_v3(v1, v0)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.p_mltply_v3(v1, v0)
+    ctx.sync()
+    ms = (time.perf_counter() - t) / args.steps * 1e3
+    npass = ctx.plan_info()[0]
+    alg = 3 * 16 * ctx.dim
+    print(json.dumps({"metric": "spin-1 N=18 H·v GB/s (algorithmic 48 B/state)", "value": round(alg / ms / 1e6, 2),
+                      "unit": "GB/s", "higher_is_better": True, "ms_per_step": round(ms, 3), "steps": args.steps,
+                      "warmup": args.warmup, "dtype": "c128",
+                      "config": {"workload": "spin-1 KH honeycomb N=18 reference cluster, phi=0.6pi, complex128 "
+                                             "dim 3^18", "dim": ctx.dim, "passes": npass,
+                                 "per_pass_GBps": round(alg * npass / ms / 1e6, 1)}}), flush=True)
 
 
 def bench_lanczos(args):
