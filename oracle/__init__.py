@@ -1,7 +1,8 @@
 """ctypes front of the CPU oracle (``oracle/pned_oracle.c``).
 
-TEST INFRASTRUCTURE ONLY: the checker that tests, the smoke entry and the bench's
-``cpu_baseline`` leg compare the HIP path against.  The product never imports it.
+TEST INFRASTRUCTURE ONLY: the reference restatement that the tests, the smoke entry and
+the bench's ``cpu_baseline`` leg compare the HIP path against.  The product never
+imports it.
 Each wrapper names the reference routine the C function restates.
 """
 from __future__ import annotations

@@ -1,8 +1,9 @@
 /*
  * pned_oracle.c -- CPU restatement of the reference's H.v and Krylov algorithms.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke entry
- * and the bench's cpu_baseline leg may load it.  The product path never calls it.
+ * TEST INFRASTRUCTURE ONLY.  The HIP path is compared against this file: only tests/,
+ * the smoke entry and the bench's cpu_baseline leg may load it.  The product path never
+ * calls it.
  *
  * Every function names the reference routine it follows (paths relative to the
  * reference root; H/ = S_one_half/Hybrid_TPS_*.f90, S/ = S_arbitrary/Hybrid_TPS9_*.f90).

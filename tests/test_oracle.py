@@ -1,4 +1,4 @@
-"""Pins of the CPU oracle (oracle/pned_oracle.c) before it is trusted as the checker.
+"""Pins of the CPU oracle (oracle/pned_oracle.c) before the GPU path is compared with it.
 
 What pins each oracle function (the reference ships no tests or golden vectors,
 so every pin is a fixture extracted from its text or a mathematical fact):
