@@ -506,6 +506,18 @@ int ensure_partial(pned_ctx *ctx, size_t nd)
     return PNED_OK;
 }
 
+static int hv_dispatch(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh, double *prdct);
+
+// v0 += H v1, then alpha = <v1|H v1> (the H.v's fused reduction, complex per column) and
+// beta^2 = |v0 - alpha v1|^2 with one readback: alpha stays on the device for the beta
+// kernel (the values it reads are the ones a host round trip would hand it)
+int hv_alpha_beta2(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double *pr, double *b2)
+{
+    PNED_TRY(ensure_result(ctx, 3 * (size_t)v1->nvec + 8));   // no reallocation between the two
+    PNED_TRY(hv_dispatch(ctx, v1, v0, 0.0, false, nullptr));
+    return beta2_after_hv(ctx, v1, v0, pr, b2);
+}
+
 int ensure_result(pned_ctx *ctx, size_t nd)
 {
     if (nd <= ctx->result_cap) return PNED_OK;
@@ -523,7 +535,7 @@ int ensure_result(pned_ctx *ctx, size_t nd)
 
 // small argument arrays for kernels: uploaded in stream order into a dedicated buffer
 // (each kernel that reads them is enqueued before the next upload overwrites them)
-int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr)
+int ensure_scalars(pned_ctx *ctx, int n, double **dptr)
 {
     if ((size_t)n > ctx->scalars_cap) {
         PNED_HIP(hipStreamSynchronize(ctx->stream));
@@ -533,8 +545,14 @@ int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr)
         PNED_HIP(hipMalloc((void **)&ctx->d_scalars, sizeof(double) * cap));
         ctx->scalars_cap = cap;
     }
-    PNED_HIP(hipMemcpyAsync(ctx->d_scalars, h, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     *dptr = ctx->d_scalars;
+    return PNED_OK;
+}
+
+int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr)
+{
+    PNED_TRY(ensure_scalars(ctx, n, dptr));
+    PNED_HIP(hipMemcpyAsync(ctx->d_scalars, h, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     return PNED_OK;
 }
 
@@ -1079,8 +1097,7 @@ int pned_calc_alpha_beta(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double *alph
     if (!ctx || !v1 || !v0 || !alpha || !beta) return fail(PNED_E_ARG, "null argument");
     const int nc = v1->nvec;
     std::vector<double> pr(2 * nc), b2(nc);
-    PNED_TRY(hv_dispatch(ctx, v1, v0, 0.0, false, pr.data()));
-    PNED_TRY(beta2(ctx, pr.data(), v1, v0, b2.data()));
+    PNED_TRY(hv_alpha_beta2(ctx, v1, v0, pr.data(), b2.data()));
     for (int c = 0; c < nc; ++c) {
         alpha[c] = pr[2 * c];
         beta[c] = std::sqrt(b2[c]);

@@ -548,6 +548,14 @@ __global__ __launch_bounds__(256) void reduce_partials(const double *__restrict_
     if (tid == 0) out[c] = accumulate ? out[c] + red[0] : red[0];
 }
 
+int reduce_rows_result(pned_ctx *ctx, int nrows, int ncol2)
+{
+    hipLaunchKernelGGL(reduce_partials, dim3(ncol2), dim3(256), 0, ctx->stream, ctx->d_partial, nrows, ncol2,
+                       ctx->d_result, 0);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
 template <typename Real, int TPB, int R>
 static int launch_pass(pned_ctx *ctx, const void *v1, void *v0, const DiagDev *ddiag, const PassDev *pd,
                        const PassArgs &a, double *partial, int grid)

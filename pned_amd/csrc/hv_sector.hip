@@ -158,13 +158,6 @@ int intalv_xy_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double 
     return PNED_OK;
 }
 
-__global__ void sector_reduce(const double *__restrict__ partial, int nrows, double *__restrict__ out)
-{
-    if (threadIdx.x >= 2) return;
-    double s = 0.0;
-    for (int b = 0; b < nrows; ++b) s += partial[2 * b + threadIdx.x];
-    out[threadIdx.x] = s;
-}
 
 void free_sector(pned_ctx *ctx)
 {
@@ -225,8 +218,7 @@ int launch_hv_sector(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double epe
                        (const SectorTerm *)ctx->d_sterms, ctx->sector_nterm, ctx->d_diag, (ctx->n + 1) / 2,
                        -(double)ctx->ibond * eperbd, fresh ? 1 : 0, dim, ctx->d_partial);
     PNED_HIP(hipGetLastError());
-    hipLaunchKernelGGL(sector_reduce, dim3(1), dim3(64), 0, ctx->stream, ctx->d_partial, grid, ctx->d_result);
-    PNED_HIP(hipGetLastError());
+    PNED_TRY(reduce_rows_result(ctx, grid, 2));
     if (prdct_host) {
         PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));

@@ -104,14 +104,6 @@ __global__ __launch_bounds__(256) void hv_spin_kernel(const double2 *__restrict_
     }
 }
 
-__global__ void reduce_two(const double *__restrict__ partial, int nrows, double *__restrict__ out)
-{
-    if (threadIdx.x >= 2) return;
-    double s = 0.0;
-    for (int b = 0; b < nrows; ++b) s += partial[2 * b + threadIdx.x];
-    out[threadIdx.x] = s;
-}
-
 // Builds (once per matrix set) the nonzero tables.  Stored in ctx->spin.
 static int build_spin_tables(pned_ctx *ctx, SpinTables &out)
 {
@@ -210,8 +202,7 @@ int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperb
         hipLaunchKernelGGL(hv_spin_kernel<0>, dim3(grid), dim3(256), 0, ctx->stream, (const double2 *)v1->data,
                            (double2 *)v0->data, tb, a, dim, ctx->d_partial);
     PNED_HIP(hipGetLastError());
-    hipLaunchKernelGGL(reduce_two, dim3(1), dim3(64), 0, ctx->stream, ctx->d_partial, grid, ctx->d_result);
-    PNED_HIP(hipGetLastError());
+    PNED_TRY(reduce_rows_result(ctx, grid, 2));
     if (prdct_host) {
         PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));
@@ -605,8 +596,7 @@ int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double
         PNED_HIP(hipGetLastError());
         rows += grid;
     }
-    hipLaunchKernelGGL(reduce_two, dim3(1), dim3(64), 0, ctx->stream, ctx->d_partial, rows, ctx->d_result);
-    PNED_HIP(hipGetLastError());
+    PNED_TRY(reduce_rows_result(ctx, rows, 2));
     if (prdct_host) {
         PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, 2 * sizeof(double), hipMemcpyDeviceToHost,
                                 ctx->stream));

@@ -191,6 +191,11 @@ namespace pned {
 int ensure_partial(pned_ctx *ctx, size_t ndoubles);
 int ensure_result(pned_ctx *ctx, size_t ndoubles);
 int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr);
+int ensure_scalars(pned_ctx *ctx, int n, double **dptr);
+// v0 += H v1; pr = <v1|H v1> (2 per column), b2 = |v0 - pr v1|^2: one host readback
+int hv_alpha_beta2(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double *pr, double *b2);
+// the beta^2 half of it: alpha taken from ctx->d_result where the H.v left it
+int beta2_after_hv(pned_ctx *ctx, const pned_vec *v1, const pned_vec *v0, double *pr, double *b2);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
 // S^z sector basis: H.v and table release
 int launch_hv_sector(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
@@ -206,6 +211,9 @@ int refresh_sector_terms(pned_ctx *ctx);   // after the bond matrices change
 // untuned tile defaults: true for vectors above 2^25 elements (dim x columns)
 bool default_large(const pned_ctx *ctx, int colbits);
 int upload_batch_coefficients(pned_ctx *ctx);
+// sums the rows of ctx->d_partial [nrows][ncol2] into ctx->d_result[0..ncol2): 256 threads
+// per column, fixed strides and a fixed tree (the same partials give the same sums)
+int reduce_rows_result(pned_ctx *ctx, int nrows, int ncol2);
 
 // kernels (launchers), all asynchronous on ctx->stream unless stated
 // d_colscale: optional device array of per-column factors s_c: v0 += (H - shift)(s_c v1_c);

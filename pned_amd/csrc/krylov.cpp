@@ -309,8 +309,7 @@ int pned_lanczos_batch_step(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int
     const int nc = v1->nvec;
     PNED_TRY(lanczos_slots(ctx, v1, v0, alpha_prev, beta_prev, mode));
     std::vector<double> pr(2 * nc), b2(nc);
-    PNED_TRY(pned_p_mltply_v3(ctx, v1, v0, 0.0, pr.data()));
-    PNED_TRY(pned_calc_beta(ctx, pr.data(), v1, v0, b2.data()));
+    PNED_TRY(hv_alpha_beta2(ctx, v1, v0, pr.data(), b2.data()));
     for (int c = 0; c < nc; ++c) {
         alpha[c] = mode[c] ? pr[2 * c] : 0.0;
         beta[c] = mode[c] ? std::sqrt(b2[c]) : 0.0;

@@ -657,6 +657,35 @@ int same_shape(const pned_vec *a, const pned_vec *b)
 
 }  // namespace
 
+int beta2_after_hv(pned_ctx *ctx, const pned_vec *v1, const pned_vec *v0, double *pr, double *b2)
+{
+    PNED_TRY(same_shape(v1, v0));
+    PNED_TRY(check_pow2_cols(v1->nvec));
+    const int nc = v1->nvec;
+    const size_t nelem = (size_t)v1->dim * nc;
+    const int grid = RED_GRID;
+    PNED_TRY(ensure_partial(ctx, (size_t)grid * nc));
+    PNED_TRY(ensure_result(ctx, 3 * (size_t)nc));
+    double *da;
+    PNED_TRY(ensure_scalars(ctx, 2 * nc, &da));
+    PNED_HIP(hipMemcpyAsync(da, ctx->d_result, sizeof(double) * 2 * nc, hipMemcpyDeviceToDevice, ctx->stream));
+    if (v1->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_beta2<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const double2 *)v1->data,
+                           (const double2 *)v0->data, (const double *)da, nelem, nc, ctx->d_partial);
+    else
+        hipLaunchKernelGGL(k_beta2<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const float2 *)v1->data,
+                           (const float2 *)v0->data, (const double *)da, nelem, nc, ctx->d_partial);
+    PNED_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_rows, dim3(nc), dim3(256), 0, ctx->stream, ctx->d_partial, grid, nc, ctx->d_result);
+    PNED_HIP(hipGetLastError());
+    PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
+    PNED_HIP(hipMemcpyAsync(ctx->h_result + nc, da, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, ctx->stream));
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < nc; ++c) b2[c] = ctx->h_result[c];
+    for (int c = 0; c < 2 * nc; ++c) pr[c] = ctx->h_result[nc + c];
+    return PNED_OK;
+}
+
 int intalv_pm_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, const double *site_vec, int ipm)
 {
     if (ctx->sector_k >= 0 || x->ctx->sector_k >= 0)
