@@ -36,6 +36,7 @@ struct PassArgs {
     const double *colcoef; // [devterm][nop][4] when nop > 1
     const double *coldiag; // [nop][ngroups+1][2] when nop > 1
     const double *colscale;// per-column factor on v1 (H (s v1)); null = 1
+    const double *yscale;  // diagonal pass: per-column factor on the v0 input (v0 = y v0 + ...); null = 1
     int ncat[4];           // term-major kernel: terms per category (see build_tm_terms)
     int ngroups;           // diagonal groups (batch contexts stage their weights)
 };
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
     const uint64_t offLo = pd->lo[tid];
     auto offHi = [&](int r) { return pd->hi[r]; };
     const bool rd0 = !(a.diag && a.fresh);
+    const double ys = (a.diag && a.yscale) ? a.yscale[tid & (a.ncol - 1)] : 1.0;
     const uint64_t lomask = (1ull << a.nlo) - 1;
     auto tile_base = [&](uint64_t tix) { return pd->blo[tix & lomask] | pd->bhi[tix >> a.nlo] | offLo; };
 
@@ -261,8 +263,8 @@ __global__ __launch_bounds__(TPB) void hv_s12_pass(const typename C2<Real>::T *_
                 pr_im += (double)x.x * ai[c] - (double)x.y * ar[c];
                 V o;
                 if (rd0) {
-                    o.x = (Real)((double)ycr[c] + ar[c]);
-                    o.y = (Real)((double)yci[c] + ai[c]);
+                    o.x = (Real)fma(ys, (double)ycr[c], ar[c]);
+                    o.y = (Real)fma(ys, (double)yci[c], ai[c]);
                 } else {
                     o.x = (Real)ar[c];
                     o.y = (Real)ai[c];
@@ -344,6 +346,7 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
             for (int i = tid; i < a.nop * (a.ngroups + 1) * 2; i += TPB) cdiag[i] = a.coldiag[i];
     }
     const A colsc = a.colscale ? (A)a.colscale[tid & (a.ncol - 1)] : (A)1;
+    const A ys = (a.diag && a.yscale) ? (A)a.yscale[tid & (a.ncol - 1)] : (A)1;
 
     double pr_re = 0.0, pr_im = 0.0;
     Real xr[R], xi[R];
@@ -501,8 +504,8 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
                 pr_im += (double)x.x * hi - (double)x.y * hr;
                 V o;
                 if (rd0) {
-                    o.x = (Real)(ycr[I] + ar[I]);
-                    o.y = (Real)(yci[I] + ai[I]);
+                    o.x = (Real)fma(ys, (A)ycr[I], ar[I]);
+                    o.y = (Real)fma(ys, (A)yci[I], ai[I]);
                 } else {
                     o.x = (Real)ar[I];
                     o.y = (Real)ai[I];
@@ -680,7 +683,7 @@ static void build_tm_terms(const S12Plan &pl, const PassPlan &pp, int TPB, int c
 }
 
 int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
-                  double *prdct_host, const double *d_colscale)
+                  double *prdct_host, const double *d_colscale, const double *d_yscale)
 {
     const int nvec = v1->nvec;
     int colbits = 0;
@@ -778,6 +781,7 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         a.colcoef = ctx->d_colcoef;
         a.coldiag = ctx->d_coldiag;
         a.colscale = d_colscale;
+        a.yscale = d_yscale;
         for (int c = 0; c < 4; ++c) a.ncat[c] = ctx->tm_ncat[4 * p + c];
         a.ngroups = ctx->diag_host.ngroups;
         const int TPB = tpb[p], R = rr[p];

@@ -196,6 +196,13 @@ int ensure_scalars(pned_ctx *ctx, int n, double **dptr);
 int hv_alpha_beta2(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double *pr, double *b2);
 // the beta^2 half of it: alpha taken from ctx->d_result where the H.v left it
 int beta2_after_hv(pned_ctx *ctx, const pned_vec *v1, const pned_vec *v0, double *pr, double *b2);
+// One Lanczos step with deferred normalisation (S=1/2 full space): y = sy y + H(sx x), then
+// y <- y - Re(alpha) sx x with alpha = sx <x|H|sx x>, and b2 = |y_before - alpha sx x|^2;
+// pr = <x|H|sx x> (2 per column).  fresh: y is taken as zero (sy ignored).
+int lanczos_lazy_step(pned_ctx *ctx, pned_vec *x, pned_vec *y, const double *sx, const double *sy, bool fresh,
+                      double *pr, double *b2);
+// after such steps: v1 <- sx x, v0 <- y + a sx x, (x, y) = (v1, v0) or, swapped, (v0, v1)
+int lanczos_lazy_restore(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, bool swapped, double sx, double a);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
 // S^z sector basis: H.v and table release
 int launch_hv_sector(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
@@ -218,8 +225,9 @@ int reduce_rows_result(pned_ctx *ctx, int nrows, int ncol2);
 // kernels (launchers), all asynchronous on ctx->stream unless stated
 // d_colscale: optional device array of per-column factors s_c: v0 += (H - shift)(s_c v1_c);
 // prdct is then <v1|H|s v1> (one factor s)
+// d_yscale: optional per-column factors y_c on the v0 input: v0 = y_c v0 + (H - shift)(s_c v1)
 int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
-                  double *prdct_host, const double *d_colscale = nullptr);
+                  double *prdct_host, const double *d_colscale = nullptr, const double *d_yscale = nullptr);
 int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                    double *prdct_host);
 

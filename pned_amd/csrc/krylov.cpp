@@ -14,6 +14,60 @@ namespace {
 
 double cabs2(const double *z) { return std::sqrt(z[0] * z[0] + z[1] * z[1]); }
 
+// The Lanczos recursion of lnc1z and dsflnc1z (Hybrid_TPS_Lanczos.f90:111-120,
+// Hybrid_TPS_DSF.f90:228-250): v1 <- (v0 - alpha v1)/beta, v0 <- -beta v1, then
+// v0 += H v1, alpha = Re <v1|H v1>, beta = |v0 - <v1|H v1> v1|.  In the S=1/2 full space
+// the divisions by beta are left to the next H.v: x holds v1/sx, the H.v input y is read
+// times sy, and the two buffers swap roles each step, so a step streams the vectors 3
+// times besides the H.v instead of 6.  Other bases run the reference's update as written.
+// finish() leaves v1 and v0 as the reference's loop leaves them (the last Lanczos vector
+// and H v1 - beta v_prev), so a dump for restart holds the same vectors.
+struct LanczosSteps {
+    pned_ctx *ctx;
+    pned_vec *v1, *v0;
+    bool lazy;
+    pned_vec *X, *Y;
+    double sx = 1.0, sy = 0.0;
+    LanczosSteps(pned_ctx *c, pned_vec *a, pned_vec *b)
+        : ctx(c), v1(a), v0(b), lazy(c->q == 2 && c->sector_k < 0 && a->nvec == 1), X(a), Y(b)
+    {
+    }
+    int lazy_ab(bool fresh, double &a1, double &b1)
+    {
+        double pr[2], b2;
+        PNED_TRY(lanczos_lazy_step(ctx, X, Y, &sx, &sy, fresh, pr, &b2));
+        a1 = sx * pr[0];
+        b1 = std::sqrt(b2);
+        return PNED_OK;
+    }
+    // v0 = H v1 from v1 (normalised) and alpha, beta
+    int first(double &a1, double &b1)
+    {
+        if (lazy) return lazy_ab(true, a1, b1);
+        PNED_TRY(pned_vec_zero(ctx, v0));
+        return pned_calc_alpha_beta(ctx, v1, v0, &a1, &b1);
+    }
+    // the update with the previous (a1, b1), then the next alpha, beta into them
+    int next(double &a1, double &b1)
+    {
+        if (lazy) {
+            const double nsx = 1.0 / b1, nsy = -b1 * sx;   // v1 is Y / b1, v0 is X times -b1 sx
+            std::swap(X, Y);
+            sx = nsx;
+            sy = nsy;
+            return lazy_ab(false, a1, b1);
+        }
+        PNED_TRY(pned_lanczos_update(ctx, v1, v0, &a1, &b1));
+        return pned_calc_alpha_beta(ctx, v1, v0, &a1, &b1);
+    }
+    // a1: the last alpha
+    int finish(double a1)
+    {
+        if (!lazy) return PNED_OK;
+        return lanczos_lazy_restore(ctx, v1, v0, X != v1, sx, a1);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -129,20 +183,20 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
     if (nvec < 0) return fail(PNED_E_ARG, "#(E06) Wrong value given to nvec in lnc1");
     if (v1->nvec != 1) return fail(PNED_E_ARG, "lnc1 runs on one column");
     const int last = fixed_steps > 0 ? fixed_steps : max_itr;
-    PNED_TRY(pned_vec_zero(ctx, v0));
     double a1, b1;
-    PNED_TRY(pned_calc_alpha_beta(ctx, v1, v0, &a1, &b1));
+    LanczosSteps run(ctx, v1, v0);
+    PNED_TRY(run.first(a1, b1));
     alpha[0] = a1;
     beta[0] = b1;
     double ebefor = 0.0, eps = 1e-25;
     const int ne = std::max(nvec, 1);
     for (int i = 2; i <= last; ++i) {
-        PNED_TRY(pned_lanczos_update(ctx, v1, v0, &a1, &b1));
-        PNED_TRY(pned_calc_alpha_beta(ctx, v1, v0, &a1, &b1));
+        PNED_TRY(run.next(a1, b1));
         alpha[i - 1] = a1;
         beta[i - 1] = b1;
         if (b1 < 0.5e-30) {
             *itr = i;
+            PNED_TRY(run.finish(a1));
             return fail(PNED_E_BREAKDOWN, "#(E07) Tridiagonalization unsuccessful in lnc1: beta too small at i=" +
                                               std::to_string(i));
         }
@@ -151,7 +205,7 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
             PNED_TRY(pned_bisec(alpha, beta, i, ene, ne, eps));
             if (std::fabs((ebefor - ene[ne - 1]) / ene[ne - 1]) < 1e-13) {
                 *itr = i;
-                return PNED_OK;
+                return run.finish(a1);
             }
             ebefor = ene[ne - 1];
         }
@@ -161,6 +215,7 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
             ebefor = ene[ne - 1];
         }
     }
+    PNED_TRY(run.finish(a1));
     if (fixed_steps > 0) {
         PNED_TRY(pned_bisec(alpha, beta, last, ene, std::min(ne, last), 1e-25));
         *itr = last;
@@ -502,19 +557,18 @@ int pned_dsflnc1z(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double cnorm, int s
                   double *sbeta)
 {
     if (!ctx || !v1 || !v0 || !salpha || !sbeta || s_maxitr < 2) return fail(PNED_E_ARG, "bad argument");
-    PNED_TRY(pned_vec_zero(ctx, v0));
     sbeta[0] = cnorm;
     double a1, b1;
-    PNED_TRY(pned_calc_alpha_beta(ctx, v1, v0, &a1, &b1));
+    LanczosSteps run(ctx, v1, v0);
+    PNED_TRY(run.first(a1, b1));
     salpha[0] = a1;
     sbeta[1] = b1;
     for (int i = 2; i <= s_maxitr - 1; ++i) {
-        PNED_TRY(pned_lanczos_update(ctx, v1, v0, &a1, &b1));
-        PNED_TRY(pned_calc_alpha_beta(ctx, v1, v0, &a1, &b1));
+        PNED_TRY(run.next(a1, b1));
         salpha[i - 1] = a1;
         sbeta[i] = b1;
     }
-    return PNED_OK;
+    return run.finish(a1);
 }
 
 int pned_renbun(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,

@@ -140,6 +140,61 @@ __global__ __launch_bounds__(RED_TPB) void k_beta2(const typename V2<Real>::T *_
     block_reduce_cols<1>(red, acc, ncol, partial + (size_t)blockIdx.x * ncol);
 }
 
+// Lanczos step with the normalisations left to the next H.v (column c): x holds v1 / s_c,
+// y holds H v1 - beta v_prev; with alpha_c = s_c prd_c (prd = <x|H|s x> from the H.v),
+//   y <- y - Re(alpha_c) s_c x          (the next v1, times beta)
+//   acc += |y - alpha_c s_c x|^2        (beta^2 as calc_beta forms it, complex alpha)
+// one read of x and y and one write of y.  sc = [s (ncol)], prd = [re, im] per column.
+template <typename Real>
+__global__ __launch_bounds__(RED_TPB) void k_lanczos_lazy(const typename V2<Real>::T *__restrict__ x,
+                                                          typename V2<Real>::T *__restrict__ y,
+                                                          const double *__restrict__ sc, const double *__restrict__ prd,
+                                                          size_t nelem, int ncol, double *__restrict__ partial)
+{
+    __shared__ double red[RED_TPB];
+    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    const size_t e0 = (size_t)blockIdx.x * RED_TPB + threadIdx.x;
+    const int col = (int)(e0 % (size_t)ncol);
+    const double s = sc[col];
+    const double ar = s * prd[2 * col], ai = s * prd[2 * col + 1];
+    double acc[1] = {0.0};
+    for (size_t e = e0; e < nelem; e += stride) {
+        double xr, xi, yr, yi;
+        ld(x, e, xr, xi);
+        ld(y, e, yr, yi);
+        xr *= s;
+        xi *= s;
+        const double wr = yr - (ar * xr - ai * xi);
+        const double wi = yi - (ar * xi + ai * xr);
+        acc[0] = fma(wr, wr, fma(wi, wi, acc[0]));
+        st(y, e, yr - ar * xr, yi - ar * xi);
+    }
+    block_reduce_cols<1>(red, acc, ncol, partial + (size_t)blockIdx.x * ncol);
+}
+
+// back to the reference's vectors after deferred-normalisation steps: with (x, y) the
+// buffers holding v1/sx and v0 - a v1 (swapped: x is p0, y is p1), p1 <- sx x and
+// p0 <- y + a sx x.  Each element is read before it is written by the same thread.
+template <typename Real>
+__global__ void k_lanczos_restore(typename V2<Real>::T *__restrict__ p1, typename V2<Real>::T *__restrict__ p0,
+                                  int swapped, double sx, double a, size_t nelem)
+{
+    const size_t stride = (size_t)blockDim.x * gridDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nelem; e += stride) {
+        double xr, xi, yr, yi;
+        if (swapped) {
+            ld(p0, e, xr, xi);
+            ld(p1, e, yr, yi);
+        } else {
+            ld(p1, e, xr, xi);
+            ld(p0, e, yr, yi);
+        }
+        const double tr = sx * xr, ti = sx * xi;
+        st(p1, e, tr, ti);
+        st(p0, e, yr + a * tr, yi + a * ti);
+    }
+}
+
 // y = a_c x + b_c y  (complex scalars per column)
 template <typename Real>
 __global__ void k_axpby(const typename V2<Real>::T *__restrict__ x, typename V2<Real>::T *__restrict__ y,
@@ -683,6 +738,61 @@ int beta2_after_hv(pned_ctx *ctx, const pned_vec *v1, const pned_vec *v0, double
     PNED_HIP(hipStreamSynchronize(ctx->stream));
     for (int c = 0; c < nc; ++c) b2[c] = ctx->h_result[c];
     for (int c = 0; c < 2 * nc; ++c) pr[c] = ctx->h_result[nc + c];
+    return PNED_OK;
+}
+
+int lanczos_lazy_step(pned_ctx *ctx, pned_vec *x, pned_vec *y, const double *sx, const double *sy, bool fresh,
+                      double *pr, double *b2)
+{
+    PNED_TRY(same_shape(x, y));
+    PNED_TRY(check_pow2_cols(x->nvec));
+    if (ctx->q != 2 || ctx->sector_k >= 0) return fail(PNED_E_UNSUPPORTED, "lazy Lanczos step: S=1/2 full space");
+    const int nc = x->nvec;
+    const size_t nelem = (size_t)x->dim * nc;
+    const int grid = RED_GRID;
+    PNED_TRY(ensure_result(ctx, 3 * (size_t)nc + 8));
+    double *d;
+    PNED_TRY(ensure_scalars(ctx, 4 * nc, &d));
+    std::vector<double> up(2 * nc);
+    for (int c = 0; c < nc; ++c) {
+        up[c] = sx[c];
+        up[nc + c] = sy[c];
+    }
+    PNED_HIP(hipMemcpyAsync(d, up.data(), sizeof(double) * 2 * nc, hipMemcpyHostToDevice, ctx->stream));
+    // y = sy y + H (sx x); the H.v's <x|H|sx x> lands in d_result
+    PNED_TRY(launch_hv_s12(ctx, x, y, 0.0, fresh, nullptr, d, fresh ? nullptr : d + nc));
+    PNED_TRY(ensure_partial(ctx, (size_t)grid * nc));
+    double *dprd = d + 2 * nc;
+    PNED_HIP(hipMemcpyAsync(dprd, ctx->d_result, sizeof(double) * 2 * nc, hipMemcpyDeviceToDevice, ctx->stream));
+    if (x->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_lanczos_lazy<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const double2 *)x->data,
+                           (double2 *)y->data, (const double *)d, (const double *)dprd, nelem, nc, ctx->d_partial);
+    else
+        hipLaunchKernelGGL(k_lanczos_lazy<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const float2 *)x->data,
+                           (float2 *)y->data, (const double *)d, (const double *)dprd, nelem, nc, ctx->d_partial);
+    PNED_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_rows, dim3(nc), dim3(256), 0, ctx->stream, ctx->d_partial, grid, nc, ctx->d_result);
+    PNED_HIP(hipGetLastError());
+    PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double) * nc, hipMemcpyDeviceToHost, ctx->stream));
+    PNED_HIP(hipMemcpyAsync(ctx->h_result + nc, dprd, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, ctx->stream));
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < nc; ++c) b2[c] = ctx->h_result[c];
+    for (int c = 0; c < 2 * nc; ++c) pr[c] = ctx->h_result[nc + c];
+    return PNED_OK;
+}
+
+int lanczos_lazy_restore(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, bool swapped, double sx, double a)
+{
+    PNED_TRY(same_shape(v1, v0));
+    const size_t nelem = (size_t)v1->dim * v1->nvec;
+    const int grid = grid_for(nelem, 256);
+    if (v1->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_lanczos_restore<double>, dim3(grid), dim3(256), 0, ctx->stream, (double2 *)v1->data,
+                           (double2 *)v0->data, swapped ? 1 : 0, sx, a, nelem);
+    else
+        hipLaunchKernelGGL(k_lanczos_restore<float>, dim3(grid), dim3(256), 0, ctx->stream, (float2 *)v1->data,
+                           (float2 *)v0->data, swapped ? 1 : 0, sx, a, nelem);
+    PNED_HIP(hipGetLastError());
     return PNED_OK;
 }
 

@@ -168,6 +168,33 @@ def test_lanczos_parity_n16():
     assert np.allclose(beta[:20], b2[:20], rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("steps", [1, 2, 7, 12])
+def test_lanczos_leaves_the_reference_vectors(steps):
+    """After lnc1 (fixed steps) and dsflnc1z the two vectors are what the reference's loop
+    leaves: v1 the last Lanczos vector (unit norm), v0 = H v1 - beta v_prev, so
+    <v1|v0> = alpha_last and |v0 - alpha_last v1| = beta_last -- also when the
+    deferred-normalisation steps left the buffers swapped."""
+    _say("lanczos vectors", steps)
+    op = model.build_operator(lattice.reference_cluster(16), 20, 126.0, field=0.1)
+    ctx = gpu.Context(op)
+    v1, v0 = ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 5)
+    itr, alpha, beta, ene = ctx.lnc1(v1, v0, fixed_steps=steps, nvec=1)
+    a, b = alpha[steps - 1], beta[steps - 1]
+    for x1, x0 in ((v1.download(), v0.download()),):
+        assert abs(np.linalg.norm(x1) - 1.0) <= 1e-13
+        assert abs(np.vdot(x1, x0) - a) <= 1e-12 * np.linalg.norm(x0)
+        assert abs(np.linalg.norm(x0 - a * x1) - b) <= 1e-12 * np.linalg.norm(x0)
+    if steps >= 2:
+        ctx.set_initial_vec(v1, 6)
+        sa, sb = ctx.dsflnc1z(v1, v0, 0.5, steps + 1)
+        x1, x0 = v1.download(), v0.download()
+        a, b = sa[steps - 1], sb[steps]
+        assert abs(np.linalg.norm(x1) - 1.0) <= 1e-13
+        assert abs(np.vdot(x1, x0) - a) <= 1e-12 * np.linalg.norm(x0)
+        assert abs(np.linalg.norm(x0 - a * x1) - b) <= 1e-12 * np.linalg.norm(x0)
+
+
 def test_inverse_iteration_and_check1_n12():
     _say("inv1 n12")
     op = model.build_operator(lattice.reference_cluster(12), 20, 54.0)
