@@ -166,7 +166,7 @@ static int build_spin_tables(pned_ctx *ctx, SpinTables &out)
 }
 
 int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
-                         double *prdct_host);
+                         double *prdct_host, double xs, double ys);
 
 int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                    double *prdct_host)
@@ -174,7 +174,7 @@ int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperb
     if (v1->nvec != 1 || v1->dtype != PNED_C128)
         return fail(PNED_E_UNSUPPORTED, "spin-S H.v takes one complex128 column");
     if (ctx->spin_tiled && ctx->q >= 3 && ctx->q <= 5)
-        return launch_hv_spin_tiled(ctx, v1, v0, eperbd, fresh, prdct_host);
+        return launch_hv_spin_tiled(ctx, v1, v0, eperbd, fresh, prdct_host, 1.0, 1.0);
     if (ctx->q > 15 || ctx->n > 32) return fail(PNED_E_UNSUPPORTED, "spin-S H.v: q <= 15, N <= 32");
     SpinTables tb;
     PNED_TRY(build_spin_tables(ctx, tb));
@@ -251,6 +251,8 @@ struct SpinPassArgs {
     int nent;                         // entries of this pass
     int nlo_mod;                      // tile_count split: tix % nlo_mod, tix / nlo_mod
     double shift;                     // -eperbd per bond diagonal
+    double xs;                        // v0 += xs (H v1); <v1|.> then carries one factor xs
+    double ys;                        // first pass: factor on the v0 input
 };
 
 template <int Q, int TPB, int R, int S>
@@ -273,6 +275,7 @@ __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ 
         cf[i] = make_double2(pd->ent_c[2 * i], pd->ent_c[2 * i + 1]);
     }
     const bool rd0 = !(a.diag_first && a.fresh);
+    const double ysc = a.diag_first ? a.ys : 1.0;
     // per-slot digit words (4 bits per tile digit) and memory offsets, fixed for the launch
     uint32_t dw[R];
     uint64_t offj[R];
@@ -353,10 +356,12 @@ __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ 
                         ai = fma(cr, p.y, fma(c.y, p.x, ai));
                     }
                 }
+                ar *= a.xs;
+                ai *= a.xs;
                 const double2 x = tile[j];
                 pr_re += x.x * ar + x.y * ai;
                 pr_im += x.x * ai - x.y * ar;
-                v0[base + offj[r]] = make_double2(yr[u] + ar, yi[u] + ai);
+                v0[base + offj[r]] = make_double2(fma(ysc, yr[u], ar), fma(ysc, yi[u], ai));
             }
         }
         __syncthreads();
@@ -442,7 +447,7 @@ static void launch_spin_pass(pned_ctx *ctx, int grid, size_t lds, const pned_vec
 }
 
 int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
-                         double *prdct_host)
+                         double *prdct_host, double xs, double ys)
 {
     const int q = ctx->q, n = ctx->n, qq = q * q;
     const int L0 = (ctx->tune_low_bits > 0) ? ctx->tune_low_bits : 2;
@@ -573,6 +578,8 @@ int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double
         a.fresh = fresh ? 1 : 0;
         a.nlo_mod = ctx->spin_nlo[p];
         a.shift = -eperbd;
+        a.xs = xs;
+        a.ys = ys;
         const int grid = (int)std::min<uint64_t>(a.tile_count, (uint64_t)grid_cap);
         const SpinPassDev *pd = reinterpret_cast<const SpinPassDev *>(ctx->spin_pd) + p;
         double *part = ctx->d_partial + 2 * (size_t)rows;

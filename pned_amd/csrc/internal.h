@@ -230,6 +230,10 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
                   double *prdct_host, const double *d_colscale = nullptr, const double *d_yscale = nullptr);
 int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                    double *prdct_host);
+// the tiled spin-S kernel (q = 3..5) with v0 = ys v0 + xs (H - shift) v1 (ys on the first
+// pass's input; <v1|.> carries the factor xs)
+int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                         double *prdct_host, double xs, double ys);
 
 // reductions return host values (synchronizing)
 int dot(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out /*2*nvec*/);

@@ -17,9 +17,10 @@ double cabs2(const double *z) { return std::sqrt(z[0] * z[0] + z[1] * z[1]); }
 // The Lanczos recursion of lnc1z and dsflnc1z (Hybrid_TPS_Lanczos.f90:111-120,
 // Hybrid_TPS_DSF.f90:228-250): v1 <- (v0 - alpha v1)/beta, v0 <- -beta v1, then
 // v0 += H v1, alpha = Re <v1|H v1>, beta = |v0 - <v1|H v1> v1|.  In the S=1/2 full space
-// the divisions by beta are left to the next H.v: x holds v1/sx, the H.v input y is read
-// times sy, and the two buffers swap roles each step, so a step streams the vectors 3
-// times besides the H.v instead of 6.  Other bases run the reference's update as written.
+// (and with the tiled spin-S kernel) the divisions by beta are left to the next H.v: x
+// holds v1/sx, the H.v input y is read times sy, and the two buffers swap roles each step,
+// so a step streams the vectors 3 times besides the H.v instead of 6.  Other bases run the
+// reference's update as written.
 // finish() leaves v1 and v0 as the reference's loop leaves them (the last Lanczos vector
 // and H v1 - beta v_prev), so a dump for restart holds the same vectors.
 struct LanczosSteps {
@@ -29,7 +30,10 @@ struct LanczosSteps {
     pned_vec *X, *Y;
     double sx = 1.0, sy = 0.0;
     LanczosSteps(pned_ctx *c, pned_vec *a, pned_vec *b)
-        : ctx(c), v1(a), v0(b), lazy(c->q == 2 && c->sector_k < 0 && a->nvec == 1), X(a), Y(b)
+        : ctx(c), v1(a), v0(b),
+          lazy(a->nvec == 1 && ((c->q == 2 && c->sector_k < 0) ||
+                                (c->q >= 3 && c->q <= 5 && c->spin_tiled && a->dtype == PNED_C128))),
+          X(a), Y(b)
     {
     }
     int lazy_ab(bool fresh, double &a1, double &b1)

@@ -746,7 +746,9 @@ int lanczos_lazy_step(pned_ctx *ctx, pned_vec *x, pned_vec *y, const double *sx,
 {
     PNED_TRY(same_shape(x, y));
     PNED_TRY(check_pow2_cols(x->nvec));
-    if (ctx->q != 2 || ctx->sector_k >= 0) return fail(PNED_E_UNSUPPORTED, "lazy Lanczos step: S=1/2 full space");
+    const bool spin = ctx->q >= 3 && ctx->q <= 5 && ctx->spin_tiled;
+    if (!((ctx->q == 2 && ctx->sector_k < 0) || (spin && x->nvec == 1 && x->dtype == PNED_C128)))
+        return fail(PNED_E_UNSUPPORTED, "lazy Lanczos step: S=1/2 full space or the tiled spin-S kernel");
     const int nc = x->nvec;
     const size_t nelem = (size_t)x->dim * nc;
     const int grid = RED_GRID;
@@ -760,7 +762,10 @@ int lanczos_lazy_step(pned_ctx *ctx, pned_vec *x, pned_vec *y, const double *sx,
     }
     PNED_HIP(hipMemcpyAsync(d, up.data(), sizeof(double) * 2 * nc, hipMemcpyHostToDevice, ctx->stream));
     // y = sy y + H (sx x); the H.v's <x|H|sx x> lands in d_result
-    PNED_TRY(launch_hv_s12(ctx, x, y, 0.0, fresh, nullptr, d, fresh ? nullptr : d + nc));
+    if (spin)
+        PNED_TRY(launch_hv_spin_tiled(ctx, x, y, 0.0, fresh, nullptr, sx[0], fresh ? 1.0 : sy[0]));
+    else
+        PNED_TRY(launch_hv_s12(ctx, x, y, 0.0, fresh, nullptr, d, fresh ? nullptr : d + nc));
     PNED_TRY(ensure_partial(ctx, (size_t)grid * nc));
     double *dprd = d + 2 * nc;
     PNED_HIP(hipMemcpyAsync(dprd, ctx->d_result, sizeof(double) * 2 * nc, hipMemcpyDeviceToDevice, ctx->stream));

@@ -168,14 +168,31 @@ def test_lanczos_parity_n16():
     assert np.allclose(beta[:20], b2[:20], rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("steps", [1, 2, 7, 12])
-def test_lanczos_leaves_the_reference_vectors(steps):
+@pytest.mark.parametrize("spec,phi", [(8, 108.0), (12, 108.0)])
+def test_lanczos_parity_spin1(spec, phi):
+    """Spin-1 Lanczos (tiled spin-S kernel, deferred normalisation) against the oracle."""
+    _say("lanczos spin1", spec)
+    op = model.build_operator(lattice.reference_cluster(spec), 20, phi, q=3)
+    ctx = gpu.Context(op)
+    v1, v0 = ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 20261202)
+    x = v1.download()
+    itr, alpha, beta, ene = ctx.lnc1(v1, v0, max_itr=1024, nvec=4)
+    i2, a2, b2, e2 = oracle.lanczos(oracle.OracleOp(op), x, max_itr=1024, nvec=4)
+    assert abs(ene[0] - e2[0]) <= 1e-10 * abs(e2[0])
+    assert np.allclose(alpha[:20], a2[:20], rtol=1e-12, atol=0)
+    assert np.allclose(beta[:20], b2[:20], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("q,steps", [(2, 1), (2, 2), (2, 7), (2, 12), (3, 2), (3, 9)])
+def test_lanczos_leaves_the_reference_vectors(q, steps):
     """After lnc1 (fixed steps) and dsflnc1z the two vectors are what the reference's loop
     leaves: v1 the last Lanczos vector (unit norm), v0 = H v1 - beta v_prev, so
     <v1|v0> = alpha_last and |v0 - alpha_last v1| = beta_last -- also when the
     deferred-normalisation steps left the buffers swapped."""
-    _say("lanczos vectors", steps)
-    op = model.build_operator(lattice.reference_cluster(16), 20, 126.0, field=0.1)
+    _say("lanczos vectors", q, steps)
+    op = (model.build_operator(lattice.reference_cluster(16), 20, 126.0, field=0.1) if q == 2
+          else model.build_operator(lattice.reference_cluster(8), 20, 108.0, q=3))
     ctx = gpu.Context(op)
     v1, v0 = ctx.vector(), ctx.vector()
     ctx.set_initial_vec(v1, 5)
