@@ -201,6 +201,9 @@ int beta2_after_hv(pned_ctx *ctx, const pned_vec *v1, const pned_vec *v0, double
 // pr = <x|H|sx x> (2 per column).  fresh: y is taken as zero (sy ignored).
 int lanczos_lazy_step(pned_ctx *ctx, pned_vec *x, pned_vec *y, const double *sx, const double *sy, bool fresh,
                       double *pr, double *b2);
+// COCG: x += a p, r -= a y; out4 = <r|r> (re, im), r^T r (re, im) of the new r
+int cocg_update(pned_ctx *ctx, const pned_vec *p, const pned_vec *y, pned_vec *x, pned_vec *r, const double *a,
+                double *out4);
 // after such steps: v1 <- sx x, v0 <- y + a sx x, (x, y) = (v1, v0) or, swapped, (v0, v1)
 int lanczos_lazy_restore(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, bool swapped, double sx, double a);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);

@@ -341,14 +341,16 @@ int pned_cocg(pned_ctx *ctx, double e0, double eta, const pned_vec *b, pned_vec 
         PNED_TRY(pned_axpby(ctx, ieta, p, one, y));          // y += i eta p
         PNED_TRY(dotu(ctx, p, y, mu));
         const std::complex<double> al = std::complex<double>(rho[0], rho[1]) / std::complex<double>(mu[0], mu[1]);
-        const double pa[2] = {al.real(), al.imag()}, ma[2] = {-al.real(), -al.imag()};
-        PNED_TRY(pned_axpby(ctx, pa, p, one, x));
-        PNED_TRY(pned_axpby(ctx, ma, y, one, r));
-        PNED_TRY(pned_calc_vec_inprdct(ctx, r, r, rr));
+        const double pa[2] = {al.real(), al.imag()};
+        double sums[4];
+        PNED_TRY(cocg_update(ctx, p, y, x, r, pa, sums));   // x += al p, r -= al y, <r|r>, r^T r
+        rr[0] = sums[0];
+        rr[1] = sums[1];
         *resid = std::sqrt(rr[0]) / bnorm;
         *iters = it;
         if (*resid < tol) return PNED_OK;
-        PNED_TRY(dotu(ctx, r, r, rho2));
+        rho2[0] = sums[2];
+        rho2[1] = sums[3];
         const std::complex<double> be = std::complex<double>(rho2[0], rho2[1]) / std::complex<double>(rho[0], rho[1]);
         const double bt[2] = {be.real(), be.imag()};
         PNED_TRY(pned_axpby(ctx, one, r, bt, p));             // p = r + beta p

@@ -195,6 +195,37 @@ __global__ void k_lanczos_restore(typename V2<Real>::T *__restrict__ p1, typenam
     }
 }
 
+// one COCG update (one column): x += a p, r -= a y, and of the new r the sums
+// <r|r> (re, im) and r^T r (re, im): one pass over p, y, x, r instead of two axpy passes
+// and two reductions
+template <typename Real>
+__global__ __launch_bounds__(RED_TPB) void k_cocg_update(const typename V2<Real>::T *__restrict__ p,
+                                                         const typename V2<Real>::T *__restrict__ y,
+                                                         typename V2<Real>::T *__restrict__ x,
+                                                         typename V2<Real>::T *__restrict__ r, double ar, double ai,
+                                                         size_t nelem, double *__restrict__ partial)
+{
+    __shared__ double red[4 * RED_TPB];
+    const size_t stride = (size_t)RED_TPB * gridDim.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (size_t e = (size_t)blockIdx.x * RED_TPB + threadIdx.x; e < nelem; e += stride) {
+        double pr, pi, yr, yi, xr, xi, rr, ri;
+        ld(p, e, pr, pi);
+        ld(y, e, yr, yi);
+        ld(x, e, xr, xi);
+        ld(r, e, rr, ri);
+        st(x, e, ar * pr - ai * pi + xr, ar * pi + ai * pr + xi);
+        rr = rr - (ar * yr - ai * yi);
+        ri = ri - (ar * yi + ai * yr);
+        st(r, e, rr, ri);
+        acc[0] = fma(rr, rr, fma(ri, ri, acc[0]));
+        acc[1] = fma(rr, ri, fma(-ri, rr, acc[1]));
+        acc[2] = fma(rr, rr, fma(-ri, ri, acc[2]));
+        acc[3] = fma(rr, ri, fma(ri, rr, acc[3]));
+    }
+    block_reduce_cols<4>(red, acc, 1, partial + (size_t)blockIdx.x * 4);
+}
+
 // y = a_c x + b_c y  (complex scalars per column)
 template <typename Real>
 __global__ void k_axpby(const typename V2<Real>::T *__restrict__ x, typename V2<Real>::T *__restrict__ y,
@@ -784,6 +815,28 @@ int lanczos_lazy_step(pned_ctx *ctx, pned_vec *x, pned_vec *y, const double *sx,
     for (int c = 0; c < nc; ++c) b2[c] = ctx->h_result[c];
     for (int c = 0; c < 2 * nc; ++c) pr[c] = ctx->h_result[nc + c];
     return PNED_OK;
+}
+
+int cocg_update(pned_ctx *ctx, const pned_vec *p, const pned_vec *y, pned_vec *x, pned_vec *r, const double *a,
+                double *out4)
+{
+    PNED_TRY(same_shape(p, y));
+    PNED_TRY(same_shape(p, x));
+    PNED_TRY(same_shape(p, r));
+    if (p->nvec != 1) return fail(PNED_E_UNSUPPORTED, "cocg update: one column");
+    const size_t nelem = (size_t)p->dim;
+    const int grid = RED_GRID;
+    PNED_TRY(ensure_partial(ctx, (size_t)grid * 4));
+    if (p->dtype == PNED_C128)
+        hipLaunchKernelGGL(k_cocg_update<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const double2 *)p->data,
+                           (const double2 *)y->data, (double2 *)x->data, (double2 *)r->data, a[0], a[1], nelem,
+                           ctx->d_partial);
+    else
+        hipLaunchKernelGGL(k_cocg_update<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (const float2 *)p->data,
+                           (const float2 *)y->data, (float2 *)x->data, (float2 *)r->data, a[0], a[1], nelem,
+                           ctx->d_partial);
+    PNED_HIP(hipGetLastError());
+    return finish_reduce(ctx, grid, 4, out4);
 }
 
 int lanczos_lazy_restore(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, bool swapped, double sx, double a)
