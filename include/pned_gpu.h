@@ -175,7 +175,11 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
  * vector, v0 is cleared; the lnc1z start, Hybrid_TPS_Lanczos.f90:78-85), 2 continue (the
  * update of Hybrid_TPS_Lanczos.f90:111-116 with alpha_prev[c], beta_prev[c]).  Then one
  * batched H.v and per-column alpha[c] = <v1|H|v1>, beta[c] = ||v0 - alpha v1||.  Every
- * column's arithmetic is independent of the others (fixed-order per-column reductions). */
+ * column's arithmetic is independent of the others (fixed-order per-column reductions).
+ * One column of a plain context (S=1/2 full space, tiled spin-S) steps as pned_lnc1 does,
+ * with the normalisations deferred: between calls v1 and v0 hold scaled vectors in roles
+ * the context tracks (only alpha and beta are meaningful; mode 1 restarts from v1, mode 0
+ * does nothing). */
 int pned_lanczos_batch_step(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int *mode,
                             const double *alpha_prev, const double *beta_prev, double *alpha,
                             double *beta);

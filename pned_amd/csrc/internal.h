@@ -156,6 +156,10 @@ struct pned_ctx {
     pned::TmTerm *d_tm = nullptr;    // per-pass tile-coordinate terms (term-major kernel)
     std::vector<int> tm_ncat;        // [pass][4] terms per term-major category
     int s12_kernel = 1;              // 1: term-major pass kernel, 0: state-major pass kernel
+    // one-column lanczos_batch_step with deferred normalisation: which handle holds x
+    // (v1/sx) and the two factors the next step applies
+    int lz_swapped = 0;
+    double lz_sx = 1.0, lz_sy = 0.0;
     // S^z sector basis (parity_type = 0); sector_k < 0: full 2^N space
     int sector_k = -1;
     int sector_nterm = 0;

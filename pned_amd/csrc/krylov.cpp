@@ -368,6 +368,34 @@ int pned_lanczos_batch_step(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int
     if (!ctx || !v1 || !v0 || !mode || !alpha_prev || !beta_prev || !alpha || !beta)
         return fail(PNED_E_ARG, "null argument");
     const int nc = v1->nvec;
+    // one column of a plain context: the deferred-normalisation step (as lnc1 runs it),
+    // the buffer roles kept in the context between calls; mode 1 restarts from v1
+    const bool lazy = nc == 1 && ctx->s12.nop <= 1 &&
+                      ((ctx->q == 2 && ctx->sector_k < 0) ||
+                       (ctx->q >= 3 && ctx->q <= 5 && ctx->spin_tiled && v1->dtype == PNED_C128));
+    if (lazy) {
+        if (mode[0] == 0) {
+            alpha[0] = beta[0] = 0.0;
+            return PNED_OK;
+        }
+        if (mode[0] == 1) {
+            ctx->lz_swapped = 0;
+            ctx->lz_sx = 1.0;
+            ctx->lz_sy = 0.0;
+        } else {
+            if (!(beta_prev[0] > 0.0)) return fail(PNED_E_ARG, "stepping a column with beta <= 0");
+            const double nsx = 1.0 / beta_prev[0], nsy = -beta_prev[0] * ctx->lz_sx;
+            ctx->lz_swapped ^= 1;
+            ctx->lz_sx = nsx;
+            ctx->lz_sy = nsy;
+        }
+        pned_vec *X = ctx->lz_swapped ? v0 : v1, *Y = ctx->lz_swapped ? v1 : v0;
+        double pr[2], b2;
+        PNED_TRY(lanczos_lazy_step(ctx, X, Y, &ctx->lz_sx, &ctx->lz_sy, mode[0] == 1, pr, &b2));
+        alpha[0] = ctx->lz_sx * pr[0];
+        beta[0] = std::sqrt(b2);
+        return PNED_OK;
+    }
     PNED_TRY(lanczos_slots(ctx, v1, v0, alpha_prev, beta_prev, mode));
     std::vector<double> pr(2 * nc), b2(nc);
     PNED_TRY(hv_alpha_beta2(ctx, v1, v0, pr.data(), b2.data()));

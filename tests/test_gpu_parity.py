@@ -184,6 +184,29 @@ def test_lanczos_parity_spin1(spec, phi):
     assert np.allclose(beta[:20], b2[:20], rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("q", [2, 3])
+def test_one_column_batch_step_equals_lnc1(q):
+    """A one-column lanczos_batch_step sequence (the serving path for large sizes) gives
+    lnc1's alpha/beta bit for bit, also after a restart (mode 1) in the middle."""
+    _say("one-column batch step", q)
+    op = (model.build_operator(lattice.reference_cluster(16), 20, 126.0, field=0.1) if q == 2
+          else model.build_operator(lattice.reference_cluster(8), 20, 108.0, q=3))
+    ctx = gpu.Context(op)
+    v1, v0 = ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 77)
+    _, alpha, beta, _ = ctx.lnc1(v1, v0, fixed_steps=25, nvec=1)
+    for _round in range(2):
+        ctx.set_initial_vec(v1, 77)
+        a_prev, b_prev = np.zeros(1), np.zeros(1)
+        got_a, got_b = [], []
+        for k in range(25):
+            a, b = ctx.lanczos_batch_step(v1, v0, np.array([1 if k == 0 else 2], dtype=np.int32), a_prev, b_prev)
+            got_a.append(a[0])
+            got_b.append(b[0])
+            a_prev, b_prev = np.atleast_1d(a).copy(), np.atleast_1d(b).copy()
+        assert np.array_equal(np.array(got_a), alpha[:25]) and np.array_equal(np.array(got_b), beta[:25])
+
+
 @pytest.mark.parametrize("q,steps", [(2, 1), (2, 2), (2, 7), (2, 12), (3, 2), (3, 9)])
 def test_lanczos_leaves_the_reference_vectors(q, steps):
     """After lnc1 (fixed steps) and dsflnc1z the two vectors are what the reference's loop
