@@ -427,6 +427,44 @@ static std::vector<std::vector<int>> spin_cover(int n, const std::vector<int32_t
     return out;
 }
 
+// Spreads the bonds over the passes whose tiles hold both of their sites: bonds with the
+// fewest covering passes are placed first, each on the pass with the fewest bonds so far,
+// so that no pass carries many more (bond, state) rows than the others. Passes left
+// without bonds are dropped.
+static void spin_balance(std::vector<std::vector<int>> &cov, const std::vector<int32_t> &sp, int n)
+{
+    const int np = (int)cov.size() / 2;
+    std::vector<std::vector<char>> has(np, std::vector<char>(n, 0));
+    std::vector<int> all;
+    for (int p = 0; p < np; ++p) {
+        for (int st : cov[2 * p]) has[p][st] = 1;
+        for (int b : cov[2 * p + 1]) all.push_back(b);
+    }
+    std::vector<std::vector<int>> choice(all.size());
+    for (size_t i = 0; i < all.size(); ++i)
+        for (int p = 0; p < np; ++p)
+            if (has[p][sp[2 * all[i]]] && has[p][sp[2 * all[i] + 1]]) choice[i].push_back(p);
+    std::vector<size_t> order(all.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return choice[a].size() < choice[b].size(); });
+    std::vector<std::vector<int>> bonds(np);
+    for (size_t i : order) {
+        int best = choice[i][0];
+        for (int p : choice[i])
+            if (bonds[p].size() < bonds[best].size()) best = p;
+        bonds[best].push_back(all[i]);
+    }
+    std::vector<std::vector<int>> out;
+    for (int p = 0; p < np; ++p) {
+        if (bonds[p].empty()) continue;
+        std::sort(bonds[p].begin(), bonds[p].end());
+        out.push_back(cov[2 * p]);
+        out.push_back(bonds[p]);
+    }
+    cov.swap(out);
+}
+
 // LDS bytes of one pass: tile, then the row table (starts, deltas, coefficients)
 static size_t spin_lds_bytes(int tsz, int nrow, int nent, int tpb)
 {
@@ -465,6 +503,7 @@ int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double
             const int L = std::min(L0, K);
             auto cov = spin_cover(n, ctx->site_pair, ctx->ibond, K, L);
             if (cov.empty()) return fail(PNED_E_UNSUPPORTED, "spin-S tile planner made no progress");
+            spin_balance(cov, ctx->site_pair, n);
             const int np = (int)cov.size() / 2;
             host.assign(np, SpinPassDev{});
             nb.assign(np, 0);
