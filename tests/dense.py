@@ -68,3 +68,71 @@ def model20_couplings(phi_deg):
     J = math.cos(a)
     K1 = math.cos(a) + 2 * math.sin(a)
     return J, K1 - J
+
+
+# -- product states: closed forms of <v|H|v> and <v|H H|v> --------------------------------
+
+def kh_terms(cluster, q, phi, h):
+    """H = sum_T c_T prod_{k in T} A_T,k as (c, {site: matrix}) from textbook operators."""
+    J, K = model20_couplings(phi)
+    S = site_ops(q)
+    out = []
+    for (a, b, t) in cluster.bonds:
+        for g in range(3):
+            c = J + (K if g == t - 1 else 0.0)
+            if c != 0.0:
+                out.append((c, {a - 1: S[g], b - 1: S[g]}))
+    if h:
+        ns = S[0] + S[1] + S[2]
+        for k in range(cluster.n):
+            out.append((h, {k: ns}))
+    return out
+
+
+def product_closed_forms(terms, phis):
+    """<v|H|v> and <v|H H|v> for the normalised product state of the single-site phis."""
+    def ev(k, m):
+        return np.vdot(phis[k], m @ phis[k])
+
+    e1 = sum(c * np.prod([ev(k, m) for k, m in A.items()]) for c, A in terms)
+    e2 = 0.0
+    for c, A in terms:
+        for c2, B in terms:
+            val = 1.0 + 0j
+            for k in set(A) | set(B):
+                m = A.get(k)
+                m2 = B.get(k)
+                if m is None:
+                    val *= ev(k, m2)
+                elif m2 is None:
+                    val *= ev(k, m)
+                else:
+                    val *= ev(k, m @ m2)
+            e2 += c * c2 * val
+    return e1, e2
+
+
+def upload_product(vec, phis, q, low):
+    """Writes prod_k phi_k[d_k] at index sum_k d_k q^k in chunks of q^low states."""
+    def kron_sites(ks):
+        out = np.ones(1, dtype=complex)
+        for k in ks:
+            out = np.outer(phis[k], out).ravel()   # site k is the slower index
+        return out
+
+    n = len(phis)
+    B = kron_sites(range(low))
+    A = kron_sites(range(low, n))
+    buf = np.empty_like(B)
+    for hi in range(A.size):
+        np.multiply(B, A[hi], out=buf)
+        vec.upload(buf, offset=hi * B.size)
+
+
+def random_product_sites(n, q, seed):
+    r = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        z = r.standard_normal(q) + 1j * r.standard_normal(q)
+        out.append(z / np.linalg.norm(z))
+    return out

@@ -436,3 +436,33 @@ def test_ctpq_constant_norm_series():
         got = oracle.ctpq_sample(beta, c4, np.zeros(nd1), bt)
         want = 1.7 * np.cosh(x) / (np.cosh(x) + 2.0 * np.sinh(x))
         assert abs(got - want) < 1e-12
+
+
+class _HostVec:
+    def __init__(self, d):
+        self.a = np.zeros(d, dtype=complex)
+
+    def upload(self, x, offset=0):
+        self.a[offset:offset + x.size] = x
+
+
+@pytest.mark.parametrize("spec,q,phi,h,low", [(12, 2, 126.0, 0.1, 7), ("2x3", 2, 77.0, 0.3, 6),
+                                               (8, 3, 108.0, 0.0, 5), (8, 3, 30.0, 0.2, 3)])
+def test_product_state_closed_forms(spec, q, phi, h, low):
+    """The full-size GPU checks' closed forms (<v|H|v>, <Hv|Hv> of a product state from the
+    textbook operator terms) and their chunked product-state writer, against the oracle's
+    H.v on small clusters; a field of the wrong sign must move <v|H|v>."""
+    cl = lattice.get_cluster(spec)
+    op, oop = _oop(cl, phi, q=q, h=h)
+    phis = dense.random_product_sites(cl.n, q, 5)
+    e1, e2 = dense.product_closed_forms(dense.kh_terms(cl, q, phi, h), phis)
+    v = _HostVec(q ** cl.n)
+    dense.upload_product(v, phis, q, low)
+    assert abs(np.vdot(v.a, v.a) - 1.0) < 1e-14
+    out = np.zeros_like(v.a)
+    pr = oracle.hv(oop, v.a, out)
+    assert abs(pr - e1) <= 1e-13 * np.sqrt(abs(e2))
+    assert abs(np.vdot(out, out) - e2) <= 1e-13 * abs(e2)
+    if h:
+        wrong, _ = dense.product_closed_forms(dense.kh_terms(cl, q, phi, -h), phis)
+        assert abs(pr - wrong) > 1e-6
