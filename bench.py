@@ -23,7 +23,7 @@ cpu_baseline: the oracle's restatement of the reference's per-element pass struc
 (OpenMP, all host threads) on a bounded sample -- output rows [0, 2^27) of the same
 N=32 operator, about 10-15 s on 16 cores -- reported as algorithmic H.v GB/s.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--cfg N32|N24|TPS|SERVE]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--cfg N32|N24|TPS|SERVE|...]
 """
 import argparse
 import json
@@ -48,7 +48,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1 << 27)
-    ap.add_argument("--cfg", default="N32", help="N32 (default), N24, TPS, SERVE, LANCZOS, SPIN1, SPINHV")
+    ap.add_argument("--cfg", default="N32",
+                    help="N32 (default), N24, TPS, SERVE, LANCZOS, LANCZOS32, SPIN1, SPINHV")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="", help="fixed plan K,L,threads (skips the autotuner)")
     ap.add_argument("--requests", type=int, default=120, help="SERVE: number of requests")
@@ -100,6 +101,8 @@ def main():
         return bench_spin_hv(args)
     if args.cfg == "LANCZOS":
         return bench_lanczos(args)
+    if args.cfg == "LANCZOS32":
+        return bench_lanczos32(args)
     if args.cfg == "SPIN1":
         return bench_spin1(args)
     if args.cfg == "SERVE":
@@ -278,6 +281,52 @@ def bench_lanczos(args):
         ctx.close()
     out = {"metric": "plain FP64 Lanczos ms per step (N=24, complex128)", "value": rows["24"]["ms_per_step_300"],
            "unit": "ms/step", "higher_is_better": False, "config": rows}
+    print(json.dumps(out), flush=True)
+
+
+def bench_lanczos32(args):
+    """Plain FP64 Lanczos at the single-GPU limit: S=1/2 KH honeycomb 4x4 PBC (N=32),
+    MODEL 20 phi=126 deg, [111] field h=0.1, complex128 (2^32 states, three resident
+    64 GiB vectors), seeded Gaussian start, 200 timed steps; then the Ritz vector of the
+    lowest Ritz value (a second 200-step run from the same start) and its residual."""
+    import numpy as np  # noqa: F401
+    import torch  # noqa: F401
+
+    from pned_amd import gpu, lattice, model
+
+    cl = lattice.torus(4, 4)
+    op = model.build_operator(cl, 20, 126.0, field=0.1)
+    ctx = gpu.Context(op)
+    v1, v0, x = ctx.vector(), ctx.vector(), ctx.vector()
+    seed, steps = 20261202, 200
+    ctx.set_initial_vec(v1, seed)
+    ctx.sync()
+    _progress(f"N=32 Lanczos: {steps} steps")
+    t = time.perf_counter()
+    itr, alpha, beta, ene = ctx.lnc1(v1, v0, fixed_steps=steps, nvec=1)
+    ctx.sync()
+    dt = time.perf_counter() - t
+    ms = dt / steps * 1e3
+    _progress(f"N=32 Lanczos: {ms:.2f} ms per step, lowest Ritz value / N {ene[0] / cl.n:.13f}")
+    coef = gpu.vec12(alpha, beta, [ene[0]], 1)[0]
+    ctx.set_initial_vec(v1, seed)
+    t = time.perf_counter()
+    ctx.ritz_vector(v1, v0, x, alpha, beta, coef)
+    ctx.sync()
+    tr = time.perf_counter() - t
+    res = ctx.residual(x, ene[0], v0)
+    _progress(f"N=32 Ritz vector {tr:.1f} s, residual {res:.3e}")
+    npass, _ = ctx.plan_info()
+    for v in (v1, v0, x):
+        v.free()
+    ctx.close()
+    alg = 3 * 16 * 2.0 ** 32
+    out = {"metric": "plain FP64 Lanczos ms per step (N=32, complex128)", "value": round(ms, 3), "unit": "ms/step",
+           "higher_is_better": False, "steps": steps, "dtype": "c128",
+           "config": {"workload": "S=1/2 KH honeycomb 4x4 PBC N=32, phi=0.7pi, [111] h=0.1, complex128 dim 2^32, "
+                                  "seeded Gaussian start, 200 Lanczos steps",
+                      "passes": npass, "hv_equivalent_GBps": round(alg / (ms * 1e-3) / 1e9, 1),
+                      "E0_per_site_200": float(ene[0]) / cl.n, "ritz_s": round(tr, 2), "residual": res}}
     print(json.dumps(out), flush=True)
 
 
