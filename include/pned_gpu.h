@@ -57,6 +57,8 @@ int pned_device_count(void);
 int pned_ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair,
                     const int32_t *ibond_type, const double *xlocal_ham,
                     const int32_t *llocal_ham, int device, pned_ctx **out);
+/* destroying a context detaches its remaining vectors: they keep their device memory until
+ * pned_vec_destroy and are refused by every call that takes a context */
 int pned_ctx_destroy(pned_ctx *ctx);
 /* batch context (no reference counterpart; for serving same-size requests together):
  * nop operators on the same cluster, one per column of the context's vectors (nop a power
@@ -183,6 +185,15 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
 int pned_lanczos_batch_step(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int *mode,
                             const double *alpha_prev, const double *beta_prev, double *alpha,
                             double *beta);
+/* nsteps calls of pned_lanczos_batch_step in one (no reference counterpart; for servers
+ * that check convergence every few steps): the first step takes mode, alpha_prev and
+ * beta_prev as given, each later one continues every stepped column (mode 2) from the
+ * previous step's alpha and beta, and idles (mode 0) a column whose beta fell below 0.5e-30
+ * or is not a number.  alpha and beta are nsteps x nvec (step-major); idle entries are 0.
+ * Every value equals what the single steps would give. */
+int pned_lanczos_batch_steps(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int *mode,
+                             const double *alpha_prev, const double *beta_prev, int nsteps,
+                             double *alpha, double *beta);
 /* column 'col' of v := normalised Gaussian start vector of 'seed' (the values of column 0
  * of pned_set_initial_vec for that seed, whatever the column) */
 int pned_set_initial_column(pned_ctx *ctx, pned_vec *v, int col, uint64_t seed);

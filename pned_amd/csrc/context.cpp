@@ -741,6 +741,7 @@ int pned_ctx_destroy(pned_ctx *ctx)
     if (!ctx) return PNED_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (pned_vec *v : ctx->vecs) v->ctx = nullptr;   // they stay valid for pned_vec_destroy
     if (ctx->d_terms) (void)hipFree(ctx->d_terms);
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
     if (ctx->pd_dev) (void)hipFree(ctx->pd_dev);
@@ -1005,6 +1006,7 @@ int pned_vec_create(pned_ctx *ctx, int nvec, int dtype, pned_vec **out)
         delete v;
         return fail(PNED_E_NOMEM, "device allocation of " + std::to_string(v->bytes) + " bytes failed");
     }
+    ctx->vecs.push_back(v);
     *out = v;
     return PNED_OK;
 }
@@ -1012,7 +1014,11 @@ int pned_vec_create(pned_ctx *ctx, int nvec, int dtype, pned_vec **out)
 int pned_vec_destroy(pned_vec *v)
 {
     if (!v) return PNED_OK;
-    if (v->ctx && v->ctx->stream) (void)hipStreamSynchronize(v->ctx->stream);
+    if (v->ctx) {
+        if (v->ctx->stream) (void)hipStreamSynchronize(v->ctx->stream);
+        auto &vs = v->ctx->vecs;
+        vs.erase(std::remove(vs.begin(), vs.end(), v), vs.end());
+    }
     if (v->data) (void)hipFree(v->data);
     delete v;
     return PNED_OK;
@@ -1032,6 +1038,7 @@ void *pned_vec_data(pned_vec *v) { return v ? v->data : nullptr; }
 int pned_vec_upload(pned_ctx *ctx, pned_vec *v, const void *host, int64_t offset, int64_t count)
 {
     if (!ctx || !v || !host) return fail(PNED_E_ARG, "null argument");
+    if (vec_detached(v)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (offset < 0 || count < 0 || offset + count > v->dim) return fail(PNED_E_ARG, "range outside the vector");
     const size_t eb = elem_bytes(v->dtype) * v->nvec;
     PNED_HIP(hipMemcpyAsync((char *)v->data + eb * offset, host, eb * count, hipMemcpyHostToDevice, ctx->stream));
@@ -1042,6 +1049,7 @@ int pned_vec_upload(pned_ctx *ctx, pned_vec *v, const void *host, int64_t offset
 int pned_vec_download(pned_ctx *ctx, const pned_vec *v, void *host, int64_t offset, int64_t count)
 {
     if (!ctx || !v || !host) return fail(PNED_E_ARG, "null argument");
+    if (vec_detached(v)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (offset < 0 || count < 0 || offset + count > v->dim) return fail(PNED_E_ARG, "range outside the vector");
     const size_t eb = elem_bytes(v->dtype) * v->nvec;
     PNED_HIP(hipMemcpyAsync(host, (const char *)v->data + eb * offset, eb * count, hipMemcpyDeviceToHost, ctx->stream));
@@ -1052,6 +1060,7 @@ int pned_vec_download(pned_ctx *ctx, const pned_vec *v, void *host, int64_t offs
 int pned_vec_zero(pned_ctx *ctx, pned_vec *v)
 {
     if (!ctx || !v) return fail(PNED_E_ARG, "null argument");
+    if (vec_detached(v)) return fail(PNED_E_ARG, "vector of a destroyed context");
     PNED_HIP(hipMemsetAsync(v->data, 0, v->bytes, ctx->stream));
     return PNED_OK;
 }
@@ -1059,6 +1068,7 @@ int pned_vec_zero(pned_ctx *ctx, pned_vec *v)
 int pned_vec_copy(pned_ctx *ctx, const pned_vec *src, pned_vec *dst)
 {
     if (!ctx || !src || !dst) return fail(PNED_E_ARG, "null argument");
+    if (vec_detached(src) || vec_detached(dst)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (src->bytes != dst->bytes || src->dtype != dst->dtype) return fail(PNED_E_ARG, "shape mismatch");
     PNED_HIP(hipMemcpyAsync(dst->data, src->data, src->bytes, hipMemcpyDeviceToDevice, ctx->stream));
     return PNED_OK;
@@ -1067,6 +1077,7 @@ int pned_vec_copy(pned_ctx *ctx, const pned_vec *src, pned_vec *dst)
 int pned_set_initial_vec(pned_ctx *ctx, pned_vec *v, uint64_t seed)
 {
     if (!ctx || !v) return fail(PNED_E_ARG, "null argument");
+    if (vec_detached(v)) return fail(PNED_E_ARG, "vector of a destroyed context");
     return gaussian_fill(ctx, v, seed);
 }
 
@@ -1128,6 +1139,7 @@ int pned_axpby(pned_ctx *ctx, const double *a, const pned_vec *x, const double *
 int pned_scale(pned_ctx *ctx, pned_vec *x, const double *a)
 {
     if (!ctx || !x || !a) return fail(PNED_E_ARG, "null argument");
+    if (vec_detached(x)) return fail(PNED_E_ARG, "vector of a destroyed context");
     return scale(ctx, x, a);
 }
 

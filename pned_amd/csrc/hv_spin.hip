@@ -253,6 +253,8 @@ struct SpinPassArgs {
     double shift;                     // -eperbd per bond diagonal
     double xs;                        // v0 += xs (H v1); <v1|.> then carries one factor xs
     double ys;                        // first pass: factor on the v0 input
+    double zr, zi;                    // first pass: v0 += (zr + i zi) v1 (complex diagonal shift)
+    int dotu;                         // 1: accumulate sum v1 * (H v1) without the conjugate
 };
 
 template <int Q, int TPB, int R, int S>
@@ -356,11 +358,20 @@ __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ 
                         ai = fma(cr, p.y, fma(c.y, p.x, ai));
                     }
                 }
+                const double2 x = tile[j];
+                if (a.diag_first) {
+                    ar = fma(a.zr, x.x, fma(-a.zi, x.y, ar));
+                    ai = fma(a.zr, x.y, fma(a.zi, x.x, ai));
+                }
                 ar *= a.xs;
                 ai *= a.xs;
-                const double2 x = tile[j];
-                pr_re += x.x * ar + x.y * ai;
-                pr_im += x.x * ai - x.y * ar;
+                if (a.dotu) {
+                    pr_re += x.x * ar - x.y * ai;
+                    pr_im += x.x * ai + x.y * ar;
+                } else {
+                    pr_re += x.x * ar + x.y * ai;
+                    pr_im += x.x * ai - x.y * ar;
+                }
                 v0[base + offj[r]] = make_double2(fma(ysc, yr[u], ar), fma(ysc, yi[u], ai));
             }
         }
@@ -486,6 +497,12 @@ static void launch_spin_pass(pned_ctx *ctx, int grid, size_t lds, const pned_vec
 
 int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                          double *prdct_host, double xs, double ys)
+{
+    return launch_hv_spin_tiled_shift(ctx, v1, v0, eperbd, fresh, prdct_host, xs, ys, 0.0, 0.0, false);
+}
+
+int launch_hv_spin_tiled_shift(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                               double *prdct_host, double xs, double ys, double zr, double zi, bool dotu)
 {
     const int q = ctx->q, n = ctx->n, qq = q * q;
     const int L0 = (ctx->tune_low_bits > 0) ? ctx->tune_low_bits : 2;
@@ -619,6 +636,9 @@ int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double
         a.shift = -eperbd;
         a.xs = xs;
         a.ys = ys;
+        a.zr = zr;
+        a.zi = zi;
+        a.dotu = dotu ? 1 : 0;
         const int grid = (int)std::min<uint64_t>(a.tile_count, (uint64_t)grid_cap);
         const SpinPassDev *pd = reinterpret_cast<const SpinPassDev *>(ctx->spin_pd) + p;
         double *part = ctx->d_partial + 2 * (size_t)rows;

@@ -134,8 +134,12 @@ struct pned_vec {
     size_t bytes = 0;
 };
 
+// a vector whose context was destroyed keeps its memory but takes part in no operation
+inline bool vec_detached(const pned_vec *v) { return v != nullptr && v->ctx == nullptr; }
+
 struct pned_ctx {
     int device = 0;
+    std::vector<pned_vec *> vecs;    // live vectors; detached (ctx = nullptr) when the context goes
     int n = 0, q = 2, ibond = 0;
     int64_t dim = 0;
     hipStream_t stream = nullptr;
@@ -241,6 +245,10 @@ int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperb
 // pass's input; <v1|.> carries the factor xs)
 int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                          double *prdct_host, double xs, double ys);
+// the same with v0 = ys v0 + xs (H - shift + z) v1, z = zr + i zi, and with dotu the
+// returned sum is v1^T (.) instead of v1^H (.) (shifted complex-symmetric solves)
+int launch_hv_spin_tiled_shift(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                               double *prdct_host, double xs, double ys, double zr, double zi, bool dotu);
 
 // reductions return host values (synchronizing)
 int dot(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out /*2*nvec*/);

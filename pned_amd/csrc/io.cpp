@@ -126,6 +126,7 @@ constexpr size_t STAGE = (size_t)64 << 20;
 
 int write_vec(FILE *f, const pned_vec *v)
 {
+    if (vec_detached(v)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (v->dtype != PNED_C128 || v->nvec != 1)
         return fail(PNED_E_UNSUPPORTED, "vector files hold one complex128 column");
     PNED_HIP(hipSetDevice(v->ctx->device));
@@ -150,6 +151,7 @@ int write_vec(FILE *f, const pned_vec *v)
 
 int read_vec(FILE *f, pned_vec *v)
 {
+    if (vec_detached(v)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (v->dtype != PNED_C128 || v->nvec != 1)
         return fail(PNED_E_UNSUPPORTED, "vector files hold one complex128 column");
     PNED_HIP(hipSetDevice(v->ctx->device));

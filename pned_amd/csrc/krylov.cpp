@@ -335,11 +335,21 @@ int pned_cocg(pned_ctx *ctx, double e0, double eta, const pned_vec *b, pned_vec 
     PNED_TRY(pned_vec_copy(ctx, b, p));
     double rho[2];
     PNED_TRY(dotu(ctx, r, r, rho));
+    // tiled spin-S kernel: the i eta shift and p^T y ride on the H.v itself
+    // (other shapes take the generic path, whose calls report what is wrong)
+    const bool fused = ctx->q >= 3 && ctx->q <= 5 && ctx->spin_tiled && ctx->sector_k < 0 && p->ctx == ctx &&
+                       y->ctx == ctx && p != y && p->nvec == 1 && y->nvec == 1 && p->dtype == PNED_C128 &&
+                       y->dtype == PNED_C128 && p->dim == ctx->dim && y->dim == ctx->dim;
     for (int it = 1; it <= max_it; ++it) {
         double pr[2], mu[2], rho2[2], rr[2];
-        PNED_TRY(pned_hv_fresh(ctx, p, y, eperbd, pr));      // y = (H - e0) p
-        PNED_TRY(pned_axpby(ctx, ieta, p, one, y));          // y += i eta p
-        PNED_TRY(dotu(ctx, p, y, mu));
+        if (fused) {
+            // y = (H - e0 + i eta) p, mu = p^T y
+            PNED_TRY(launch_hv_spin_tiled_shift(ctx, p, y, eperbd, true, mu, 1.0, 1.0, 0.0, eta, true));
+        } else {
+            PNED_TRY(pned_hv_fresh(ctx, p, y, eperbd, pr));      // y = (H - e0) p
+            PNED_TRY(pned_axpby(ctx, ieta, p, one, y));          // y += i eta p
+            PNED_TRY(dotu(ctx, p, y, mu));
+        }
         const std::complex<double> al = std::complex<double>(rho[0], rho[1]) / std::complex<double>(mu[0], mu[1]);
         const double pa[2] = {al.real(), al.imag()};
         double sums[4];
@@ -402,6 +412,28 @@ int pned_lanczos_batch_step(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int
     for (int c = 0; c < nc; ++c) {
         alpha[c] = mode[c] ? pr[2 * c] : 0.0;
         beta[c] = mode[c] ? std::sqrt(b2[c]) : 0.0;
+    }
+    return PNED_OK;
+}
+
+int pned_lanczos_batch_steps(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int *mode, const double *alpha_prev,
+                             const double *beta_prev, int nsteps, double *alpha, double *beta)
+{
+    if (!ctx || !v1 || !v0 || !mode || !alpha_prev || !beta_prev || !alpha || !beta)
+        return fail(PNED_E_ARG, "null argument");
+    if (nsteps < 1) return fail(PNED_E_ARG, "nsteps must be positive");
+    const int nc = v1->nvec;
+    std::vector<int> m(mode, mode + nc);
+    std::vector<double> ap(alpha_prev, alpha_prev + nc), bp(beta_prev, beta_prev + nc);
+    for (int s = 0; s < nsteps; ++s) {
+        double *a = alpha + (size_t)s * nc, *b = beta + (size_t)s * nc;
+        PNED_TRY(pned_lanczos_batch_step(ctx, v1, v0, m.data(), ap.data(), bp.data(), a, b));
+        for (int c = 0; c < nc; ++c) {
+            if (m[c] == 0) continue;
+            ap[c] = a[c];
+            bp[c] = b[c];
+            m[c] = (b[c] >= 0.5e-30) ? 2 : 0;
+        }
     }
     return PNED_OK;
 }

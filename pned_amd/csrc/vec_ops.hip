@@ -736,6 +736,7 @@ int finish_reduce(pned_ctx *ctx, int nrows, int width, double *out)
 
 int same_shape(const pned_vec *a, const pned_vec *b)
 {
+    if (vec_detached(a) || vec_detached(b)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (!a || !b || a->dim != b->dim || a->nvec != b->nvec || a->dtype != b->dtype)
         return fail(PNED_E_ARG, "vector shapes or types differ");
     return PNED_OK;
@@ -1027,6 +1028,7 @@ int lanczos_slots(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double *alpha
 
 int gaussian_column(pned_ctx *ctx, pned_vec *v, int col, uint64_t seed)
 {
+    if (vec_detached(v)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (col < 0 || col >= v->nvec) return fail(PNED_E_ARG, "column outside the vector");
     const int grid = RED_GRID;
     PNED_TRY(ensure_partial(ctx, (size_t)grid));

@@ -116,6 +116,7 @@ def lib():
     L.pned_ctx_create_batch.argtypes = [i32, i32, i32, _ip, _ip, i32, _dp, _ip, i32, C.POINTER(P)]
     L.pned_ctx_set_column_matrices.argtypes = [P, i32, _dp, _ip]
     L.pned_lanczos_batch_step.argtypes = [P, P, P, _ip, _dp, _dp, _dp, _dp]
+    L.pned_lanczos_batch_steps.argtypes = [P, P, P, _ip, _dp, _dp, i32, _dp, _dp]
     L.pned_set_initial_column.argtypes = [P, P, i32, u64]
     L.pned_s12_plan_export.argtypes = [i32, i32, _ip, _ip, _dp, _ip, i32, i32, i32, C.POINTER(i32),
                                        C.POINTER(i32), C.POINTER(i32), _ip, _ip, _dp,
@@ -221,6 +222,16 @@ class Context:
                                             np.ascontiguousarray(alpha_prev, dtype=float),
                                             np.ascontiguousarray(beta_prev, dtype=float), a, b))
         return a, b
+
+    def lanczos_batch_steps(self, v1: "Vector", v0: "Vector", mode, alpha_prev, beta_prev, nsteps: int):
+        """nsteps batch steps in one call; alpha, beta as (nsteps, nvec) arrays."""
+        nc = v1.nvec
+        a = np.zeros(nsteps * nc)
+        b = np.zeros(nsteps * nc)
+        check(lib().pned_lanczos_batch_steps(self.h, v1.h, v0.h, np.ascontiguousarray(mode, dtype=np.int32),
+                                             np.ascontiguousarray(alpha_prev, dtype=float),
+                                             np.ascontiguousarray(beta_prev, dtype=float), int(nsteps), a, b))
+        return a.reshape(nsteps, nc), b.reshape(nsteps, nc)
 
     # -- management -------------------------------------------------------------------
     def vector(self, nvec: int = 1, dtype: int = C128) -> Vector:

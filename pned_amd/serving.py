@@ -124,31 +124,41 @@ class Pool:
         self.beta[c] = []
 
     def step(self, clock) -> List[Result]:
-        a, b = self.ctx.lanczos_batch_step(self.v1, self.v0, self.mode, self.aprev, self.bprev)
+        """Steps every busy slot up to its next convergence check (one library call; slots
+        are admitted between calls, so each slot's step count is then a multiple of
+        check_every unless it stopped early) and retires the slots that finished."""
+        k = self.check_every
+        for c in range(self.slots):
+            if self.req[c] is not None:
+                k = min(k, self.max_steps - len(self.alpha[c]))
+        A, B = self.ctx.lanczos_batch_steps(self.v1, self.v0, self.mode, self.aprev, self.bprev, max(k, 1))
         done = []
         for c in range(self.slots):
             if self.req[c] is None:
                 continue
-            self.alpha[c].append(a[c])
-            self.beta[c].append(b[c])
-            self.aprev[c], self.bprev[c] = a[c], b[c]
-            self.mode[c] = 2
-            m = len(self.alpha[c])
-            finished = False
-            if b[c] < 0.5e-30:
-                e0, res = lowest_ritz(self.alpha[c], self.beta[c])
-                finished = True
-            elif m >= self.check_every * 2 and m % self.check_every == 0 or m >= self.max_steps:
-                e0, res = lowest_ritz(self.alpha[c], self.beta[c])
-                finished = res < self.tol or m >= self.max_steps
-            if finished:
-                r = self.req[c]
-                done.append(Result(r.rid, r.spec, lattice.get_cluster(r.spec).n, e0, m, res, r.arrival,
-                                   self.start[c], clock(), np.array(self.alpha[c]), np.array(self.beta[c])))
-                self.req[c] = None
-                self.mode[c] = 0
-                self.aprev[c] = 0.0
-                self.bprev[c] = 0.0
+            for s in range(A.shape[0]):
+                a, b = A[s, c], B[s, c]
+                self.alpha[c].append(a)
+                self.beta[c].append(b)
+                self.aprev[c], self.bprev[c] = a, b
+                self.mode[c] = 2
+                m = len(self.alpha[c])
+                finished = False
+                if not b >= 0.5e-30:
+                    e0, res = lowest_ritz(self.alpha[c], self.beta[c])
+                    finished = True
+                elif m >= self.check_every * 2 and m % self.check_every == 0 or m >= self.max_steps:
+                    e0, res = lowest_ritz(self.alpha[c], self.beta[c])
+                    finished = res < self.tol or m >= self.max_steps
+                if finished:
+                    r = self.req[c]
+                    done.append(Result(r.rid, r.spec, lattice.get_cluster(r.spec).n, e0, m, res, r.arrival,
+                                       self.start[c], clock(), np.array(self.alpha[c]), np.array(self.beta[c])))
+                    self.req[c] = None
+                    self.mode[c] = 0
+                    self.aprev[c] = 0.0
+                    self.bprev[c] = 0.0
+                    break
         return done
 
     def close(self):

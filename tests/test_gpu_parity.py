@@ -340,6 +340,27 @@ def test_cocg_complex_shift_spin1():
     assert np.linalg.norm(xs - xd) <= 1e-8 * np.linalg.norm(xd)
 
 
+def test_cocg_fused_shift_equals_generic_path():
+    """The tiled spin-S kernel folds i eta p and p^T y into the H.v; the gather kernel
+    takes the generic path (H.v, axpby, separate p^T y): same recurrence, same answer."""
+    _say("cocg spin1 n12 fused vs generic")
+    cl = lattice.reference_cluster(12)
+    op = model.build_operator(cl, 20, 108.0, q=3)
+    out = []
+    for tiled in (True, False):
+        ctx = gpu.Context(op)
+        ctx.spin_kernel(tiled)
+        b, x, r, p, y = (ctx.vector() for _ in range(5))
+        ctx.set_initial_vec(b, 7)
+        it, res = ctx.cocg(-5.0, 0.2, b, x, r, p, y, max_it=4000, tol=1e-10)
+        out.append((it, res, x.download()))
+        ctx.close()
+    (it1, r1, x1), (it2, r2, x2) = out
+    assert r1 < 1e-10 and r2 < 1e-10
+    assert abs(it1 - it2) <= 2
+    assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x2)
+
+
 def test_fortran_host_driver(tmp_path):
     """The ISO_C_BINDING layer end to end: a Fortran host reads the data_mem tables and
     runs lnc1 + p_mltply_v3 through pned_parallelized_diag."""
@@ -591,3 +612,24 @@ def test_ctpq_from_gpu_mtpq():
         RC.append(c_ref.T)
     want = oracle.ctpq(np.array(RA), np.array(RB), np.array(RC), 0.1, 10.0, 30)
     assert np.allclose(got[:, :2], want[:, :2], rtol=1e-7, atol=1e-9)
+
+
+def test_vectors_outlive_their_context():
+    """pned_ctx_destroy detaches live vectors: freeing them later is safe and another
+    context refuses them."""
+    _say("vectors after their context is destroyed")
+    op = model.build_operator(lattice.get_cluster(8), 20, 54.0)
+    ctx = gpu.Context(op)
+    v, w = ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v, 1)
+    ctx.close()
+    other = gpu.Context(op)
+    with pytest.raises(gpu.PnedError):
+        other.calc_vec_inprdct(v, v)
+    v.free()
+    w.free()
+    u = other.vector()
+    other.set_initial_vec(u, 1)
+    assert abs(other.calc_vec_inprdct(u, u) - 1.0) < 1e-14
+    other.close()
+    u.free()

@@ -59,6 +59,40 @@ def test_batch_hv_per_column_operators():
     assert np.linalg.norm(v0.download()[:, 2] - want) <= 1e-12 * np.linalg.norm(want)
 
 
+@pytest.mark.parametrize("slots", [1, 4])
+def test_batch_steps_equal_single_steps(slots):
+    """pned_lanczos_batch_steps gives bit for bit the alpha/beta of the same number of
+    single steps (batch context with an idle column, and one column of a plain context)."""
+    _say(f"batch steps vs single steps, {slots} slots")
+    reqs = serving.make_requests(slots, ["2x4"], seed=9)
+    runs = []
+    for chunked in (False, True):
+        pool = serving.Pool("2x4", slots, 1e-10, 300)
+        for c, r in enumerate(reqs):
+            if slots == 1 or c != 2:
+                pool.admit(c, r, 0.0)
+        mode, ap, bp = pool.mode.copy(), pool.aprev.copy(), pool.bprev.copy()
+        if chunked:
+            A, B = pool.ctx.lanczos_batch_steps(pool.v1, pool.v0, mode, ap, bp, 7)
+        else:
+            rows_a, rows_b = [], []
+            for _ in range(7):
+                a, b = pool.ctx.lanczos_batch_step(pool.v1, pool.v0, mode, ap, bp)
+                rows_a.append(a)
+                rows_b.append(b)
+                mode = np.where(mode != 0, 2, 0).astype(np.int32)
+                ap, bp = a, b
+            A, B = np.array(rows_a), np.array(rows_b)
+        runs.append((A, B))
+        pool.close()
+    (A1, B1), (A2, B2) = runs
+    assert A1.shape == A2.shape == (7, slots)
+    assert np.array_equal(A1, A2) and np.array_equal(B1, B2)
+    if slots > 1:
+        assert not A2[:, 2].any() and not B2[:, 2].any()
+    assert (B2[:, 0] > 0).all()
+
+
 def test_alone_equals_in_the_mix():
     _say("batch invariance")
     reqs = serving.make_requests(8, ["3x3"], seed=5)
