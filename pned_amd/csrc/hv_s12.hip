@@ -39,6 +39,12 @@ struct PassArgs {
     const double *yscale;  // diagonal pass: per-column factor on the v0 input (v0 = y v0 + ...); null = 1
     int ncat[4];           // term-major kernel: terms per category (see build_tm_terms)
     int ngroups;           // diagonal groups (batch contexts stage their weights)
+    // term-major kernel, last pass of a deferred-normalisation Lanczos step:
+    //   1  only the <v1|H_pass v1> partials (no v0 traffic)
+    //   2  v0 := w - Re(alpha) s v1 with w = v0 + H_pass (s v1), alpha = s * lz_prd[c],
+    //      partials [|w - alpha s v1|^2, 0]
+    int lz_mode;
+    const double *lz_prd;
 };
 
 // Per-pass address tables in device memory (read with scalar loads):
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
     V *tile = reinterpret_cast<V *>(smem_raw);
     const int tid = threadIdx.x;
     const uint64_t offLo = pd->lo[tid];
-    const bool rd0 = !(a.diag && a.fresh);
+    const bool rd0 = !(a.diag && a.fresh) && a.lz_mode != 1;
     const uint64_t lomask = (1ull << a.nlo) - 1;
     auto outer_base = [&](uint64_t tix) { return pd->blo[tix & lomask] | pd->bhi[tix >> a.nlo]; };
     constexpr uint32_t RSTRIDE = (uint32_t)(TPB * sizeof(V));   // bytes between register slots
@@ -347,6 +353,11 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
     }
     const A colsc = a.colscale ? (A)a.colscale[tid & (a.ncol - 1)] : (A)1;
     const A ys = (a.diag && a.yscale) ? (A)a.yscale[tid & (a.ncol - 1)] : (A)1;
+    double lar = 0.0, lai = 0.0;
+    if (a.lz_mode == 2) {
+        lar = (double)colsc * a.lz_prd[2 * (tid & (a.ncol - 1))];
+        lai = (double)colsc * a.lz_prd[2 * (tid & (a.ncol - 1)) + 1];
+    }
 
     double pr_re = 0.0, pr_im = 0.0;
     Real xr[R], xi[R];
@@ -499,18 +510,33 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
             }
             static_for<S>([&](auto I) {
                 const V x = tile[tid + TPB * (r0 + I)];
-                const double hr = (double)ar[I], hi = (double)ai[I];
-                pr_re += (double)x.x * hr + (double)x.y * hi;
-                pr_im += (double)x.x * hi - (double)x.y * hr;
-                V o;
-                if (rd0) {
-                    o.x = (Real)fma(ys, (A)ycr[I], ar[I]);
-                    o.y = (Real)fma(ys, (A)yci[I], ai[I]);
+                if (a.lz_mode == 2) {
+                    const double wr0 = (double)fma(ys, (A)ycr[I], ar[I]);
+                    const double wi0 = (double)fma(ys, (A)yci[I], ai[I]);
+                    const double xr_ = (double)x.x * (double)colsc, xi_ = (double)x.y * (double)colsc;
+                    const double wr = wr0 - (lar * xr_ - lai * xi_);
+                    const double wi = wi0 - (lar * xi_ + lai * xr_);
+                    pr_re = fma(wr, wr, fma(wi, wi, pr_re));
+                    V o;
+                    o.x = (Real)(wr0 - lar * xr_);
+                    o.y = (Real)(wi0 - lar * xi_);
+                    v0[tb0 | pd->hi[r0 + I]] = o;
                 } else {
-                    o.x = (Real)ar[I];
-                    o.y = (Real)ai[I];
+                    const double hr = (double)ar[I], hi = (double)ai[I];
+                    pr_re += (double)x.x * hr + (double)x.y * hi;
+                    pr_im += (double)x.x * hi - (double)x.y * hr;
+                    if (a.lz_mode != 1) {
+                        V o;
+                        if (rd0) {
+                            o.x = (Real)fma(ys, (A)ycr[I], ar[I]);
+                            o.y = (Real)fma(ys, (A)yci[I], ai[I]);
+                        } else {
+                            o.x = (Real)ar[I];
+                            o.y = (Real)ai[I];
+                        }
+                        v0[tb0 | pd->hi[r0 + I]] = o;
+                    }
                 }
-                v0[tb0 | pd->hi[r0 + I]] = o;
             });
         });
         __syncthreads();
@@ -683,8 +709,9 @@ static void build_tm_terms(const S12Plan &pl, const PassPlan &pp, int TPB, int c
 }
 
 int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
-                  double *prdct_host, const double *d_colscale, const double *d_yscale)
+                  double *prdct_host, const double *d_colscale, const double *d_yscale, bool *lz_fused)
 {
+    if (lz_fused) *lz_fused = false;
     const int nvec = v1->nvec;
     int colbits = 0;
     while ((1 << colbits) < nvec) ++colbits;
@@ -696,10 +723,10 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         return fail(PNED_E_ARG, "a batch context's vectors need one column per operator");
     const int npass = (int)pl.passes.size();
     const int grid_cap = 4096;
-    // partial layout: [pass][block][2*ncol]
-    size_t need = (size_t)npass * grid_cap * 2 * nvec + 64;
+    // partial layout: [pass][block][2*ncol] (plus one pass for a fused Lanczos step)
+    size_t need = (size_t)(npass + 1) * grid_cap * 2 * nvec + 64;
     PNED_TRY(ensure_partial(ctx, need));
-    PNED_TRY(ensure_result(ctx, 2 * (size_t)nvec + 8));
+    PNED_TRY(ensure_result(ctx, 4 * (size_t)nvec + 8));
     // per-pass address tables (rebuilt when the plan or the thread count changes)
     const int TPBpref = (ctx->tune_threads == 256 || ctx->tune_threads == 512 || ctx->tune_threads == 1024)
                             ? ctx->tune_threads
@@ -791,8 +818,29 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         int rc;
         const bool c128 = v1->dtype == PNED_C128;
         const size_t tile_bytes = (c128 ? 16 : 8) * (size_t)TPB * R;
-        if (ctx->s12_kernel == 1 && (c128 ? has_tm<double>(TPB, R) : has_tm<float>(TPB, R)) &&
-            tile_bytes + tm_coef_bytes(a) <= LDS_LIMIT) {
+        const bool tm = ctx->s12_kernel == 1 && (c128 ? has_tm<double>(TPB, R) : has_tm<float>(TPB, R)) &&
+                        tile_bytes + tm_coef_bytes(a) <= LDS_LIMIT;
+        if (lz_fused && p == npass - 1 && p > 0 && tm && c128 && nvec == 1 && pl.nop <= 1) {
+            // deferred-normalisation Lanczos step: this pass's <v1|H v1> share first (no v0
+            // traffic), alpha from all passes on the device, then the pass itself forms
+            // v0 - Re(alpha) s v1 and the partials of beta^2
+            a.lz_mode = 1;
+            PNED_TRY(dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid));
+            total_rows += grid;
+            hipLaunchKernelGGL(reduce_partials, dim3(2 * nvec), dim3(256), 0, ctx->stream, ctx->d_partial, total_rows,
+                               2 * nvec, ctx->d_result, 0);
+            PNED_HIP(hipGetLastError());
+            a.lz_mode = 2;
+            a.lz_prd = ctx->d_result;
+            part = ctx->d_partial + (size_t)total_rows * 2 * nvec;
+            PNED_TRY(dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid));
+            hipLaunchKernelGGL(reduce_partials, dim3(2 * nvec), dim3(256), 0, ctx->stream, part, grid, 2 * nvec,
+                               ctx->d_result + 2 * nvec, 0);
+            PNED_HIP(hipGetLastError());
+            *lz_fused = true;
+            return PNED_OK;
+        }
+        if (tm) {
             // the term table is in pass order, so the kernel indexes it by first_term as well
             if (c128)
                 rc = dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid);

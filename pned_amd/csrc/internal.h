@@ -237,8 +237,12 @@ int reduce_rows_result(pned_ctx *ctx, int nrows, int ncol2);
 // d_colscale: optional device array of per-column factors s_c: v0 += (H - shift)(s_c v1_c);
 // prdct is then <v1|H|s v1> (one factor s)
 // d_yscale: optional per-column factors y_c on the v0 input: v0 = y_c v0 + (H - shift)(s_c v1)
+// lz_fused (deferred-normalisation Lanczos step, one complex128 column): when not null and
+// the last pass runs on the term-major kernel, that pass also forms v0 - Re(alpha) s v1;
+// then d_result holds <v1|H s v1> (2 doubles) followed by [beta^2, 0] and *lz_fused = true
 int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
-                  double *prdct_host, const double *d_colscale = nullptr, const double *d_yscale = nullptr);
+                  double *prdct_host, const double *d_colscale = nullptr, const double *d_yscale = nullptr,
+                  bool *lz_fused = nullptr);
 int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                    double *prdct_host);
 // the tiled spin-S kernel (q = 3..5) with v0 = ys v0 + xs (H - shift) v1 (ys on the first

@@ -794,10 +794,19 @@ int lanczos_lazy_step(pned_ctx *ctx, pned_vec *x, pned_vec *y, const double *sx,
     }
     PNED_HIP(hipMemcpyAsync(d, up.data(), sizeof(double) * 2 * nc, hipMemcpyHostToDevice, ctx->stream));
     // y = sy y + H (sx x); the H.v's <x|H|sx x> lands in d_result
+    bool fused = false;
     if (spin)
         PNED_TRY(launch_hv_spin_tiled(ctx, x, y, 0.0, fresh, nullptr, sx[0], fresh ? 1.0 : sy[0]));
     else
-        PNED_TRY(launch_hv_s12(ctx, x, y, 0.0, fresh, nullptr, d, fresh ? nullptr : d + nc));
+        PNED_TRY(launch_hv_s12(ctx, x, y, 0.0, fresh, nullptr, d, fresh ? nullptr : d + nc, &fused));
+    if (fused) {   // the last pass formed y - Re(alpha) sx x and beta^2 already
+        PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double) * 4 * nc, hipMemcpyDeviceToHost,
+                                ctx->stream));
+        PNED_HIP(hipStreamSynchronize(ctx->stream));
+        for (int c = 0; c < 2 * nc; ++c) pr[c] = ctx->h_result[c];
+        for (int c = 0; c < nc; ++c) b2[c] = ctx->h_result[2 * nc + 2 * c];
+        return PNED_OK;
+    }
     PNED_TRY(ensure_partial(ctx, (size_t)grid * nc));
     double *dprd = d + 2 * nc;
     PNED_HIP(hipMemcpyAsync(dprd, ctx->d_result, sizeof(double) * 2 * nc, hipMemcpyDeviceToDevice, ctx->stream));
