@@ -169,8 +169,9 @@ def main():
         with open(tfile) as fh:
             prof = json.load(fh)
         plan = [tune.get("tile_bits"), tune.get("low_bits"), tune.get("threads")] if tune else None
-        if plan is not None and prof.get("tune") == plan and prof.get("kernel") == "hv_s12_tm":
-            traffic = int(prof["hbm_bytes_per_hv"])
+        for entry in prof.get("plans", []):
+            if plan is not None and entry.get("tune") == plan and prof.get("kernel") == "hv_s12_tm":
+                traffic = int(entry["hbm_bytes_per_hv"])
 
     out = {
         "metric": "N=32 H·v HBM GB/s (% peak); batched/mixed solves/s + p99 vs plain @1e-10",
