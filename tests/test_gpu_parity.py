@@ -633,3 +633,35 @@ def test_vectors_outlive_their_context():
     assert abs(other.calc_vec_inprdct(u, u) - 1.0) < 1e-14
     other.close()
     u.free()
+
+
+def test_argument_errors_leave_the_context_usable():
+    """Bad calls come back as PnedError with the library's code, before any kernel runs,
+    and the context keeps working afterwards."""
+    _say("argument errors")
+    op = model.build_operator(lattice.get_cluster(12), 20, 126.0, field=0.1)
+    ctx = gpu.Context(op)
+    v1, v0 = ctx.vector(), ctx.vector()
+    wide = ctx.vector(2)
+    three = ctx.vector(3)
+    small = ctx.vector(1, gpu.C64)
+    ctx.set_initial_vec(v1, 3)
+    cases = [
+        (lambda: ctx.p_mltply_v3(v1, v1), gpu.E_ARG),                       # v1 and v0 the same vector
+        (lambda: ctx.p_mltply_v3(v1, wide), gpu.E_ARG),                     # column counts differ
+        (lambda: ctx.axpby(1.0, v1, 1.0, small), gpu.E_ARG),                # storage types differ
+        (lambda: ctx.p_mltply_v3(three, ctx.vector(3)), gpu.E_UNSUPPORTED),  # 3 columns
+        (lambda: ctx.intalv_xy(v1, v0, [0.0, 0.0], np.zeros((12, 2)), ipm=2), gpu.E_ARG),
+        (lambda: ctx.lanczos_batch_steps(v1, v0, np.ones(1, np.int32), np.zeros(1), np.zeros(1), 0), gpu.E_ARG),
+        (lambda: ctx.cocg(-3.0, 0.1, v1, v0, *(ctx.vector() for _ in range(3))), gpu.E_UNSUPPORTED),  # complex H
+    ]
+    for k, (call, code) in enumerate(cases):
+        with pytest.raises(gpu.PnedError) as err:
+            call()
+        assert err.value.code == code, (k, err.value)
+    v0.zero()
+    pr = ctx.p_mltply_v3(v1, v0)
+    want = np.zeros(ctx.dim, complex)
+    p_ref = oracle.hv(oracle.OracleOp(op), v1.download(), want)
+    assert np.linalg.norm(v0.download() - want) <= 1e-12 * np.linalg.norm(want)
+    assert abs(pr - p_ref) <= 1e-12 * abs(p_ref)
