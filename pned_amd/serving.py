@@ -298,6 +298,14 @@ def make_requests(count: int, specs, seed: int = 20261202, rate: Optional[float]
 
 def summarize(results: List[Result], wall: float) -> dict:
     lat = np.array([r.latency for r in results])
+    per_size = {}
+    for spec in sorted({r.spec for r in results}):
+        ls = np.array([r.latency for r in results if r.spec == spec])
+        ss = np.array([r.finish - r.start for r in results if r.spec == spec])
+        per_size[spec] = {"n": int(ls.size), "p50_s": round(float(np.percentile(ls, 50)), 4),
+                          "p99_s": round(float(np.percentile(ls, 99)), 4),
+                          "mean_service_s": round(float(ss.mean()), 4)}
     return {"solves": len(results), "wall_s": round(wall, 3), "solves_per_s": round(len(results) / wall, 3),
             "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
-            "max_resid": float(max(r.resid for r in results)), "mean_steps": float(np.mean([r.steps for r in results]))}
+            "max_resid": float(max(r.resid for r in results)), "mean_steps": float(np.mean([r.steps for r in results])),
+            "per_size": per_size}
