@@ -439,10 +439,12 @@ static std::vector<std::vector<int>> spin_cover(int n, const std::vector<int32_t
 }
 
 // Spreads the bonds over the passes whose tiles hold both of their sites: bonds with the
-// fewest covering passes are placed first, each on the pass with the fewest bonds so far,
-// so that no pass carries many more (bond, state) rows than the others. Passes left
-// without bonds are dropped.
-static void spin_balance(std::vector<std::vector<int>> &cov, const std::vector<int32_t> &sp, int n)
+// fewest covering passes are placed first, each on the pass with the fewest nonzero
+// elements so far (w[b]: elements of bond b's matrix that are not skipped), so that no pass
+// carries many more (bond, state) elements than the others. Passes left without bonds are
+// dropped.
+static void spin_balance(std::vector<std::vector<int>> &cov, const std::vector<int32_t> &sp, int n,
+                         const std::vector<int> &w)
 {
     const int np = (int)cov.size() / 2;
     std::vector<std::vector<char>> has(np, std::vector<char>(n, 0));
@@ -460,11 +462,13 @@ static void spin_balance(std::vector<std::vector<int>> &cov, const std::vector<i
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return choice[a].size() < choice[b].size(); });
     std::vector<std::vector<int>> bonds(np);
+    std::vector<long> load(np, 0);
     for (size_t i : order) {
         int best = choice[i][0];
         for (int p : choice[i])
-            if (bonds[p].size() < bonds[best].size()) best = p;
+            if (load[p] < load[best]) best = p;
         bonds[best].push_back(all[i]);
+        load[best] += w[all[i]];
     }
     std::vector<std::vector<int>> out;
     for (int p = 0; p < np; ++p) {
@@ -520,7 +524,14 @@ int launch_hv_spin_tiled_shift(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, 
             const int L = std::min(L0, K);
             auto cov = spin_cover(n, ctx->site_pair, ctx->ibond, K, L);
             if (cov.empty()) return fail(PNED_E_UNSUPPORTED, "spin-S tile planner made no progress");
-            spin_balance(cov, ctx->site_pair, n);
+            {
+                std::vector<int> w(ctx->ibond, 0);
+                for (int b = 0; b < ctx->ibond; ++b) {
+                    const size_t t = (size_t)(ctx->btype[b] - 1);
+                    for (int e = 0; e < qq * qq; ++e) w[b] += ctx->ll[(size_t)e + (size_t)qq * qq * t] ? 0 : 1;
+                }
+                spin_balance(cov, ctx->site_pair, n, w);
+            }
             const int np = (int)cov.size() / 2;
             host.assign(np, SpinPassDev{});
             nb.assign(np, 0);
