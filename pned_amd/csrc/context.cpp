@@ -622,7 +622,7 @@ int upload_batch_coefficients(pned_ctx *ctx)
     return PNED_OK;
 }
 
-static int hv_dispatch(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh, double *prdct)
+int check_hv_pair(const pned_ctx *ctx, const pned_vec *v1, const pned_vec *v0)
 {
     if (!ctx || !v1 || !v0) return fail(PNED_E_ARG, "null argument");
     if (v1->ctx != ctx || v0->ctx != ctx) return fail(PNED_E_ARG, "vector belongs to another context");
@@ -630,6 +630,12 @@ static int hv_dispatch(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double e
         return fail(PNED_E_ARG, "v1 and v0 differ in shape or type");
     if (v1 == v0) return fail(PNED_E_ARG, "v1 and v0 must be different vectors");
     if (v1->dim != ctx->dim) return fail(PNED_E_ARG, "vector dimension differs from the context's basis");
+    return PNED_OK;
+}
+
+static int hv_dispatch(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh, double *prdct)
+{
+    PNED_TRY(check_hv_pair(ctx, v1, v0));
     if (ctx->sector_k >= 0) return launch_hv_sector(ctx, v1, v0, eperbd, fresh, prdct);
     if (ctx->q == 2) return launch_hv_s12(ctx, v1, v0, eperbd, fresh, prdct);
     return launch_hv_spin(ctx, v1, v0, eperbd, fresh, prdct);
@@ -870,7 +876,7 @@ int pned_ctx_tune(pned_ctx *ctx, int tile_bits, int low_bits, int threads)
 int pned_ctx_autotune(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, int reps, int *best_tile_bits,
                       int *best_low_bits, int *best_threads, double *best_ms)
 {
-    if (!ctx || !v1 || !v0) return fail(PNED_E_ARG, "null argument");
+    PNED_TRY(check_hv_pair(ctx, v1, v0));
     if (ctx->q != 2) {
         if (best_ms) *best_ms = 0.0;
         return PNED_OK;   // spin-S kernel has no tile knobs

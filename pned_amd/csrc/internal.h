@@ -195,6 +195,9 @@ struct pned_ctx {
 };
 
 namespace pned {
+// the H.v operand checks: both vectors of this context, same shape, the context's
+// dimension, and two different buffers (every entry that launches an H.v calls it first)
+int check_hv_pair(const pned_ctx *ctx, const pned_vec *v1, const pned_vec *v0);
 // scratch management
 int ensure_partial(pned_ctx *ctx, size_t ndoubles);
 int ensure_result(pned_ctx *ctx, size_t ndoubles);

@@ -388,21 +388,22 @@ int pned_lanczos_batch_step(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const int
             alpha[0] = beta[0] = 0.0;
             return PNED_OK;
         }
-        if (mode[0] == 1) {
-            ctx->lz_swapped = 0;
-            ctx->lz_sx = 1.0;
-            ctx->lz_sy = 0.0;
-        } else {
+        // the new buffer roles and factors are kept only once the step has run
+        int swapped = 0;
+        double sx = 1.0, sy = 0.0;
+        if (mode[0] != 1) {
             if (!(beta_prev[0] > 0.0)) return fail(PNED_E_ARG, "stepping a column with beta <= 0");
-            const double nsx = 1.0 / beta_prev[0], nsy = -beta_prev[0] * ctx->lz_sx;
-            ctx->lz_swapped ^= 1;
-            ctx->lz_sx = nsx;
-            ctx->lz_sy = nsy;
+            swapped = ctx->lz_swapped ^ 1;
+            sx = 1.0 / beta_prev[0];
+            sy = -beta_prev[0] * ctx->lz_sx;
         }
-        pned_vec *X = ctx->lz_swapped ? v0 : v1, *Y = ctx->lz_swapped ? v1 : v0;
+        pned_vec *X = swapped ? v0 : v1, *Y = swapped ? v1 : v0;
         double pr[2], b2;
-        PNED_TRY(lanczos_lazy_step(ctx, X, Y, &ctx->lz_sx, &ctx->lz_sy, mode[0] == 1, pr, &b2));
-        alpha[0] = ctx->lz_sx * pr[0];
+        PNED_TRY(lanczos_lazy_step(ctx, X, Y, &sx, &sy, mode[0] == 1, pr, &b2));
+        ctx->lz_swapped = swapped;
+        ctx->lz_sx = sx;
+        ctx->lz_sy = sy;
+        alpha[0] = sx * pr[0];
         beta[0] = std::sqrt(b2);
         return PNED_OK;
     }
@@ -543,6 +544,9 @@ int pned_tps_fused(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, doub
     if (!ctx || !v1 || !va || !vb || !alpha || !beta || !coef) return fail(PNED_E_ARG, "null argument");
     if (ctx->site_vec.empty()) return fail(PNED_E_ARG, "call pned_tps_set_observables first");
     if (ctx->q != 2) return fail(PNED_E_UNSUPPORTED, "fused mTPQ is S=1/2");
+    PNED_TRY(check_hv_pair(ctx, v1, va));
+    PNED_TRY(check_hv_pair(ctx, v1, vb));
+    PNED_TRY(check_hv_pair(ctx, va, vb));
     const int nc = v1->nvec, nsq = ctx->nsq, ncoef = 7 + nsq, width = 10 + 2 * nsq;
     pned_vec *u = v1, *up = va, *w = vb;
     PNED_TRY(pned_vec_zero(ctx, up));

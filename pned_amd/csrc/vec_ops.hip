@@ -776,7 +776,7 @@ int beta2_after_hv(pned_ctx *ctx, const pned_vec *v1, const pned_vec *v0, double
 int lanczos_lazy_step(pned_ctx *ctx, pned_vec *x, pned_vec *y, const double *sx, const double *sy, bool fresh,
                       double *pr, double *b2)
 {
-    PNED_TRY(same_shape(x, y));
+    PNED_TRY(check_hv_pair(ctx, x, y));
     PNED_TRY(check_pow2_cols(x->nvec));
     const bool spin = ctx->q >= 3 && ctx->q <= 5 && ctx->spin_tiled;
     if (!((ctx->q == 2 && ctx->sector_k < 0) || (spin && x->nvec == 1 && x->dtype == PNED_C128)))

@@ -220,11 +220,24 @@ class Server:
         def feeder():
             for r in reqs:
                 dt = r.arrival - self.clock()
-                if dt > 0:
-                    time.sleep(dt)
+                while dt > 0 and not stop.is_set():
+                    time.sleep(min(dt, 0.05))
+                    dt = r.arrival - self.clock()
+                if stop.is_set():
+                    return
                 self.queues[r.spec].put(r)
 
+        errors: List[BaseException] = []
+
         def worker(spec: str):
+            try:
+                serve(spec)
+            except BaseException as exc:   # a library error ends the run instead of hanging it
+                with self.lock:
+                    errors.append(exc)
+                stop.set()
+
+        def serve(spec: str):
             pool = None
             q = self.queues[spec]
             while not stop.is_set():
@@ -262,6 +275,8 @@ class Server:
             t.join()
         for p in self.pools.values():
             p.close()
+        if errors:
+            raise errors[0]
         return sorted(self.results, key=lambda r: r.rid)
 
     def _run_sequential(self, reqs: List[Request]) -> List[Result]:

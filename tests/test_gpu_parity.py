@@ -312,6 +312,34 @@ def test_dsf_parity_n12():
     assert np.array_equal(conv, c2)
 
 
+@pytest.mark.parametrize("spec", [8, 12])
+def test_dsf_parity_spin1(spec):
+    """The spin-S branch of intalv_zz (weights d - S, S_arbitrary/Hybrid_TPS9_DSF.f90:18-89)
+    at every momentum, then dsflnc1z and the continued fraction, against the oracle."""
+    _say("dsf spin1", spec)
+    cl = lattice.reference_cluster(spec)
+    op = model.build_operator(cl, 20, 108.0, q=3)
+    ctx = gpu.Context(op)
+    v1, v0, w = ctx.vector(), ctx.vector(), ctx.vector()
+    ctx.set_initial_vec(v1, 7)
+    x = v1.download()
+    for k, q in enumerate(cl.q_list):
+        cn = ctx.intalv_zz(v1, w, q, cl.site_vec)
+        vx = oracle.sz_q(cl.n, 3, cl.site_vec, q, x)
+        cn_ref = np.vdot(vx, vx).real
+        assert abs(cn - cn_ref / cl.n) <= 1e-13 * cn_ref, k
+        assert np.linalg.norm(w.download() - vx / np.sqrt(cn_ref)) <= 1e-13, k
+    sa, sb = ctx.dsflnc1z(w, v0, cn, 40)
+    sa_ref, sb_ref = oracle.dsf_lanczos(oracle.OracleOp(op), vx / np.sqrt(cn_ref), cn, 40)
+    assert np.allclose(sa[:20], sa_ref[:20], rtol=1e-11)
+    assert np.allclose(sb[:20], sb_ref[:20], rtol=1e-11)
+    ww, dsf, conv = ctx.renbun(sa, sb, -8.0, 0.05, 2000, 2.5e-3, 40, sq=cn)
+    w2, d2, c2 = oracle.renbun(sa, sb, -8.0, 0.05, 2000, 2.5e-3, 40)
+    d2 = oracle.normalize_dsf(d2, 2.5e-3, cn)
+    assert np.allclose(dsf, d2, rtol=1e-10, atol=1e-14)
+    assert np.array_equal(conv, c2)
+
+
 def test_cocg_complex_shift_spin1():
     _say("cocg spin1 n8")
     cl = lattice.reference_cluster(8)
@@ -646,8 +674,22 @@ def test_argument_errors_leave_the_context_usable():
     three = ctx.vector(3)
     small = ctx.vector(1, gpu.C64)
     ctx.set_initial_vec(v1, 3)
+    cl = lattice.get_cluster(12)
+    ctx.tps_set_observables(cl.site_vec, cl.sq_momenta())
+    other = gpu.Context(model.build_operator(lattice.get_cluster(8), 20, 54.0))
+    foreign = other.vector()                                                # 2^8 states, another context
+    one = np.ones(1, np.int32)
     cases = [
         (lambda: ctx.p_mltply_v3(v1, v1), gpu.E_ARG),                       # v1 and v0 the same vector
+        (lambda: ctx.lnc1(v1, v1, fixed_steps=5), gpu.E_ARG),
+        (lambda: ctx.lnc1(foreign, v0, fixed_steps=5), gpu.E_ARG),
+        (lambda: ctx.lanczos_batch_steps(v1, v1, one, np.zeros(1), np.zeros(1), 5), gpu.E_ARG),
+        (lambda: ctx.lanczos_batch_steps(foreign, v0, one, np.zeros(1), np.zeros(1), 5), gpu.E_ARG),
+        (lambda: ctx.lanczos_batch_step(v1, foreign, one, np.zeros(1), np.zeros(1)), gpu.E_ARG),
+        (lambda: ctx.tps_fused(v1, v0, v0, 1.0, 2), gpu.E_ARG),
+        (lambda: ctx.tps_fused(v1, foreign, v0, 1.0, 2), gpu.E_ARG),
+        (lambda: ctx.autotune(v1, v1), gpu.E_ARG),
+        (lambda: ctx.autotune(foreign, v0), gpu.E_ARG),
         (lambda: ctx.p_mltply_v3(v1, wide), gpu.E_ARG),                     # column counts differ
         (lambda: ctx.axpby(1.0, v1, 1.0, small), gpu.E_ARG),                # storage types differ
         (lambda: ctx.p_mltply_v3(three, ctx.vector(3)), gpu.E_UNSUPPORTED),  # 3 columns

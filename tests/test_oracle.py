@@ -15,8 +15,12 @@ so every pin is a fixture extracted from its text or a mathematical fact):
                          tridiagonal of the Krylov basis built by numpy;
 * orc_inverse_iteration -- residual ||Hx - E0 x|| and overlap with the dense GS;
 * orc_tps             -- (L - H)^k psi by dense matrix powers; energies per step;
-* orc_sz_q / DSF CF   -- explicit sums; m=1 Lorentzian closed form; normalisation
-                         integral equals S(q).
+                         every eval_phys_val column (internal energy, z-bond
+                         correlation with the reference's bit offset, static structure
+                         factors) from dense operators of the textbook S^z;
+* orc_sz_q            -- sum_j e^{iq.r_j} S^z_j of the textbook S^z (S=1/2 and spin 1);
+* DSF CF              -- backward evaluation of the same fraction; m=1 Lorentzian
+                         closed form; normalisation integral equals S(q).
 """
 import json
 import math
@@ -277,17 +281,79 @@ def test_tps_vs_dense_powers_n8():
     assert np.allclose(coef[:, 1], L)
 
 
-def test_sz_q_and_cf_closed_forms():
-    c = lattice.reference_cluster(8)
-    x = oracle.gaussian(2 ** 8, 5)
-    qv = c.q_list[4]
-    v = oracle.sz_q(8, 2, c.site_vec, qv, x)
-    idx = np.arange(2 ** 8)
-    ref = np.zeros_like(x)
-    for j in range(8):
-        ph = np.exp(1j * (qv @ c.site_vec[j]))
-        ref += (((idx >> j) & 1) - 0.5) * ph * x
-    assert np.allclose(v, ref, atol=1e-15)
+def _sz_diag(n, q):
+    """Diagonals of the textbook S^z_j (tests/dense.py basis order) for j = 0..n-1."""
+    _, _, sz = dense.site_ops(q)
+    return [dense.embed(sz, j, n, q).diagonal().real for j in range(n)]
+
+
+@pytest.mark.parametrize("spec,phi,h", [(8, 54.0, 0.1), (12, 126.0, 0.0)])
+def test_tps_observables_vs_dense_operators(spec, phi, h):
+    """coef(:, 5..7+sq_num) of eval_phys_val (H/TPS.f90:83-111) from dense operators built
+    with the textbook S^z (basis: bit 0 = up, so a bit value b gives b - 1/2 = -S^z):
+      col 5   Re<vg|H|v1>  (= Re(l<vg|v1> - <vg|(l-H)v1>), calc_internal_energy :114-125);
+      col 6/7 <v1|Z|v1>, Re<vg|Z|v1> with Z = sum over type-3 bonds of f(p1) f(p2), where
+              p = the 1-based ipair entry used as a bit position (the reference's
+              off-by-one, :290-316) and f(p) = -S^z_p for p < n, -1/2 (bit of rank id 0)
+              for p = n;
+      sq      per momentum q_k: Re<v1|A|v1>, Re<vg|A|v1>, Re<v1|B|v1>, Re<vg|B|v1> with
+              A = sum_j e^{iq.r_j} S^z_j S^z_0 (calc_sq_zz) and
+              B = sum_j e^{iq.r_j} (1/8 - S^z_j S^z_0 / 2) (= [s_j != s_0]/4, calc_sq_pp);
+              of these 4*sq_num values only the first sq_num are stored (:106-109).
+    vg and v1 are the normalised (l - H)^{k-2} psi and (l - H)^{k-1} psi (vg = 0 at k = 1)."""
+    c = lattice.get_cluster(spec)
+    n = c.n
+    J, K = dense.model20_couplings(phi)
+    H = dense.kh_dense(c, J, K, K, K, h=h)
+    op_tps = model.build_operator(c, 20, phi, field=h, tps=True)
+    L = op_tps.shift_L
+    psi = oracle.gaussian(2 ** n, 17)
+    steps = 6
+    sq = c.sq_momenta()
+    nsq = len(sq)
+    _, _, coef = oracle.tps(oracle.OracleOp(op_tps), c.site_vec, sq, L, steps, psi)
+    szd = _sz_diag(n, 2)
+    f = [-szd[p] if p < n else -0.5 * np.ones(2 ** n) for p in range(n + 1)]
+    zdiag = sum(f[a] * f[b] for (a, b, t) in c.bonds if t == 3)
+    A = [sum(np.exp(1j * (qk @ c.site_vec[j])) * szd[j] * szd[0] for j in range(n)) for qk in sq]
+    B = [sum(np.exp(1j * (qk @ c.site_vec[j])) * (0.125 - 0.5 * szd[j] * szd[0]) for j in range(n)) for qk in sq]
+    vg = np.zeros(2 ** n, dtype=complex)
+    v1 = psi.copy()
+    for k in range(steps):
+        row = coef[k]
+        assert abs(row[4] - np.vdot(vg, H @ v1).real) < 1e-11, k
+        assert abs(row[5] - np.vdot(v1, zdiag * v1).real) < 1e-12, k
+        assert abs(row[6] - np.vdot(vg, zdiag * v1).real) < 1e-12, k
+        four = []
+        for a, b in zip(A, B):
+            four += [np.vdot(v1, a * v1).real, np.vdot(vg, a * v1).real,
+                     np.vdot(v1, b * v1).real, np.vdot(vg, b * v1).real]
+        assert np.allclose(row[7:7 + nsq], four[:nsq], rtol=0, atol=1e-12), k
+        w = L * v1 - H @ v1
+        vg, v1 = v1, w / np.linalg.norm(w)
+    # the off-by-one matters on these clusters: the textbook z-bond sum differs
+    zz_true = sum(szd[a - 1] * szd[b - 1] for (a, b, t) in c.bonds if t == 3)
+    assert np.abs(zz_true - zdiag).max() > 0.1
+
+
+@pytest.mark.parametrize("spec,q", [(8, 2), (12, 2), (8, 3)])
+def test_sz_q_is_the_dense_sz_sum(spec, q):
+    """orc_sz_q (intalv_zz before normalisation, H/DSF.f90:18-87, S/DSF.f90:18-89) against
+    sum_j e^{iq.r_j} S^z_j from the textbook S^z.  Basis conventions (both trees): the
+    S=1/2 tree's bit 0 is up, so its weight (b - 1/2) is -S^z_j; the spin-S tree's digit d
+    has m = d - S, so its weight (d - S) is +S^z_j."""
+    c = lattice.get_cluster(spec)
+    n = c.n
+    sign = -1.0 if q == 2 else 1.0
+    szd = _sz_diag(n, q)
+    x = np.random.default_rng(5).standard_normal(q ** n) + 1j * np.random.default_rng(6).standard_normal(q ** n)
+    for qv in c.q_list:
+        v = oracle.sz_q(n, q, c.site_vec, qv, x)
+        ref = sign * sum(np.exp(1j * (qv @ c.site_vec[j])) * szd[j] for j in range(n)) * x
+        assert np.linalg.norm(v - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_cf_closed_forms():
     # continued fraction: the modified Lentz recursion must equal the backward
     # evaluation of f = t + a1/(b1 + a2/(b2 + ...)), a_m = -sbeta_m^2, b_m = w+E-alpha_m+i eta
     rng = np.random.default_rng(9)
