@@ -141,6 +141,21 @@ int pned_p_mltply_v3(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double epe
 /* same, with v0 treated as zero on input (fresh output: 2 instead of 3 vector streams) */
 int pned_hv_fresh(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, double *prdct);
 
+/* Allocation-free p_mltply_v3.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same v0 += (H - ibond*eperbd) v1 as pned_p_mltply_v3 on an S=1/2
+ * full-space context, with the same kernels; prdct_dev (device memory, 2 doubles per column)
+ * receives <v1_c|(H - ibond*eperbd)|v1_c>.  It allocates nothing and does not synchronise:
+ * the pass tables must exist for this vector shape (one earlier pned_p_mltply_v3 with such
+ * vectors builds them; otherwise PNED_E_ARG), and the per-block partial sums go to a static
+ * device array of PNED_NOALLOC_SCRATCH_BYTES per GPU (passes x 4096 x 2 x nvec doubles must
+ * fit: nvec <= 8 at up to 8 passes; larger requests fail with PNED_E_ARG).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (several streams or threads)
+ * race. */
+#define PNED_NOALLOC_SCRATCH_BYTES (4u << 20)
+int pned_p_mltply_v3_noalloc(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd,
+                             double *prdct_dev);
+
 /* calc_vec_inprdct (Hybrid_TPS_parallel.f90:104-126): out[c] = sum conj(x_c) y_c, FP64 */
 int pned_calc_vec_inprdct(pned_ctx *ctx, const pned_vec *x, const pned_vec *y, double *out);
 /* calc_beta (Hybrid_TPS_parallel.f90:161-188): beta2[c] = ||v0_c - alpha_c v1_c||^2 */

@@ -671,6 +671,7 @@ int pned_ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair, co
         return fail(PNED_E_HIP, "no HIP device available (libpned_gpu needs an MI355X)");
     if (device < 0 || device >= ndev) return fail(PNED_E_ARG, "device index out of range");
     PNED_HIP(hipSetDevice(device));
+    PNED_TRY(noalloc_scratch_init(device));
     // datack (Hybrid_TPS_basic.f90:8-21)
     for (int k = 0; k < 2 * ibond; ++k)
         if (ipair[k] <= 0 || ipair[k] > n_sites)

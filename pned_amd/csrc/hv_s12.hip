@@ -15,6 +15,7 @@
 // add never-flipped column bits at the bottom, which the planner keeps in the tile so
 // the 16-byte lanes of a wave form contiguous runs.
 #include <cstring>
+#include <mutex>
 #include <utility>
 
 #include "internal.h"
@@ -708,8 +709,60 @@ static void build_tm_terms(const S12Plan &pl, const PassPlan &pp, int TPB, int c
     }
 }
 
+// ------------------------------------------------------------------------------------
+// Scratch of the allocation-free H.v (pned_p_mltply_v3_noalloc): the per-block partial
+// sums of <v1|H|v1> of every pass.  One array per GPU for the life of the process; its
+// device address is looked up once per device when the first context on that device is
+// created, and host code only ever passes that address (the array's name is a host-side
+// stand-in).  Calls on one GPU share it, so overlapping calls on one GPU race.
+// ------------------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = PNED_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+int noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(PNED_E_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return PNED_OK;
+    int cur = 0;
+    PNED_HIP(hipGetDevice(&cur));
+    if (cur != device) PNED_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) PNED_HIP(hipSetDevice(cur));
+    if (e != hipSuccess || p == nullptr)
+        return fail(PNED_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double *>(p);
+    return PNED_OK;
+}
+
+static double *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+// noalloc: the caller's plan, pass tables and buffers must already exist; partials go to
+// part_ext (cap doubles) and the sums of <v1|H|v1> to result_ext (device memory).  Nothing
+// is allocated and the stream is not synchronised.
+static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                      double *prdct_host, const double *d_colscale, const double *d_yscale, bool *lz_fused,
+                      bool noalloc, double *part_ext, size_t part_cap, double *result_ext);
+
 int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                   double *prdct_host, const double *d_colscale, const double *d_yscale, bool *lz_fused)
+{
+    return hv_s12_run(ctx, v1, v0, eperbd, fresh, prdct_host, d_colscale, d_yscale, lz_fused, false, nullptr, 0,
+                      nullptr);
+}
+
+static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
+                      double *prdct_host, const double *d_colscale, const double *d_yscale, bool *lz_fused,
+                      bool noalloc, double *part_ext, size_t part_cap, double *result_ext)
 {
     if (lz_fused) *lz_fused = false;
     const int nvec = v1->nvec;
@@ -717,16 +770,32 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
     while ((1 << colbits) < nvec) ++colbits;
     if ((1 << colbits) != nvec)
         return fail(PNED_E_UNSUPPORTED, "batched H.v needs a power-of-two column count");
-    PNED_TRY(ensure_s12_plan(ctx, colbits, v1->dtype));
+    if (noalloc) {
+        if (ctx->plan_colbits != colbits || ctx->plan_dtype != v1->dtype || ctx->pd_dev == nullptr)
+            return fail(PNED_E_ARG, "no H.v plan for this vector shape yet: call pned_p_mltply_v3 once with "
+                                    "vectors of this shape first (that call allocates the plan's tables)");
+    } else {
+        PNED_TRY(ensure_s12_plan(ctx, colbits, v1->dtype));
+    }
     const S12Plan &pl = ctx->s12;
     if (pl.nop > 1 && pl.nop != nvec)
         return fail(PNED_E_ARG, "a batch context's vectors need one column per operator");
     const int npass = (int)pl.passes.size();
     const int grid_cap = 4096;
     // partial layout: [pass][block][2*ncol] (plus one pass for a fused Lanczos step)
-    size_t need = (size_t)(npass + 1) * grid_cap * 2 * nvec + 64;
-    PNED_TRY(ensure_partial(ctx, need));
-    PNED_TRY(ensure_result(ctx, 4 * (size_t)nvec + 8));
+    if (noalloc) {
+        const size_t need = (size_t)npass * grid_cap * 2 * nvec;
+        if (need > part_cap)
+            return fail(PNED_E_ARG, "allocation-free H.v: " + std::to_string(npass) + " passes x " +
+                                        std::to_string(nvec) + " columns need " + std::to_string(need * 8) +
+                                        " bytes of partial sums, more than the " +
+                                        std::to_string(part_cap * 8) + "-byte static scratch");
+    } else {
+        size_t need = (size_t)(npass + 1) * grid_cap * 2 * nvec + 64;
+        PNED_TRY(ensure_partial(ctx, need));
+        PNED_TRY(ensure_result(ctx, 4 * (size_t)nvec + 8));
+    }
+    double *const part_base = noalloc ? part_ext : ctx->d_partial;
     // per-pass address tables (rebuilt when the plan or the thread count changes)
     const int TPBpref = (ctx->tune_threads == 256 || ctx->tune_threads == 512 || ctx->tune_threads == 1024)
                             ? ctx->tune_threads
@@ -741,6 +810,9 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         if (rr[p] > PD_HI) return fail(PNED_E_UNSUPPORTED, "tile too large for the thread count");
         if ((int)pl.passes[p].ob.size() > 22) return fail(PNED_E_UNSUPPORTED, "too many outer bits");
     }
+    if (noalloc && ctx->pd_threads != TPBpref)
+        return fail(PNED_E_ARG, "no H.v plan for this thread count yet: call pned_p_mltply_v3 once after "
+                                "changing the tuning (that call rebuilds the plan's tables)");
     if (ctx->pd_threads != TPBpref || ctx->pd_dev == nullptr) {
         std::vector<PassDev> host(npass);
         for (int p = 0; p < npass; ++p) {
@@ -813,14 +885,14 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         a.ngroups = ctx->diag_host.ngroups;
         const int TPB = tpb[p], R = rr[p];
         int grid = (int)((a.tile_count < (uint64_t)grid_cap) ? a.tile_count : (uint64_t)grid_cap);
-        double *part = ctx->d_partial + (size_t)total_rows * 2 * nvec;
+        double *part = part_base + (size_t)total_rows * 2 * nvec;
         const PassDev *pdp = reinterpret_cast<const PassDev *>(ctx->pd_dev) + p;
         int rc;
         const bool c128 = v1->dtype == PNED_C128;
         const size_t tile_bytes = (c128 ? 16 : 8) * (size_t)TPB * R;
         const bool tm = ctx->s12_kernel == 1 && (c128 ? has_tm<double>(TPB, R) : has_tm<float>(TPB, R)) &&
                         tile_bytes + tm_coef_bytes(a) <= LDS_LIMIT;
-        if (lz_fused && p == npass - 1 && p > 0 && tm && c128 && nvec == 1 && pl.nop <= 1) {
+        if (!noalloc && lz_fused && p == npass - 1 && p > 0 && tm && c128 && nvec == 1 && pl.nop <= 1) {
             // deferred-normalisation Lanczos step: this pass's <v1|H v1> share first (no v0
             // traffic), alpha from all passes on the device, then the pass itself forms
             // v0 - Re(alpha) s v1 and the partials of beta^2
@@ -855,9 +927,10 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
         total_rows += grid;
     }
     const int ncol2 = 2 * nvec;
-    hipLaunchKernelGGL(reduce_partials, dim3(ncol2), dim3(256), 0, ctx->stream, ctx->d_partial, total_rows, ncol2,
-                       ctx->d_result, 0);
+    hipLaunchKernelGGL(reduce_partials, dim3(ncol2), dim3(256), 0, ctx->stream, part_base, total_rows, ncol2,
+                       noalloc ? result_ext : ctx->d_result, 0);
     PNED_HIP(hipGetLastError());
+    if (noalloc) return PNED_OK;
     if (prdct_host) {
         PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double) * ncol2,
                                 hipMemcpyDeviceToHost, ctx->stream));
@@ -868,3 +941,18 @@ int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd
 }
 
 }  // namespace pned
+
+extern "C" int pned_p_mltply_v3_noalloc(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd,
+                                        double *prdct_dev)
+{
+    using namespace pned;
+    PNED_TRY(check_hv_pair(ctx, v1, v0));
+    if (!prdct_dev) return fail(PNED_E_ARG, "null prdct_dev");
+    if (ctx->q != 2 || ctx->sector_k >= 0)
+        return fail(PNED_E_UNSUPPORTED, "the allocation-free H.v covers S=1/2 full-space contexts only");
+    double *part = noalloc_scratch(ctx->device);
+    if (!part) return fail(PNED_E_HIP, "allocation-free H.v: no scratch address for this device");
+    PNED_HIP(hipSetDevice(ctx->device));
+    return hv_s12_run(ctx, v1, v0, eperbd, false, nullptr, nullptr, nullptr, nullptr, true, part,
+                      NOALLOC_SCRATCH_DOUBLES, prdct_dev);
+}

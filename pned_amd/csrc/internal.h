@@ -243,6 +243,9 @@ int reduce_rows_result(pned_ctx *ctx, int nrows, int ncol2);
 // lz_fused (deferred-normalisation Lanczos step, one complex128 column): when not null and
 // the last pass runs on the term-major kernel, that pass also forms v0 - Re(alpha) s v1;
 // then d_result holds <v1|H s v1> (2 doubles) followed by [beta^2, 0] and *lz_fused = true
+// resolves the device address of the allocation-free H.v's static scratch on one device
+// (once per device; called at context creation with that device current)
+int noalloc_scratch_init(int device);
 int launch_hv_s12(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                   double *prdct_host, const double *d_colscale = nullptr, const double *d_yscale = nullptr,
                   bool *lz_fused = nullptr);

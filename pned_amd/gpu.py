@@ -91,6 +91,7 @@ def lib():
     L.pned_set_initial_vec.argtypes = [P, P, u64]
     L.pned_p_mltply_v3.argtypes = [P, P, P, f64, _dp]
     L.pned_hv_fresh.argtypes = [P, P, P, f64, _dp]
+    L.pned_p_mltply_v3_noalloc.argtypes = [P, P, P, f64, P]
     L.pned_calc_vec_inprdct.argtypes = [P, P, P, _dp]
     L.pned_calc_beta.argtypes = [P, _dp, P, P, _dp]
     L.pned_calc_alpha_beta.argtypes = [P, P, P, _dp, _dp]
@@ -336,6 +337,11 @@ class Context:
         check(lib().pned_p_mltply_v3(self.h, v1.h, v0.h, eperbd, pr))
         z = pr[0::2] + 1j * pr[1::2]
         return z[0] if v1.nvec == 1 else z
+
+    def p_mltply_v3_noalloc(self, v1: Vector, v0: Vector, eperbd: float, prdct_dev: int):
+        """Allocation-free H.v: <v1|(H - shift)|v1> lands in device memory at prdct_dev
+        (2 doubles per column); nothing is synchronised."""
+        check(lib().pned_p_mltply_v3_noalloc(self.h, v1.h, v0.h, eperbd, C.c_void_p(prdct_dev)))
 
     def hv_fresh(self, v1: Vector, v0: Vector, eperbd: float = 0.0):
         pr = np.zeros(2 * v1.nvec)
