@@ -109,7 +109,24 @@ int pned_s12_plan_export(int n_sites, int ibond, const int32_t *ipair, const int
                          const double *xlocal_ham, const int32_t *llocal_ham, int colbits,
                          int tile_bits, int low_bits, int *npass, int *nterm, int *ngroup,
                          int32_t *pass_info, int32_t *pass_bits, double *terms,
-                         uint32_t *term_gmask, double *diag, int max_pass, int max_term);
+                         uint32_t *term_gmask, double *diag, int max_pass, int max_term,
+                         const int32_t *site_bit /* memory layout (see below), NULL = identity */);
+/* ---- memory layout of full-space S=1/2 vectors (no reference counterpart) ----
+ * Device memory holds basis state s at memory index swap(s), where swap exchanges up to 4
+ * disjoint pairs of bits: site k (0-based) lives in memory bit site_bit[k], and
+ * site_bit[site_bit[k]] == k.  A new single-operator S=1/2 context picks the layout that
+ * puts the three sites into the lowest memory bits (the bits every tile pass keeps, for
+ * contiguous 128-byte runs) that need the fewest tile passes; identity when no choice
+ * saves a pass.  Every host boundary -- upload/download, vector files, the Gaussian start
+ * vector (counter = basis index), S^z(q), S^+-(q), mTPQ observables, check1 -- maps
+ * through it, so callers see the reference's basis order; only pned_vec_data exposes the
+ * layout.  pned_ctx_sz_sector returns a context to the identity layout. */
+int pned_ctx_layout(const pned_ctx *ctx, int32_t *site_bit /* n_sites */);
+/* set the layout explicitly (NULL = identity); only before the context has vectors */
+int pned_ctx_set_layout(pned_ctx *ctx, const int32_t *site_bit);
+/* diagnostic, needs no GPU: the layout pned_ctx_create would choose for these tables */
+int pned_s12_layout_choose(int n_sites, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                           const double *xlocal_ham, const int32_t *llocal_ham, int32_t *site_bit);
 /* synchronize the context stream */
 int pned_ctx_sync(pned_ctx *ctx);
 /* the hipStream_t the context launches on (for event timing by the caller) */
@@ -124,7 +141,8 @@ int pned_vec_upload(pned_ctx *ctx, pned_vec *v, const void *host, int64_t offset
 int pned_vec_download(pned_ctx *ctx, const pned_vec *v, void *host, int64_t offset, int64_t count);
 int pned_vec_zero(pned_ctx *ctx, pned_vec *v);
 int pned_vec_copy(pned_ctx *ctx, const pned_vec *src, pned_vec *dst);
-/* device pointer of the data (for zero-copy use by the caller's own kernels) */
+/* device pointer of the data (for zero-copy use by the caller's own kernels; states are in
+ * the context's memory layout, pned_ctx_layout) */
 void *pned_vec_data(pned_vec *v);
 
 /* ---- parallelized_diag (S_one_half/Hybrid_TPS_parallel.f90:6-8) ---------------------- */

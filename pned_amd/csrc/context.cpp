@@ -58,8 +58,10 @@ struct RawDecomp {
     Cx c0;
 };
 
+// pos: memory bit of each site (null: bit k = site k).  The bond's local index keeps the
+// reference's site order (a < b by site number); only the bits the terms act on move.
 static int decompose_raw(int n, int ibond, const int32_t *ipair, const int32_t *btype, const double *xl,
-                         const int32_t *ll, RawDecomp &rd)
+                         const int32_t *ll, RawDecomp &rd, const int8_t *pos)
 {
     rd = RawDecomp();
     rd.site_w.assign(n, Cx());
@@ -92,11 +94,13 @@ static int decompose_raw(int n, int ibond, const int32_t *ipair, const int32_t *
         Cx e0, e1, e2, e12;
         walsh(g, e0, e1, e2, e12);
         // d = e0 + e1 sa + e2 sb + e12 sa sb, sigma = 1 - 2s, sigma_a sigma_b = 1 - 2 par
+        const int m_a = pos ? pos[s_a] : s_a, m_b = pos ? pos[s_b] : s_b;   // memory bits
         rd.c0 = rd.c0 + e0 + e1 + e2 + e12;
-        rd.site_w[s_a] = rd.site_w[s_a] - 2.0 * e1;
-        rd.site_w[s_b] = rd.site_w[s_b] - 2.0 * e2;
-        rd.pair_w[{s_a, s_b}] = rd.pair_w[{s_a, s_b}] - 2.0 * e12;
-        const uint32_t ba = 1u << s_a, bb = 1u << s_b;
+        rd.site_w[m_a] = rd.site_w[m_a] - 2.0 * e1;
+        rd.site_w[m_b] = rd.site_w[m_b] - 2.0 * e2;
+        const std::pair<int, int> pk{std::min(m_a, m_b), std::max(m_a, m_b)};
+        rd.pair_w[pk] = rd.pair_w[pk] - 2.0 * e12;
+        const uint32_t ba = 1u << m_a, bb = 1u << m_b;
         for (int f = 1; f <= 3; ++f) {
             for (int l = 0; l < 4; ++l) g[l] = X(t, l ^ f, l);
             walsh(g, e0, e1, e2, e12);
@@ -115,21 +119,21 @@ static int decompose_raw(int n, int ibond, const int32_t *ipair, const int32_t *
 static bool term_nz(const Term &T) { return nz(Cx{T.c0r, T.c0i}) || nz(Cx{T.c1r, T.c1i}); }
 
 int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype, const double *xl,
-                  const int32_t *ll, S12Plan &plan)
+                  const int32_t *ll, S12Plan &plan, const int8_t *pos)
 {
     const double *xls[1] = {xl};
     const int32_t *lls[1] = {ll};
-    return s12_decompose_batch(n, ibond, ipair, btype, 1, xls, lls, plan);
+    return s12_decompose_batch(n, ibond, ipair, btype, 1, xls, lls, plan, pos);
 }
 
 int s12_decompose_batch(int n, int ibond, const int32_t *ipair, const int32_t *btype, int nop,
-                        const double *const *xl, const int32_t *const *ll, S12Plan &plan)
+                        const double *const *xl, const int32_t *const *ll, S12Plan &plan, const int8_t *pos)
 {
     plan = S12Plan();
     plan.n = n;
     plan.nop = nop;
     std::vector<RawDecomp> rd(nop);
-    for (int o = 0; o < nop; ++o) PNED_TRY(decompose_raw(n, ibond, ipair, btype, xl[o], ll[o], rd[o]));
+    for (int o = 0; o < nop; ++o) PNED_TRY(decompose_raw(n, ibond, ipair, btype, xl[o], ll[o], rd[o], pos));
     // term structure: union of the nonzero (mask, selector) keys
     std::map<std::pair<uint32_t, uint32_t>, int> key_index;
     for (int o = 0; o < nop; ++o)
@@ -209,11 +213,11 @@ int s12_decompose_batch(int n, int ibond, const int32_t *ipair, const int32_t *b
 }
 
 int s12_set_column(S12Plan &plan, int ibond, const int32_t *ipair, const int32_t *btype, int col,
-                   const double *xl, const int32_t *ll)
+                   const double *xl, const int32_t *ll, const int8_t *pos)
 {
     if (col < 0 || col >= plan.nop) return fail(PNED_E_ARG, "column outside the batch");
     RawDecomp rd;
-    PNED_TRY(decompose_raw(plan.n, ibond, ipair, btype, xl, ll, rd));
+    PNED_TRY(decompose_raw(plan.n, ibond, ipair, btype, xl, ll, rd, pos));
     std::map<std::pair<uint32_t, uint32_t>, int> idx;
     for (size_t t = 0; t < plan.terms.size(); ++t) idx[{plan.terms[t].gmask, plan.terms[t].selmask}] = (int)t;
     std::vector<double> cc(plan.terms.size() * 4, 0.0);
@@ -400,6 +404,42 @@ static bool anneal_cover(const std::vector<uint64_t> &masks, int nb, int L, int 
     return false;
 }
 
+// The same local search from random tiles (the forced low bits plus K-L random free
+// bits), `tries` seeds derived from `seed`: finds covers the greedy start misses.  Fixed
+// seeds make the outcome a function of the mask set alone.
+static bool random_cover(const std::vector<uint64_t> &um, int nb, int L, int K, int P, uint64_t seed, int tries,
+                         int iters, std::vector<uint64_t> &out)
+{
+    if (P < 1 || nb <= K || K <= L) return false;
+    for (int t = 0; t < tries; ++t) {
+        Lcg rng{seed * 0x9E3779B97F4A7C15ull + 0xD1B54A32D192ED03ull * (uint64_t)(t + 1)};
+        std::vector<uint64_t> init((size_t)P);
+        for (int p = 0; p < P; ++p) {
+            uint64_t T = (1ull << L) - 1;
+            int cnt = L;
+            while (cnt < K) {
+                const int b = L + (int)(rng.next() % (uint32_t)(nb - L));
+                if (!(T >> b & 1)) { T |= 1ull << b; ++cnt; }
+            }
+            init[(size_t)p] = T;
+        }
+        if (anneal_cover(um, nb, L, K, init, rng, iters, out)) return true;
+    }
+    return false;
+}
+
+static std::vector<uint64_t> unique_masks(const S12Plan &plan, int colbits)
+{
+    std::vector<uint64_t> um;
+    for (const Term &t : plan.terms) um.push_back((uint64_t)t.gmask << colbits);
+    std::sort(um.begin(), um.end());
+    um.erase(std::unique(um.begin(), um.end()), um.end());
+    return um;
+}
+
+constexpr uint64_t RANDOM_COVER_SEED = 0x5EEDull;
+constexpr int RANDOM_COVER_TRIES = 2, RANDOM_COVER_ITERS = 30000;
+
 int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
 {
     plan.passes.clear();
@@ -447,6 +487,14 @@ int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
             if (!ok) break;
             best = found;
             if (best.size() <= 1) break;
+        }
+        // then from random tiles, one tile below the best cover so far
+        while (best.size() > 1) {
+            std::vector<uint64_t> found;
+            if (!random_cover(um, nb, L, K, (int)best.size() - 1, RANDOM_COVER_SEED, RANDOM_COVER_TRIES,
+                              RANDOM_COVER_ITERS, found))
+                break;
+            best = found;
         }
     }
     std::vector<char> done(plan.terms.size(), 0);
@@ -556,6 +604,125 @@ int upload_scalars(pned_ctx *ctx, const double *h, int n, double **dptr)
     return PNED_OK;
 }
 
+int layout_from_low_sites(int n, const int *sites, int nsites, Layout &out)
+{
+    out = Layout();
+    if (nsites < 0 || nsites > LAYOUT_MAX_SWAPS || n < nsites || n > 32) return fail(PNED_E_ARG, "bad layout sites");
+    std::vector<char> in_f(n, 0);
+    for (int k = 0; k < nsites; ++k) {
+        if (sites[k] < 0 || sites[k] >= n || in_f[sites[k]]) return fail(PNED_E_ARG, "bad layout sites");
+        in_f[sites[k]] = 1;
+    }
+    // sites outside the low bits trade places with low bits that hold no chosen site
+    int free_low = 0;
+    for (int k = 0; k < nsites; ++k) {
+        const int s = sites[k];
+        if (s < nsites) continue;
+        while (in_f[free_low]) ++free_low;
+        out.a[out.nswap] = free_low;
+        out.b[out.nswap] = s;
+        ++out.nswap;
+        out.pos[free_low] = (int8_t)s;
+        out.pos[s] = (int8_t)free_low;
+        ++free_low;
+    }
+    return PNED_OK;
+}
+
+int layout_from_table(int n, const int32_t *site_bit, Layout &out)
+{
+    out = Layout();
+    if (n < 1 || n > 32 || !site_bit) return fail(PNED_E_ARG, "bad layout table");
+    for (int k = 0; k < n; ++k) {
+        const int b = site_bit[k];
+        if (b < 0 || b >= n || site_bit[b] != k)
+            return fail(PNED_E_ARG, "layout table must exchange pairs of sites (site_bit[site_bit[k]] == k)");
+        if (b > k) {
+            if (out.nswap == LAYOUT_MAX_SWAPS) return fail(PNED_E_ARG, "layout table swaps more than 4 pairs");
+            out.a[out.nswap] = k;
+            out.b[out.nswap] = b;
+            ++out.nswap;
+        }
+        out.pos[k] = (int8_t)b;
+    }
+    return PNED_OK;
+}
+
+// Candidate layouts put three sites into the forced low tile bits (L = 3).  For the tile
+// sizes of large vectors, K = 12 and 13 (2^K complex128 states fill the LDS at 13), the
+// identity layout's pass count P_K is the planner's own; a candidate gains one for every
+// K at which the random-start local search (the planner's last stage, same seeds) covers
+// the mapped flip masks with P_K - 1 tiles.  The best candidate must gain; ties keep the
+// first found.  Seeded, so a cluster always gets the same layout.  Vectors below 2^18
+// states keep the identity: their H.v is bound by launches, not by passes.
+constexpr int LAYOUT_K0 = 12, LAYOUT_NK = 2, LAYOUT_CANDIDATES = 32, LAYOUT_MIN_SITES = 18;
+Layout choose_layout(const S12Plan &ip)
+{
+    Layout best;
+    const int n = ip.n, L = 3;
+    if (ip.nop != 1 || n < LAYOUT_MIN_SITES || n > 32 || ip.terms.empty()) return best;
+    const std::vector<uint64_t> um = unique_masks(ip, 0);
+    int pnat[LAYOUT_NK];
+    for (int ki = 0; ki < LAYOUT_NK; ++ki) {
+        S12Plan tmp;
+        tmp.n = ip.n;
+        tmp.terms = ip.terms;
+        if (s12_make_passes(tmp, 0, LAYOUT_K0 + ki, L) != PNED_OK) return best;
+        pnat[ki] = (int)tmp.passes.size();
+    }
+    Lcg rng{0xC0FFEE123ull};
+    int best_gain = 0;
+    for (int c = 0; c < LAYOUT_CANDIDATES && best_gain < LAYOUT_NK; ++c) {
+        int st[3];
+        for (int k = 0; k < 3; ++k) {
+            bool dup;
+            do {
+                st[k] = (int)(rng.next() % (uint32_t)n);
+                dup = false;
+                for (int j = 0; j < k; ++j) dup = dup || st[j] == st[k];
+            } while (dup);
+        }
+        Layout lay;
+        if (layout_from_low_sites(n, st, 3, lay) != PNED_OK || lay.identity()) continue;
+        const SwapArgs sw = swap_args(lay);
+        std::vector<uint64_t> mm;
+        for (uint64_t m : um) mm.push_back(swap_bits(sw, m));
+        std::sort(mm.begin(), mm.end());
+        int gain = 0;
+        for (int ki = 0; ki < LAYOUT_NK; ++ki) {
+            std::vector<uint64_t> found;
+            if (pnat[ki] > 1 && random_cover(mm, n, L, LAYOUT_K0 + ki, pnat[ki] - 1, RANDOM_COVER_SEED,
+                                             RANDOM_COVER_TRIES, RANDOM_COVER_ITERS, found))
+                ++gain;
+        }
+        if (gain > best_gain) {
+            best_gain = gain;
+            best = lay;
+        }
+    }
+    return best;
+}
+
+int reset_layout(pned_ctx *ctx)
+{
+    if (ctx->lay.identity()) return PNED_OK;
+    ctx->lay = Layout();
+    PNED_TRY(s12_decompose(ctx->n, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), ctx->xl.data(), ctx->ll.data(),
+                           ctx->s12, nullptr));
+    ctx->plan_colbits = -1;
+    ctx->plan_dtype = -1;
+    ctx->pd_threads = -1;
+    return PNED_OK;
+}
+
+int check_same_layout(const pned_vec *x, const pned_vec *y)
+{
+    if (!x->ctx || !y->ctx || x->ctx == y->ctx) return PNED_OK;
+    if (!x->ctx->lay.same(y->ctx->lay))
+        return fail(PNED_E_ARG, "the two vectors' contexts lay their states out differently in memory");
+    return PNED_OK;
+}
+
 bool default_large(const pned_ctx *ctx, int colbits)
 {
     return (double)ctx->dim * (double)(1ull << colbits) > (double)(1ull << 25);
@@ -570,10 +737,19 @@ int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
     // vectors one bit below (1024 threads)
     const int kmax = (dtype == PNED_C128) ? 13 : 14;
     int K = ctx->tune_tile_bits;
-    if (K <= 0) K = default_large(ctx, colbits) ? kmax - 1 : kmax - 2;
+    const bool untuned = K <= 0;
+    if (untuned) K = default_large(ctx, colbits) ? kmax - 1 : kmax - 2;
     if (K > kmax) K = kmax;
     int L = ctx->tune_low_bits > 0 ? ctx->tune_low_bits : 3;
     PNED_TRY(s12_make_passes(ctx->s12, colbits, K, L));
+    if (untuned && K < kmax) {
+        // one tile bit more when it saves a whole pass (a pass streams v1 and v0 once)
+        S12Plan wider;
+        wider.n = ctx->s12.n;
+        wider.terms = ctx->s12.terms;
+        PNED_TRY(s12_make_passes(wider, colbits, K + 1, L));
+        if (wider.passes.size() < ctx->s12.passes.size()) PNED_TRY(s12_make_passes(ctx->s12, colbits, K + 1, L));
+    }
     S12Plan &pl = ctx->s12;
     const size_t nt = std::max<size_t>(pl.devterms.size(), 1);
     PNED_HIP(hipStreamSynchronize(ctx->stream));
@@ -657,8 +833,19 @@ int pned_device_count(void)
     return n;
 }
 
+static int ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                      const double *xlocal_ham, const int32_t *llocal_ham, int device, bool pick_layout,
+                      pned_ctx **out);
+
 int pned_ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair, const int32_t *ibond_type,
                     const double *xlocal_ham, const int32_t *llocal_ham, int device, pned_ctx **out)
+{
+    return ctx_create(n_sites, q_spin, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, device, true, out);
+}
+
+static int ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                      const double *xlocal_ham, const int32_t *llocal_ham, int device, bool pick_layout,
+                      pned_ctx **out)
 {
     if (!out || !ipair || !ibond_type || !xlocal_ham || !llocal_ham) return fail(PNED_E_ARG, "null argument");
     *out = nullptr;
@@ -701,6 +888,11 @@ int pned_ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair, co
     }
     if (q_spin == 2) {
         int rc = s12_decompose(n_sites, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, ctx->s12);
+        if (rc == PNED_OK && pick_layout) {
+            ctx->lay = choose_layout(ctx->s12);
+            if (!ctx->lay.identity())
+                rc = s12_decompose(n_sites, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, ctx->s12, ctx->lay.pos);
+        }
         if (rc != PNED_OK) {
             pned_ctx_destroy(ctx);
             return rc;
@@ -715,7 +907,7 @@ int pned_ctx_create_batch(int n_sites, int q_spin, int ibond, const int32_t *ipa
 {
     if (q_spin != 2) return fail(PNED_E_UNSUPPORTED, "batch contexts are S=1/2");
     if (nop < 1 || nop > 64 || (nop & (nop - 1))) return fail(PNED_E_ARG, "nop must be a power of two <= 64");
-    PNED_TRY(pned_ctx_create(n_sites, q_spin, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, device, out));
+    PNED_TRY(ctx_create(n_sites, q_spin, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, device, false, out));
     pned_ctx *ctx = *out;
     const size_t nx = 2 * 16 * 9, nl = 16 * 9;
     std::vector<const double *> xs(nop);
@@ -760,6 +952,7 @@ int pned_ctx_destroy(pned_ctx *ctx)
     if (ctx->d_result) (void)hipFree(ctx->d_result);
     if (ctx->h_result) (void)hipHostFree(ctx->h_result);
     if (ctx->d_scalars) (void)hipFree(ctx->d_scalars);
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
     if (ctx->spin.pairs) (void)hipFree(ctx->spin.pairs);
     if (ctx->spin.btype) (void)hipFree(ctx->spin.btype);
     reset_spin_tables(ctx);
@@ -780,7 +973,8 @@ static int refit_plan(pned_ctx *ctx, const double *xl, const int32_t *ll)
     S12Plan &pl = ctx->s12;
     if (pl.nop != 1 || pl.passes.empty() || ctx->plan_colbits < 0 || pl.colcoef.empty()) return 1;
     S12Plan trial = pl;
-    if (s12_set_column(trial, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), 0, xl, ll) != PNED_OK) return 1;
+    if (s12_set_column(trial, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), 0, xl, ll, ctx->lay.pos) != PNED_OK)
+        return 1;
     for (size_t t = 0; t < trial.terms.size(); ++t) {
         Term &T = trial.terms[t];
         T.c0r = trial.colcoef[t * 4 + 0];
@@ -831,7 +1025,7 @@ int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t
         if (rc < 0) return rc;   // a HIP error
         if (rc > 0) {
             PNED_TRY(s12_decompose(ctx->n, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), xlocal_ham, llocal_ham,
-                                   ctx->s12));
+                                   ctx->s12, ctx->lay.pos));
             ctx->plan_colbits = -1;
             ctx->plan_dtype = -1;
         }
@@ -942,12 +1136,14 @@ int pned_s12_plan_export(int n_sites, int ibond, const int32_t *ipair, const int
                          const double *xlocal_ham, const int32_t *llocal_ham, int colbits, int tile_bits,
                          int low_bits, int *npass, int *nterm, int *ngroup, int32_t *pass_info,
                          int32_t *pass_bits, double *terms, uint32_t *term_gmask, double *diag, int max_pass,
-                         int max_term)
+                         int max_term, const int32_t *site_bit)
 {
     if (!ipair || !ibond_type || !xlocal_ham || !llocal_ham || !npass || !nterm || !ngroup)
         return fail(PNED_E_ARG, "null argument");
+    Layout lay;
+    if (site_bit) PNED_TRY(layout_from_table(n_sites, site_bit, lay));
     S12Plan pl;
-    PNED_TRY(s12_decompose(n_sites, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, pl));
+    PNED_TRY(s12_decompose(n_sites, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, pl, lay.pos));
     PNED_TRY(s12_make_passes(pl, colbits, tile_bits, low_bits));
     *npass = (int)pl.passes.size();
     *nterm = (int)pl.devterms.size();
@@ -984,6 +1180,45 @@ int pned_s12_plan_export(int n_sites, int ibond, const int32_t *ipair, const int
     }
     diag[5 * *ngroup] = pl.c0r;
     diag[5 * *ngroup + 1] = pl.c0i;
+    return PNED_OK;
+}
+
+int pned_s12_layout_choose(int n_sites, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                           const double *xlocal_ham, const int32_t *llocal_ham, int32_t *site_bit)
+{
+    if (!ipair || !ibond_type || !xlocal_ham || !llocal_ham || !site_bit) return fail(PNED_E_ARG, "null argument");
+    if (n_sites < 2 || n_sites > 32) return fail(PNED_E_ARG, "bad site count");
+    S12Plan pl;
+    PNED_TRY(s12_decompose(n_sites, ibond, ipair, ibond_type, xlocal_ham, llocal_ham, pl));
+    const Layout lay = choose_layout(pl);
+    for (int k = 0; k < n_sites; ++k) site_bit[k] = lay.pos[k];
+    return PNED_OK;
+}
+
+int pned_ctx_layout(const pned_ctx *ctx, int32_t *site_bit)
+{
+    if (!ctx || !site_bit) return fail(PNED_E_ARG, "null argument");
+    for (int k = 0; k < ctx->n; ++k) site_bit[k] = ctx->lay.pos[k];
+    return PNED_OK;
+}
+
+int pned_ctx_set_layout(pned_ctx *ctx, const int32_t *site_bit)
+{
+    if (!ctx) return fail(PNED_E_ARG, "null context");
+    if (ctx->q != 2 || ctx->s12.nop != 1 || ctx->sector_k >= 0)
+        return fail(PNED_E_UNSUPPORTED, "memory layouts apply to single-operator S=1/2 full-space contexts");
+    if (!ctx->vecs.empty()) return fail(PNED_E_ARG, "set the layout before creating the context's vectors");
+    Layout lay;
+    if (site_bit) PNED_TRY(layout_from_table(ctx->n, site_bit, lay));
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
+    PNED_TRY(reset_layout(ctx));
+    if (lay.identity()) return PNED_OK;
+    PNED_TRY(s12_decompose(ctx->n, ctx->ibond, ctx->ipair.data(), ctx->btype.data(), ctx->xl.data(), ctx->ll.data(),
+                           ctx->s12, lay.pos));
+    ctx->lay = lay;
+    ctx->plan_colbits = -1;
+    ctx->plan_dtype = -1;
+    ctx->pd_threads = -1;
     return PNED_OK;
 }
 
@@ -1047,10 +1282,8 @@ int pned_vec_upload(pned_ctx *ctx, pned_vec *v, const void *host, int64_t offset
     if (!ctx || !v || !host) return fail(PNED_E_ARG, "null argument");
     if (vec_detached(v)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (offset < 0 || count < 0 || offset + count > v->dim) return fail(PNED_E_ARG, "range outside the vector");
-    const size_t eb = elem_bytes(v->dtype) * v->nvec;
-    PNED_HIP(hipMemcpyAsync((char *)v->data + eb * offset, host, eb * count, hipMemcpyHostToDevice, ctx->stream));
-    PNED_HIP(hipStreamSynchronize(ctx->stream));
-    return PNED_OK;
+    if (v->ctx != ctx) return fail(PNED_E_ARG, "vector belongs to another context");
+    return vec_from_host(v, host, offset, count);
 }
 
 int pned_vec_download(pned_ctx *ctx, const pned_vec *v, void *host, int64_t offset, int64_t count)
@@ -1058,10 +1291,8 @@ int pned_vec_download(pned_ctx *ctx, const pned_vec *v, void *host, int64_t offs
     if (!ctx || !v || !host) return fail(PNED_E_ARG, "null argument");
     if (vec_detached(v)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (offset < 0 || count < 0 || offset + count > v->dim) return fail(PNED_E_ARG, "range outside the vector");
-    const size_t eb = elem_bytes(v->dtype) * v->nvec;
-    PNED_HIP(hipMemcpyAsync(host, (const char *)v->data + eb * offset, eb * count, hipMemcpyDeviceToHost, ctx->stream));
-    PNED_HIP(hipStreamSynchronize(ctx->stream));
-    return PNED_OK;
+    if (v->ctx != ctx) return fail(PNED_E_ARG, "vector belongs to another context");
+    return vec_to_host(v, host, offset, count);
 }
 
 int pned_vec_zero(pned_ctx *ctx, pned_vec *v)
@@ -1077,6 +1308,7 @@ int pned_vec_copy(pned_ctx *ctx, const pned_vec *src, pned_vec *dst)
     if (!ctx || !src || !dst) return fail(PNED_E_ARG, "null argument");
     if (vec_detached(src) || vec_detached(dst)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (src->bytes != dst->bytes || src->dtype != dst->dtype) return fail(PNED_E_ARG, "shape mismatch");
+    PNED_TRY(check_same_layout(src, dst));
     PNED_HIP(hipMemcpyAsync(dst->data, src->data, src->bytes, hipMemcpyDeviceToDevice, ctx->stream));
     return PNED_OK;
 }

@@ -247,6 +247,7 @@ extern "C" int pned_ctx_sz_sector(pned_ctx *ctx, int n_down)
     }
     if (n_down > n) return fail(PNED_E_ARG, "#(E02A)# Incorrect ldim or n given to sz");
     if (n > 32) return fail(PNED_E_UNSUPPORTED, "sector basis: N <= 32");
+    PNED_TRY(reset_layout(ctx));   // the sector tables index basis states directly
     std::vector<SectorTerm> st;
     PNED_TRY(sector_terms(ctx->s12, st));
     const int ihf = (n + 1) / 2, ihf0 = n / 2;

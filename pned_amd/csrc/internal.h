@@ -93,16 +93,71 @@ struct S12Plan {
 // batch decomposition: nop operators over one cluster -> common term structure and
 // diagonal groups (the common refinement of every operator's weight classes)
 int s12_decompose_batch(int n, int ibond, const int32_t *ipair, const int32_t *btype, int nop,
-                        const double *const *xl, const int32_t *const *ll, S12Plan &plan);
+                        const double *const *xl, const int32_t *const *ll, S12Plan &plan,
+                        const int8_t *pos = nullptr);
 // replace the coefficients of one column; the operator must fit the existing structure
 int s12_set_column(S12Plan &plan, int ibond, const int32_t *ipair, const int32_t *btype, int col,
-                   const double *xl, const int32_t *ll);
+                   const double *xl, const int32_t *ll, const int8_t *pos = nullptr);
 
-// build the S=1/2 decomposition of the reference tables (no passes yet)
+// build the S=1/2 decomposition of the reference tables (no passes yet); pos: memory bit
+// of each site (null = identity layout)
 int s12_decompose(int n, int ibond, const int32_t *ipair, const int32_t *btype,
-                  const double *xl_fortran, const int32_t *ll_fortran, S12Plan &plan);
+                  const double *xl_fortran, const int32_t *ll_fortran, S12Plan &plan,
+                  const int8_t *pos = nullptr);
 // group terms into tile passes for a given memory layout
 int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits);
+
+// Memory layout of a full-space S=1/2 context: the memory index of basis state s is
+// swap(s), which exchanges nswap disjoint pairs of bits (an involution, so the same map
+// takes memory indices back to basis states).  Site k (0-based) sits at memory bit pos[k].
+// Identity unless the planner finds fewer tile passes with other sites in the lowest bits.
+constexpr int LAYOUT_MAX_SWAPS = 4;
+struct Layout {
+    int nswap = 0;
+    int a[LAYOUT_MAX_SWAPS] = {0, 0, 0, 0};
+    int b[LAYOUT_MAX_SWAPS] = {0, 0, 0, 0};
+    int8_t pos[32];
+    Layout() { for (int k = 0; k < 32; ++k) pos[k] = (int8_t)k; }
+    bool identity() const { return nswap == 0; }
+    bool same(const Layout &o) const
+    {
+        for (int k = 0; k < 32; ++k)
+            if (pos[k] != o.pos[k]) return false;
+        return true;
+    }
+};
+// the swaps as a kernel argument (bit positions within the state part of the index)
+struct SwapArgs {
+    int n;
+    uint32_t a[LAYOUT_MAX_SWAPS], b[LAYOUT_MAX_SWAPS];
+};
+inline SwapArgs swap_args(const Layout &l)
+{
+    SwapArgs s{};
+    s.n = l.nswap;
+    for (int k = 0; k < l.nswap; ++k) {
+        s.a[k] = (uint32_t)l.a[k];
+        s.b[k] = (uint32_t)l.b[k];
+    }
+    return s;
+}
+__host__ __device__ inline uint64_t swap_bits(const SwapArgs &s, uint64_t x)
+{
+    for (int k = 0; k < s.n; ++k) {
+        const uint64_t t = ((x >> s.a[k]) ^ (x >> s.b[k])) & 1ull;
+        x ^= (t << s.a[k]) | (t << s.b[k]);
+    }
+    return x;
+}
+// builds the layout that puts the given sites (at most 4, distinct, 0-based) into memory
+// bits 0, 1, ... in that order of bits; sites already there stay
+int layout_from_low_sites(int n, const int *sites, int nsites, Layout &out);
+// layout from a site -> memory bit table (must be an involution with <= 4 swapped pairs)
+int layout_from_table(int n, const int32_t *site_bit, Layout &out);
+// chooses the layout of a new single-operator S=1/2 context from its decomposition (in
+// the identity layout): the sites in the forced low tile bits that give the fewest tile
+// passes at 2^11..2^13-state tiles; identity when nothing is gained
+Layout choose_layout(const S12Plan &identity_plan);
 
 struct DiagDev {             // flattened diagonal for the kernel (constant per context)
     int ngroups;
@@ -164,6 +219,10 @@ struct pned_ctx {
     // (v1/sx) and the two factors the next step applies
     int lz_swapped = 0;
     double lz_sx = 1.0, lz_sy = 0.0;
+    // memory layout of the full-space S=1/2 vectors (identity for every other context)
+    pned::Layout lay;
+    double *d_stage = nullptr;       // device staging of layout-mapped uploads/downloads
+    size_t stage_bytes = 0;
     // S^z sector basis (parity_type = 0); sector_k < 0: full 2^N space
     int sector_k = -1;
     int sector_nterm = 0;
@@ -218,6 +277,14 @@ int cocg_update(pned_ctx *ctx, const pned_vec *p, const pned_vec *y, pned_vec *x
 // after such steps: v1 <- sx x, v0 <- y + a sx x, (x, y) = (v1, v0) or, swapped, (v0, v1)
 int lanczos_lazy_restore(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, bool swapped, double sx, double a);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
+// returns the context to the identity layout (re-decomposes its operator); only while no
+// vector of the context holds data in the old layout matters to the caller
+int reset_layout(pned_ctx *ctx);
+// vectors of two contexts can exchange data only in one memory layout
+int check_same_layout(const pned_vec *x, const pned_vec *y);
+// layout-mapped copies of count states (all columns) starting at basis state offset
+int vec_to_host(const pned_vec *v, void *host, int64_t offset, int64_t count);
+int vec_from_host(pned_vec *v, const void *host, int64_t offset, int64_t count);
 // S^z sector basis: H.v and table release
 int launch_hv_sector(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                      double *prdct_host);

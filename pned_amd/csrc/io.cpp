@@ -134,13 +134,24 @@ int write_vec(FILE *f, const pned_vec *v)
     void *pin = nullptr;
     PNED_HIP(hipHostMalloc(&pin, STAGE, hipHostMallocDefault));
     const unsigned char *dev = (const unsigned char *)v->data;
+    const bool mapped = !v->ctx->lay.identity();   // file order = basis order
     const int rc = write_record(f, v->bytes, [&](FILE *ff, size_t off, size_t n) -> int {
         size_t done = 0;
         while (done < n) {
-            const size_t m = std::min(STAGE, n - done);
-            if (hipMemcpy(pin, dev + off + done, m, hipMemcpyDeviceToHost) != hipSuccess)
-                return fail(PNED_E_HIP, "device to host copy failed");
-            if (fwrite(pin, 1, m, ff) != m) return fail(PNED_E_ARG, "write failed");
+            if (!mapped) {
+                const size_t m = std::min(STAGE, n - done);
+                if (hipMemcpy(pin, dev + off + done, m, hipMemcpyDeviceToHost) != hipSuccess)
+                    return fail(PNED_E_HIP, "device to host copy failed");
+                if (fwrite(pin, 1, m, ff) != m) return fail(PNED_E_ARG, "write failed");
+                done += m;
+                continue;
+            }
+            // subrecords need not end on a state: fetch whole states around the bytes
+            const size_t eb = 16, b0 = off + done, m = std::min(STAGE - 2 * eb, n - done);
+            const size_t s0 = b0 / eb, s1 = (b0 + m + eb - 1) / eb;
+            PNED_TRY(vec_to_host(v, pin, (int64_t)s0, (int64_t)(s1 - s0)));
+            if (fwrite((const unsigned char *)pin + (b0 - s0 * eb), 1, m, ff) != m)
+                return fail(PNED_E_ARG, "write failed");
             done += m;
         }
         return PNED_OK;
@@ -159,7 +170,17 @@ int read_vec(FILE *f, pned_vec *v)
     void *pin = nullptr;
     PNED_HIP(hipHostMalloc(&pin, STAGE, hipHostMallocDefault));
     unsigned char *dev = (unsigned char *)v->data;
-    size_t fill = 0, base = 0;   // staged bytes, device offset of the stage
+    size_t fill = 0, base = 0;   // staged bytes, byte offset of the stage in the vector
+    // the stage is flushed at multiples of STAGE and at the end, so it always holds whole
+    // states; a layout-mapped vector takes them through the state map
+    auto flush = [&]() -> int {
+        if (v->ctx->lay.identity()) {
+            if (hipMemcpy(dev + base, pin, fill, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(PNED_E_HIP, "host to device copy failed");
+            return PNED_OK;
+        }
+        return vec_from_host(v, pin, (int64_t)(base / 16), (int64_t)(fill / 16));
+    };
     int rc = read_record(f, v->bytes, [&](size_t off, const unsigned char *src, size_t n) -> int {
         size_t done = 0;
         while (done < n) {
@@ -169,15 +190,13 @@ int read_vec(FILE *f, pned_vec *v)
             fill += m;
             done += m;
             if (fill == STAGE) {
-                if (hipMemcpy(dev + base, pin, fill, hipMemcpyHostToDevice) != hipSuccess)
-                    return fail(PNED_E_HIP, "host to device copy failed");
+                PNED_TRY(flush());
                 fill = 0;
             }
         }
         return PNED_OK;
     });
-    if (rc == PNED_OK && fill > 0 && hipMemcpy(dev + base, pin, fill, hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(PNED_E_HIP, "host to device copy failed");
+    if (rc == PNED_OK && fill > 0) rc = flush();
     (void)hipHostFree(pin);
     return rc;
 }

@@ -307,13 +307,15 @@ __global__ void k_lanczos_slots(typename V2<Real>::T *__restrict__ v1, typename 
 // the column a request occupies; returns the column's norm^2 partials
 template <typename Real>
 __global__ __launch_bounds__(RED_TPB) void k_gauss_col(typename V2<Real>::T *__restrict__ v, uint64_t seed,
-                                                       size_t dim, int ncol, int col, double *__restrict__ partial)
+                                                       size_t dim, int ncol, int col, SwapArgs sw,
+                                                       double *__restrict__ partial)
 {
     __shared__ double red[RED_TPB];
     const size_t stride = (size_t)RED_TPB * gridDim.x;
     double acc[1] = {0.0};
     for (size_t i = (size_t)blockIdx.x * RED_TPB + threadIdx.x; i < dim; i += stride) {
-        uint32_t c0 = (uint32_t)i, c1 = (uint32_t)(i >> 32), c2 = 0u, c3 = 0u;
+        const uint64_t bs = swap_bits(sw, i);   // basis state of memory state i
+        uint32_t c0 = (uint32_t)bs, c1 = (uint32_t)(bs >> 32), c2 = 0u, c3 = 0u;
         philox(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
         const uint64_t aa = ((uint64_t)c0 << 21) ^ (c1 >> 11);
         const uint64_t bb = ((uint64_t)c2 << 21) ^ (c3 >> 11);
@@ -381,13 +383,13 @@ __device__ __forceinline__ void philox(uint32_t &c0, uint32_t &c1, uint32_t &c2,
 
 template <typename Real>
 __global__ __launch_bounds__(RED_TPB) void k_gauss(typename V2<Real>::T *__restrict__ v, uint64_t seed,
-                                                   size_t nelem, int ncol, double *__restrict__ partial)
+                                                   size_t nelem, int ncol, SwapArgs sw, double *__restrict__ partial)
 {
     __shared__ double red[RED_TPB];
     const size_t stride = (size_t)RED_TPB * gridDim.x;
     double acc[1] = {0.0};
     for (size_t e = (size_t)blockIdx.x * RED_TPB + threadIdx.x; e < nelem; e += stride) {
-        const uint64_t i = e / (size_t)ncol;
+        const uint64_t i = swap_bits(sw, e / (size_t)ncol);   // basis state
         const uint32_t col = (uint32_t)(e % (size_t)ncol);
         uint32_t c0 = (uint32_t)i, c1 = (uint32_t)(i >> 32), c2 = col, c3 = 0u;
         philox(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
@@ -413,6 +415,7 @@ struct SzArgs {
     int n, q;
     double phr[32], phi[32];  // e^{i q.r_j}
     double wlow;              // weight offset: 0.5 (S=1/2) or S
+    int8_t pos[32];           // S=1/2: memory bit of site j
 };
 
 template <typename Real>
@@ -424,8 +427,13 @@ __global__ void k_sz_q(const typename V2<Real>::T *__restrict__ x, typename V2<R
         double cr = 0.0, ci = 0.0;
         uint64_t s = i;
         for (int j = 0; j < a.n; ++j) {
-            const int d = (int)(s % (uint64_t)a.q);
-            s /= (uint64_t)a.q;
+            int d;
+            if (a.q == 2) {
+                d = (int)((i >> a.pos[j]) & 1u);
+            } else {
+                d = (int)(s % (uint64_t)a.q);
+                s /= (uint64_t)a.q;
+            }
             const double w = (double)d - a.wlow;
             cr = fma(w, a.phr[j], cr);
             ci = fma(w, a.phi[j], ci);
@@ -446,6 +454,7 @@ struct PmArgs {
     double phr[32], phi[32];
     double fac[16];        // ladder factor by source digit (1 for the S=1/2 tree)
     uint64_t pw[32];       // q^j
+    SwapArgs sw_v, sw_x;   // S=1/2: memory layouts of v and x (basis state <-> memory index)
 };
 
 __global__ __launch_bounds__(256) void k_intalv_pm(const double2 *__restrict__ x, double2 *__restrict__ v, PmArgs a,
@@ -455,17 +464,19 @@ __global__ __launch_bounds__(256) void k_intalv_pm(const double2 *__restrict__ x
     for (uint64_t l = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; l < dim; l += stride) {
         double ar = 0.0, ai = 0.0;
         uint32_t rem = (uint32_t)l;   // q >= 3: dim < 2^32 (checked on the host)
+        const uint64_t bt = (a.q == 2) ? swap_bits(a.sw_v, l) : l;   // basis state of the target
         for (int j = 0; j < a.n; ++j) {
             int dt;
             if (a.q == 2) {
-                dt = (int)((l >> j) & 1u);
+                dt = (int)((bt >> j) & 1u);
             } else {
                 dt = (int)(rem % (uint32_t)a.q);
                 rem /= (uint32_t)a.q;
             }
             const int ds = dt - a.ipm;
             if (ds < 0 || ds >= a.q) continue;
-            const double2 y = x[a.ipm > 0 ? l - a.pw[j] : l + a.pw[j]];
+            const uint64_t bsrc = a.ipm > 0 ? bt - a.pw[j] : bt + a.pw[j];
+            const double2 y = x[(a.q == 2) ? swap_bits(a.sw_x, bsrc) : bsrc];
             const double rr = a.phr[j] * y.x - a.phi[j] * y.y, ri = a.phi[j] * y.x + a.phr[j] * y.y;
             ar += a.fac[ds] * rr;
             ai += a.fac[ds] * ri;
@@ -487,6 +498,7 @@ struct ObsArgs {
     int sq_left[OBS_MAX_SQ];              // 0: <v1|, 1: <vg|
     int sq_q[OBS_MAX_SQ];                 // momentum slot
     double phr[8][32], phi[8][32];        // e^{i q.r_j} per momentum slot
+    int8_t pos[32];                       // memory bit of site j
 };
 
 template <typename Real, int NOUT>
@@ -519,12 +531,12 @@ __global__ __launch_bounds__(RED_TPB) void k_tps_obs(const typename V2<Real>::T 
         acc[2] = fma(wzz, n1, acc[2]);
         acc[4] = fma(wzz, gr * xr + gi * xi, acc[4]);
         acc[5] = fma(wzz, gr * xi - gi * xr, acc[5]);
-        const int s0 = (int)(s & 1u);
+        const int s0 = (int)((s >> a.pos[0]) & 1u);
         for (int k = 0; k < a.nsq && 6 + 2 * k + 1 < NOUT; ++k) {
             const int qs = a.sq_q[k];
             double cr = 0.0, ci = 0.0;
             for (int j = 0; j < a.n; ++j) {
-                const int sj = (int)((s >> j) & 1u);
+                const int sj = (int)((s >> a.pos[j]) & 1u);
                 double w;
                 if (a.sq_mode[k] == 0) w = ((double)sj - 0.5) * ((double)s0 - 0.5);
                 else w = (sj != s0) ? 0.25 : 0.0;
@@ -605,10 +617,10 @@ __global__ __launch_bounds__(RED_TPB) void k_tps_fused(const typename V2<Real>::
                 wt[sl * nw] = wzz;
             } else {
                 const int k = kind - 1, qs = a.sq_q[k];
-                const int s0 = (int)(s & 1u);
+                const int s0 = (int)((s >> a.pos[0]) & 1u);
                 double cr = 0.0, ci = 0.0;
                 for (int j = 0; j < a.n; ++j) {
-                    const int sj = (int)((s >> j) & 1u);
+                    const int sj = (int)((s >> a.pos[j]) & 1u);
                     double wj;
                     if (a.sq_mode[k] == 0) wj = ((double)sj - 0.5) * ((double)s0 - 0.5);
                     else wj = (sj != s0) ? 0.25 : 0.0;
@@ -739,10 +751,84 @@ int same_shape(const pned_vec *a, const pned_vec *b)
     if (vec_detached(a) || vec_detached(b)) return fail(PNED_E_ARG, "vector of a destroyed context");
     if (!a || !b || a->dim != b->dim || a->nvec != b->nvec || a->dtype != b->dtype)
         return fail(PNED_E_ARG, "vector shapes or types differ");
+    return check_same_layout(a, b);
+}
+
+// layout-mapped staging copy: memory state swap(first + i) <-> stage row i, every row
+// 'words' 8-byte words (all columns of one state); to_dev = 1 writes the vector
+__global__ void k_layout_copy(uint64_t *__restrict__ vec, uint64_t *__restrict__ stage, SwapArgs sw, uint64_t first,
+                              uint64_t count, int words, int to_dev)
+{
+    const uint64_t total = count * (uint64_t)words;
+    const uint64_t stride = (uint64_t)blockDim.x * gridDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const uint64_t i = e / (uint64_t)words, w = e % (uint64_t)words;
+        const uint64_t m = swap_bits(sw, first + i) * (uint64_t)words + w;
+        if (to_dev) vec[m] = stage[e];
+        else stage[e] = vec[m];
+    }
+}
+
+constexpr size_t LAYOUT_STAGE_BYTES = (size_t)64 << 20;
+
+int ensure_stage(pned_ctx *ctx)
+{
+    if (ctx->d_stage) return PNED_OK;
+    if (hipMalloc((void **)&ctx->d_stage, LAYOUT_STAGE_BYTES) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->d_stage = nullptr;
+        return fail(PNED_E_NOMEM, "device staging buffer for a layout-mapped copy");
+    }
+    ctx->stage_bytes = LAYOUT_STAGE_BYTES;
+    return PNED_OK;
+}
+
+// count states from basis state offset, through the staging buffer in chunks
+int layout_copy(pned_vec *v, void *host, int64_t offset, int64_t count, bool to_dev)
+{
+    pned_ctx *ctx = v->ctx;
+    const size_t row = elem_bytes(v->dtype) * (size_t)v->nvec;
+    if (ctx->lay.identity()) {
+        char *dev = (char *)v->data + row * (size_t)offset;
+        if (to_dev) PNED_HIP(hipMemcpyAsync(dev, host, row * (size_t)count, hipMemcpyHostToDevice, ctx->stream));
+        else PNED_HIP(hipMemcpyAsync(host, dev, row * (size_t)count, hipMemcpyDeviceToHost, ctx->stream));
+        PNED_HIP(hipStreamSynchronize(ctx->stream));
+        return PNED_OK;
+    }
+    PNED_TRY(ensure_stage(ctx));
+    const SwapArgs sw = swap_args(ctx->lay);
+    const int64_t per = (int64_t)(ctx->stage_bytes / row);
+    for (int64_t done = 0; done < count;) {
+        const int64_t c = std::min<int64_t>(per, count - done);
+        char *h = (char *)host + row * (size_t)done;
+        const int grid = grid_for((size_t)c * row / 8, 256);
+        if (to_dev)
+            PNED_HIP(hipMemcpyAsync(ctx->d_stage, h, row * (size_t)c, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_layout_copy, dim3(grid), dim3(256), 0, ctx->stream, (uint64_t *)v->data,
+                           (uint64_t *)ctx->d_stage, sw, (uint64_t)(offset + done), (uint64_t)c, (int)(row / 8),
+                           to_dev ? 1 : 0);
+        PNED_HIP(hipGetLastError());
+        if (!to_dev)
+            PNED_HIP(hipMemcpyAsync(h, ctx->d_stage, row * (size_t)c, hipMemcpyDeviceToHost, ctx->stream));
+        PNED_HIP(hipStreamSynchronize(ctx->stream));
+        done += c;
+    }
     return PNED_OK;
 }
 
 }  // namespace
+
+int vec_to_host(const pned_vec *v, void *host, int64_t offset, int64_t count)
+{
+    PNED_HIP(hipSetDevice(v->ctx->device));
+    return layout_copy(const_cast<pned_vec *>(v), host, offset, count, false);
+}
+
+int vec_from_host(pned_vec *v, const void *host, int64_t offset, int64_t count)
+{
+    PNED_HIP(hipSetDevice(v->ctx->device));
+    return layout_copy(v, const_cast<void *>(host), offset, count, true);
+}
 
 int beta2_after_hv(pned_ctx *ctx, const pned_vec *v1, const pned_vec *v0, double *pr, double *b2)
 {
@@ -878,6 +964,8 @@ int intalv_pm_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double 
     a.n = ctx->n;
     a.q = ctx->q;
     a.ipm = ipm;
+    a.sw_v = swap_args(ctx->lay);
+    a.sw_x = swap_args(x->ctx->lay);
     const double ss = 0.5 * (ctx->q - 1);
     for (int d = 0; d < ctx->q; ++d) {
         const double sm = (double)d - ss;
@@ -1043,10 +1131,10 @@ int gaussian_column(pned_ctx *ctx, pned_vec *v, int col, uint64_t seed)
     PNED_TRY(ensure_partial(ctx, (size_t)grid));
     if (v->dtype == PNED_C128)
         hipLaunchKernelGGL(k_gauss_col<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (double2 *)v->data, seed,
-                           (size_t)v->dim, v->nvec, col, ctx->d_partial);
+                           (size_t)v->dim, v->nvec, col, swap_args(v->ctx->lay), ctx->d_partial);
     else
         hipLaunchKernelGGL(k_gauss_col<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (float2 *)v->data, seed,
-                           (size_t)v->dim, v->nvec, col, ctx->d_partial);
+                           (size_t)v->dim, v->nvec, col, swap_args(v->ctx->lay), ctx->d_partial);
     PNED_HIP(hipGetLastError());
     double nrm;
     PNED_TRY(finish_reduce(ctx, grid, 1, &nrm));
@@ -1088,10 +1176,10 @@ int gaussian_fill(pned_ctx *ctx, pned_vec *v, uint64_t seed)
     PNED_TRY(ensure_partial(ctx, (size_t)grid * v->nvec));
     if (v->dtype == PNED_C128)
         hipLaunchKernelGGL(k_gauss<double>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (double2 *)v->data, seed,
-                           nelem, v->nvec, ctx->d_partial);
+                           nelem, v->nvec, swap_args(v->ctx->lay), ctx->d_partial);
     else
         hipLaunchKernelGGL(k_gauss<float>, dim3(grid), dim3(RED_TPB), 0, ctx->stream, (float2 *)v->data, seed,
-                           nelem, v->nvec, ctx->d_partial);
+                           nelem, v->nvec, swap_args(v->ctx->lay), ctx->d_partial);
     PNED_HIP(hipGetLastError());
     std::vector<double> nrm(v->nvec);
     PNED_TRY(finish_reduce(ctx, grid, v->nvec, nrm.data()));
@@ -1106,10 +1194,12 @@ int sz_q_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, 
     PNED_TRY(same_shape(x, v));
     if (x->nvec != 1) return fail(PNED_E_UNSUPPORTED, "S^z(q) excitation takes one column");
     if (ctx->n > 32) return fail(PNED_E_UNSUPPORTED, "at most 32 sites");
+    PNED_TRY(check_same_layout(x, v));
     SzArgs a{};
     a.n = ctx->n;
     a.q = ctx->q;
     a.wlow = 0.5 * (ctx->q - 1);
+    for (int j = 0; j < 32; ++j) a.pos[j] = ctx->lay.pos[j];
     for (int j = 0; j < ctx->n; ++j) {
         const double qn = qv[0] * site_vec[2 * j] + qv[1] * site_vec[2 * j + 1];
         a.phr[j] = std::cos(qn);
@@ -1132,13 +1222,15 @@ static int build_obs_args(pned_ctx *ctx, ObsArgs &a)
     a = ObsArgs{};
     a.n = ctx->n;
     a.nzz = 0;
+    for (int j = 0; j < 32; ++j) a.pos[j] = ctx->lay.pos[j];
     for (int b = 0; b < ctx->ibond; ++b) {
         if (ctx->btype[b] != 3) continue;
         // H/TPS.f90:290-316 shifts by the 1-based site number: bit 'ipair'; past the last
         // site it reads the rank id (0 at one process) -> mask 0 -> bit value 0
+        // (basis bit p sits at memory bit pos[p])
         const int p1 = ctx->ipair[2 * b], p2 = ctx->ipair[2 * b + 1];
-        a.zz_mask1[a.nzz] = p1 < ctx->n ? (1u << p1) : 0u;
-        a.zz_mask2[a.nzz] = p2 < ctx->n ? (1u << p2) : 0u;
+        a.zz_mask1[a.nzz] = p1 < ctx->n ? (1u << ctx->lay.pos[p1]) : 0u;
+        a.zz_mask2[a.nzz] = p2 < ctx->n ? (1u << ctx->lay.pos[p2]) : 0u;
         if (++a.nzz >= 64) break;
     }
     // entries stored by the reference: sq(1..nsq), sq(4k+1..4k+4) = (zz,v1),(zz,vg),(pp,v1),(pp,vg)

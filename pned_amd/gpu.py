@@ -121,7 +121,11 @@ def lib():
     L.pned_set_initial_column.argtypes = [P, P, i32, u64]
     L.pned_s12_plan_export.argtypes = [i32, i32, _ip, _ip, _dp, _ip, i32, i32, i32, C.POINTER(i32),
                                        C.POINTER(i32), C.POINTER(i32), _ip, _ip, _dp,
-                                       np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), _dp, i32, i32]
+                                       np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), _dp, i32, i32,
+                                       C.c_void_p]
+    L.pned_s12_layout_choose.argtypes = [i32, i32, _ip, _ip, _dp, _ip, _ip]
+    L.pned_ctx_layout.argtypes = [P, _ip]
+    L.pned_ctx_set_layout.argtypes = [P, C.c_void_p]
     _lib = L
     return L
 
@@ -242,6 +246,20 @@ class Context:
         self.op = op
         check(lib().pned_ctx_set_matrices(self.h, np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
                                           op.fortran_llocal()))
+
+    def layout(self) -> np.ndarray:
+        """Memory bit of each site (identity unless the planner chose another layout)."""
+        out = np.zeros(self.n, dtype=np.int32)
+        check(lib().pned_ctx_layout(self.h, out))
+        return out
+
+    def set_layout(self, site_bit=None):
+        """Force a memory layout (before any vector of the context exists)."""
+        if site_bit is None:
+            check(lib().pned_ctx_set_layout(self.h, None))
+        else:
+            sb = np.ascontiguousarray(site_bit, dtype=np.int32)
+            check(lib().pned_ctx_set_layout(self.h, sb.ctypes.data))
 
     def tune(self, tile_bits=0, low_bits=0, threads=0):
         check(lib().pned_ctx_tune(self.h, tile_bits, low_bits, threads))
@@ -485,9 +503,22 @@ class Context:
         return w, dsf, conv
 
 
-def s12_plan(op, colbits=0, tile_bits=12, low_bits=4):
-    """The planner's decomposition of an S=1/2 operator (no GPU needed)."""
+def s12_layout(op):
+    """The memory layout a new context of this S=1/2 operator gets: site k -> memory bit
+    (no GPU needed)."""
+    out = np.zeros(op.n, dtype=np.int32)
+    check(lib().pned_s12_layout_choose(op.n, op.ibond, np.ascontiguousarray(op.ipair, dtype=np.int32),
+                                       np.ascontiguousarray(op.ibond_type, dtype=np.int32),
+                                       np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
+                                       op.fortran_llocal(), out))
+    return out
+
+
+def s12_plan(op, colbits=0, tile_bits=12, low_bits=4, site_bit=None):
+    """The planner's decomposition of an S=1/2 operator (no GPU needed); site_bit: the
+    memory layout (site k at memory bit site_bit[k]), None = identity."""
     mp, mt = 64, 4096
+    sb = None if site_bit is None else np.ascontiguousarray(site_bit, dtype=np.int32)
     npass, nterm, ngroup = C.c_int(), C.c_int(), C.c_int()
     pinfo = np.zeros(4 * mp, dtype=np.int32)
     pbits = np.zeros(40 * mp, dtype=np.int32)
@@ -498,7 +529,8 @@ def s12_plan(op, colbits=0, tile_bits=12, low_bits=4):
                                      np.ascontiguousarray(op.ibond_type, dtype=np.int32),
                                      np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
                                      op.fortran_llocal(), colbits, tile_bits, low_bits, C.byref(npass),
-                                     C.byref(nterm), C.byref(ngroup), pinfo, pbits, terms, gmask, diag, mp, mt))
+                                     C.byref(nterm), C.byref(ngroup), pinfo, pbits, terms, gmask, diag, mp, mt,
+                                     None if sb is None else sb.ctypes.data))
     P, T, G = npass.value, nterm.value, ngroup.value
     passes = []
     for p in range(P):

@@ -7,12 +7,6 @@
 
 #include "../../pned_amd/csrc/internal.h"
 
-namespace pned {
-int s12_decompose_batch(int n, int ibond, const int32_t *ipair, const int32_t *btype, int nop,
-                        const double *const *xl, const int32_t *const *ll, S12Plan &plan);
-int s12_set_column(S12Plan &plan, int ibond, const int32_t *ipair, const int32_t *btype, int col,
-                   const double *xl, const int32_t *ll);
-}  // namespace pned
 extern "C" const char *pned_last_error(void);
 
 int main(int argc, char **argv)
