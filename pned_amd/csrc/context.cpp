@@ -497,8 +497,29 @@ int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
             best = found;
         }
     }
-    std::vector<char> done(plan.terms.size(), 0);
-    for (size_t p = 0; p < best.size(); ++p) {
+    // Each term goes to the first tile that contains it; the diagonal goes to the pass with
+    // the least term work (1 per real term, 2 per complex term: twice the FMAs), which then
+    // runs first (it also takes the v0 input).  Measured at N=32 with 4 passes (MI355X):
+    // 206.3 ms per H.v, against 209-210 ms with the diagonal on the first tile's pass and
+    // 211.3 ms when multiply covered terms are also spread over the passes by work.
+    const size_t P = best.size(), nt = plan.terms.size();
+    std::vector<int> best_owner(nt, -1);
+    std::vector<double> load(P, 0.0);
+    for (size_t i = 0; i < nt; ++i)
+        for (size_t p = 0; p < P; ++p)
+            if ((mmask[i] & ~best[p]) == 0) {
+                best_owner[i] = (int)p;
+                const Term &t = plan.terms[i];
+                load[p] += (plan.nop > 1 || t.c0i != 0.0 || t.c1i != 0.0) ? 2.0 : 1.0;
+                break;
+            }
+    size_t diag_pass = 0;
+    for (size_t p = 1; p < P; ++p)
+        if (load[p] < load[diag_pass]) diag_pass = p;
+    std::vector<size_t> pass_order{diag_pass};
+    for (size_t p = 0; p < P; ++p)
+        if (p != diag_pass) pass_order.push_back(p);
+    for (size_t p : pass_order) {
         const uint64_t T = best[p];
         PassPlan pp;
         pp.K = __builtin_popcountll(T);
@@ -506,11 +527,10 @@ int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
             if (T >> bit & 1) pp.tb.push_back(bit);
             else pp.ob.push_back(bit);
         }
-        pp.diag = (p == 0);
+        pp.diag = (p == diag_pass);
         pp.first_term = (int)plan.devterms.size();
         for (size_t i = 0; i < plan.terms.size(); ++i) {
-            if (done[i] || (mmask[i] & ~T) != 0) continue;
-            done[i] = 1;
+            if (best_owner[i] != (int)p) continue;
             const Term &t = plan.terms[i];
             DevTerm d{};
             uint32_t tm = 0;

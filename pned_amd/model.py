@@ -29,7 +29,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
-from typing import Tuple
+from typing import Optional, Tuple
 
 import numpy as np
 
@@ -275,15 +275,18 @@ def local_mask(H: np.ndarray, q: int, tree: str = "s12") -> Tuple[np.ndarray, in
 
 
 def tps_shift(H: np.ndarray, q: int, ibond: int, c_count: int = 3,
-              tree: str = "s12") -> Tuple[np.ndarray, float, float]:
+              tree: str = "s12", lb: Optional[float] = None) -> Tuple[np.ndarray, float, float]:
     """``change_to_TPS_hamiltonian``: h_b <- l_b I - h_b.  Returns (H', l_b, L = ibond*l_b).
 
     S=1/2 (Hybrid_TPS_params.f90:1600-1623): l_b = sum_t sum_k |h_kk|^2 / c_count.
     spin-S (Hybrid_TPS9_params.f90:1660-1686): l_b = 2 sum_t sum_{k<4} S^2 |h_kk| / c_count / q^2,
     and only local indices 0..3 of each diagonal are shifted (reference quirk, kept).
+    ``lb`` given: that per-bond shift instead of the reference's formula.
     """
     Hs = -H.copy()
-    if tree == "s12":
+    if lb is not None:
+        pass
+    elif tree == "s12":
         lb = 0.0
         for t in range(9):
             for i in range(4):
@@ -333,9 +336,24 @@ class Operator:
         return np.ascontiguousarray(np.transpose(self.llocal, (0, 2, 1))).reshape(-1).astype(np.int32)
 
 
+def bond_norm_shift(cluster, model: int = 20, param: float = 0.0, q: int = 2, field: float = 0.0,
+                    factor: float = 1.2) -> float:
+    """A per-bond mTPQ shift l_b = factor * max_t ||h_t||_2 over the bond matrices in use:
+    then L = ibond * l_b >= factor * sum_b ||h_b|| bounds the spectrum of H from above, and
+    l = L / N = factor * max|bond sum| / N (no reference counterpart: the reference's l_b is
+    sum_t sum_k |h_kk|^2 / c_count)."""
+    tree = "s12" if q == 2 else "spin"
+    H = bond_matrices(set_lattice(model, param, tree), q, tree, field=field)
+    used = sorted({int(t) - 1 for t in cluster.ibond_type})
+    return factor * max(float(np.linalg.norm(H[t], 2)) for t in used)
+
+
 def build_operator(cluster, model: int = 20, param: float = 0.0, q: int = 2,
-                   tree: str = None, field: float = 0.0, tps: bool = False) -> Operator:
-    """Reference data_mem tables for a cluster/model; ``param`` in the reference's units."""
+                   tree: str = None, field: float = 0.0, tps: bool = False,
+                   tps_lb: Optional[float] = None) -> Operator:
+    """Reference data_mem tables for a cluster/model; ``param`` in the reference's units.
+    ``tps``: the shifted operator l_b - h_b of change_to_TPS_hamiltonian, with the
+    reference's l_b or, when given, ``tps_lb``."""
     if tree is None:
         tree = "s12" if q == 2 else "spin"
     c = set_lattice(model, param, tree)
@@ -343,5 +361,5 @@ def build_operator(cluster, model: int = 20, param: float = 0.0, q: int = 2,
     mask, parity = local_mask(H, q, tree)
     L = 0.0
     if tps:
-        H, lb, L = tps_shift(H, q, cluster.ibond, c.c_count, tree)
+        H, lb, L = tps_shift(H, q, cluster.ibond, c.c_count, tree, lb=tps_lb)
     return Operator(cluster.n, q, cluster.ipair, cluster.ibond_type, H, mask, parity, tree, L)
