@@ -93,6 +93,18 @@ module pned_gpu
       real(c_double), intent(out) :: out(2)
       integer(c_int) :: rc
     end function
+    function pned_vec_copy(ctx, src, dst) bind(C, name="pned_vec_copy") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, src, dst
+      integer(c_int) :: rc
+    end function
+    function pned_calc_beta(ctx, alpha, v1, v0, beta2) bind(C, name="pned_calc_beta") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, v1, v0
+      real(c_double), intent(in) :: alpha(2)
+      real(c_double), intent(out) :: beta2
+      integer(c_int) :: rc
+    end function
     function pned_calc_alpha_beta(ctx, v1, v0, alpha, beta) bind(C, name="pned_calc_alpha_beta") result(rc)
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx, v1, v0
@@ -322,12 +334,31 @@ contains
   end subroutine pned_check
 end module pned_gpu
 
-!! Reference-named entry points on device handles.
+!! Reference-named entry points on device handles.  Every routine keeps the reference's
+!! name and argument list, with type(c_ptr) vector handles where the reference passes
+!! complex(8) arrays.  What the reference keeps in modules -- parameters (max_itr, s_maxitr,
+!! i_range, split, delt, dsf_comp, itr_counter(9:10)), data_mem (site_vec, q_list) and
+!! working_area (wwk(:,1:4), alpha, beta, coef, salpha, sbeta) -- lives here: pned_init
+!! builds the context, pned_set_run the parameters, and the work vectors are created with
+!! the context.
 module pned_parallelized_diag
   use iso_c_binding
   use pned_gpu
   implicit none
   type(c_ptr), save :: pned_context = c_null_ptr
+  ! working_area (Hybrid_TPS_params.f90:272-311): wwk(:,1:4) on the device, the rest on the host
+  type(c_ptr), save :: wwk(4) = c_null_ptr
+  double precision, allocatable, save :: pned_alpha(:), pned_beta(:), pned_coef(:,:)
+  double precision, allocatable, save :: pned_salpha(:), pned_sbeta(:)
+  ! parameters / data_mem values the Krylov routines read (defaults: Hybrid_TPS_params.f90:34-42)
+  integer, save :: pned_max_itr = 1024, pned_s_maxitr = 4096, pned_i_range = 2000
+  integer, save :: pned_first = 1, pned_last = 1          ! itr_counter(9:10): momenta / samples
+  integer, save :: pned_nsites = 0, pned_nvec = 8
+  double precision, save :: pned_split = 2.5d-3, pned_delt = 1d-2
+  double precision, save :: localHam_max = 0d0            ! the TPS shift L
+  integer(c_int64_t), save :: pned_seed = 20261202_c_int64_t
+  character(len=2), save :: pned_dsf_comp = 'zz'
+  double precision, allocatable, save :: pned_site_vec(:,:), pned_q_list(:,:), pned_sq_q(:,:)
 contains
   subroutine pned_init(n, q, ibond, ipair, ibond_type, xlocal_ham, llocal_ham)
     integer, intent(in) :: n, q, ibond, ipair(:), ibond_type(:)
@@ -335,6 +366,34 @@ contains
     logical, intent(in) :: llocal_ham(:,:,:)
     integer(c_int), allocatable :: ll(:)
     real(c_double), allocatable :: xl(:)
+    integer :: k
+    call tables(q, xlocal_ham, llocal_ham, xl, ll)
+    call pned_check(pned_ctx_create(int(n, c_int), int(q, c_int), int(ibond, c_int), int(ipair, c_int), &
+                    int(ibond_type, c_int), xl, ll, 0_c_int, pned_context), 'pned_ctx_create')
+    pned_nsites = n
+    do k = 1, 4
+      wwk(k) = pned_new_vector()
+    end do
+  end subroutine pned_init
+
+  !! new bond matrices on the same cluster, e.g. after change_to_TPS_hamiltonian
+  !! (Hybrid_TPS_params.f90:1600-1623) rewrote xlocal_Ham
+  subroutine pned_update_matrices(q, xlocal_ham, llocal_ham)
+    integer, intent(in) :: q
+    complex(kind(0d0)), intent(in) :: xlocal_ham(:,:,:)
+    logical, intent(in) :: llocal_ham(:,:,:)
+    integer(c_int), allocatable :: ll(:)
+    real(c_double), allocatable :: xl(:)
+    call tables(q, xlocal_ham, llocal_ham, xl, ll)
+    call pned_check(pned_ctx_set_matrices(pned_context, xl, ll), 'pned_ctx_set_matrices')
+  end subroutine pned_update_matrices
+
+  subroutine tables(q, xlocal_ham, llocal_ham, xl, ll)
+    integer, intent(in) :: q
+    complex(kind(0d0)), intent(in) :: xlocal_ham(:,:,:)
+    logical, intent(in) :: llocal_ham(:,:,:)
+    real(c_double), allocatable, intent(out) :: xl(:)
+    integer(c_int), allocatable, intent(out) :: ll(:)
     integer :: i, j, k, m, qq
     qq = q*q
     allocate(xl(2*qq*qq*9), ll(qq*qq*9))
@@ -349,20 +408,72 @@ contains
         end do
       end do
     end do
-    call pned_check(pned_ctx_create(int(n, c_int), int(q, c_int), int(ibond, c_int), int(ipair, c_int), &
-                    int(ibond_type, c_int), xl, ll, 0_c_int, pned_context), 'pned_ctx_create')
-  end subroutine pned_init
+  end subroutine tables
+
+  !! the run parameters the reference reads from its modules: iteration limits, the DSF
+  !! frequency grid, the momentum (DSF) or sample (TPS) range, the cluster's site positions
+  !! and momenta, the TPS structure-factor momenta, the start-vector seed
+  subroutine pned_set_run(max_itr, s_maxitr, i_range, split, delt, first, last, nvec, dsf_comp, seed, &
+                          site_vec, q_list, sq_q)
+    integer, intent(in) :: max_itr, s_maxitr, i_range, first, last, nvec
+    double precision, intent(in) :: split, delt
+    character(len=2), intent(in) :: dsf_comp
+    integer(c_int64_t), intent(in) :: seed
+    double precision, intent(in) :: site_vec(:,:), q_list(:,:), sq_q(:,:)
+    pned_max_itr = max_itr
+    pned_s_maxitr = s_maxitr
+    pned_i_range = i_range
+    pned_split = split
+    pned_delt = delt
+    pned_first = first
+    pned_last = last
+    pned_nvec = nvec
+    pned_dsf_comp = dsf_comp
+    pned_seed = seed
+    if (allocated(pned_site_vec)) deallocate(pned_site_vec, pned_q_list, pned_sq_q)
+    allocate(pned_site_vec(2, size(site_vec, 2)), pned_q_list(2, size(q_list, 2)), pned_sq_q(2, size(sq_q, 2)))
+    pned_site_vec = site_vec
+    pned_q_list = q_list
+    pned_sq_q = sq_q
+    if (allocated(pned_alpha)) deallocate(pned_alpha, pned_beta, pned_coef, pned_salpha, pned_sbeta)
+    ! working_area shapes (Hybrid_TPS_params.f90:285-286); coef also holds 7+sq_num columns
+    allocate(pned_alpha(max_itr), pned_beta(max_itr), pned_coef(max_itr, max(nvec + 1, 7 + size(sq_q, 2))))
+    allocate(pned_salpha(s_maxitr), pned_sbeta(s_maxitr))
+    pned_alpha = 0d0
+    pned_beta = 0d0
+    pned_coef = 0d0
+    pned_salpha = 0d0
+    pned_sbeta = 0d0
+    call pned_check(pned_tps_set_observables(pned_context, pned_site_vec, int(size(sq_q, 2), c_int), pned_sq_q), &
+                    'tps_set_observables')
+  end subroutine pned_set_run
+
+  !! working_area arrays with the current parameters, when pned_set_run has not made them
+  subroutine ensure_work()
+    if (allocated(pned_alpha)) return
+    allocate(pned_alpha(pned_max_itr), pned_beta(pned_max_itr), pned_coef(pned_max_itr, pned_nvec + 1))
+    allocate(pned_salpha(pned_s_maxitr), pned_sbeta(pned_s_maxitr))
+    pned_alpha = 0d0
+    pned_beta = 0d0
+    pned_coef = 0d0
+    pned_salpha = 0d0
+    pned_sbeta = 0d0
+  end subroutine ensure_work
 
   function pned_new_vector() result(v)
     type(c_ptr) :: v
     call pned_check(pned_vec_create(pned_context, 1_c_int, PNED_C128, v), 'pned_vec_create')
   end function pned_new_vector
 
-  !! parallelized_set_initial_vec_p (Hybrid_TPS_parallel.f90:17-102), seeded Gaussian
+  !! parallelized_set_initial_vec_p (Hybrid_TPS_parallel.f90:17-102): seeded Gaussian
+  !! (seed: pned_set_run's, unless given)
   subroutine parallelized_set_initial_vec_p(v, seed)
     type(c_ptr), intent(in) :: v
-    integer(c_int64_t), intent(in) :: seed
-    call pned_check(pned_set_initial_vec(pned_context, v, seed), 'set_initial_vec')
+    integer(c_int64_t), intent(in), optional :: seed
+    integer(c_int64_t) :: s
+    s = pned_seed
+    if (present(seed)) s = seed
+    call pned_check(pned_set_initial_vec(pned_context, v, s), 'set_initial_vec')
   end subroutine
 
   !! p_mltply_v3 (Hybrid_TPS_parallel.f90:190-263): v0 = v0 + (H - eperbd) v1
@@ -375,8 +486,9 @@ contains
     prdct = dcmplx(p(1), p(2))
   end subroutine
 
-  !! calc_vec_inprdct (Hybrid_TPS_parallel.f90:104-126)
-  subroutine calc_vec_inprdct(x, y, norm)
+  !! calc_vec_inprdct (Hybrid_TPS_parallel.f90:104-126): norm = sum conj(x) y
+  subroutine calc_vec_inprdct(ldim, x, y, norm)
+    integer, intent(in) :: ldim
     type(c_ptr), intent(in) :: x, y
     complex(kind(0d0)), intent(inout) :: norm
     real(c_double) :: p(2)
@@ -384,24 +496,181 @@ contains
     norm = dcmplx(p(1), p(2))
   end subroutine
 
-  !! calc_alpha_beta (Hybrid_TPS_parallel.f90:145-159)
-  subroutine calc_alpha_beta(t_alp, t_bet, v1, v0)
+  !! calc_beta (Hybrid_TPS_parallel.f90:161-188): beta = ||v0 - alpha v1||^2
+  subroutine calc_beta(data_num, alpha, beta, v1, v0)
+    integer, intent(in) :: data_num
+    complex(kind(0d0)), intent(in) :: alpha
+    double precision, intent(inout) :: beta
+    type(c_ptr), intent(in) :: v1, v0
+    real(c_double) :: a(2), b2
+    a = [dble(alpha), dimag(alpha)]
+    call pned_check(pned_calc_beta(pned_context, a, v1, v0, b2), 'calc_beta')
+    beta = b2
+  end subroutine
+
+  !! calc_alpha_beta (Hybrid_TPS_parallel.f90:145-159): p_mltply_v3 then calc_beta
+  subroutine calc_alpha_beta(ldim, t_alp, t_bet, v1, v0)
+    integer, intent(in) :: ldim
     double precision, intent(inout) :: t_alp, t_bet
     type(c_ptr), intent(in) :: v1, v0
     call pned_check(pned_calc_alpha_beta(pned_context, v1, v0, t_alp, t_bet), 'calc_alpha_beta')
   end subroutine
 
-  !! parallelized_lnc1 (Hybrid_TPS_Lanczos.f90:7-45) from the start vector in v1
-  subroutine parallelized_lnc1(nvec2, itr, ene, v1, v0, alpha, beta, max_itr)
-    integer, intent(in) :: nvec2, max_itr
-    integer, intent(inout) :: itr
-    double precision, intent(inout) :: ene(:), alpha(:), beta(:)
+  !! calc_alpha_beta2 (Hybrid_TPS_parallel.f90:128-142): p_mltply_v3 then ||v0||
+  subroutine calc_alpha_beta2(ldim, t_alp, t_bet, v1, v0)
+    integer, intent(in) :: ldim
+    double precision, intent(inout) :: t_alp, t_bet
     type(c_ptr), intent(in) :: v1, v0
+    call pned_check(pned_calc_alpha_beta2(pned_context, v1, v0, t_alp, t_bet), 'calc_alpha_beta2')
+  end subroutine
+
+  !! parallelized_lnc1 (Hybrid_TPS_Lanczos.f90:7-45): Lanczos eigenvalues from the seeded
+  !! start vector, or from x when restart; wwk(1:2) are the Lanczos vectors, alpha/beta go
+  !! to pned_alpha/pned_beta (working_area)
+  subroutine parallelized_lnc1(nvec2, itr, ene, x, restart)
+    integer, intent(in) :: nvec2
+    integer, intent(inout) :: itr
+    double precision :: ene(:)
+    type(c_ptr), intent(in) :: x
+    logical, intent(in) :: restart
     integer(c_int) :: it
-    call pned_check(pned_lnc1(pned_context, v1, v0, int(max_itr, c_int), int(nvec2, c_int), 0_c_int, &
-                    alpha, beta, ene, it), 'lnc1')
+    if (nvec2 < 0) then
+      print *, ' #(E06)# Wrong value given to nvec in lnc1'
+      print *, '         Only the eigenvalues are calculated'
+      stop
+    end if
+    call ensure_work()
+    if (.not. restart) then
+      call parallelized_set_initial_vec_p(wwk(1))
+    else
+      call pned_check(pned_vec_copy(pned_context, x, wwk(1)), 'lnc1')
+    end if
+    call pned_check(pned_lnc1(pned_context, wwk(1), wwk(2), int(pned_max_itr, c_int), int(nvec2, c_int), 0_c_int, &
+                    pned_alpha, pned_beta, ene, it), 'lnc1')
     itr = it
   end subroutine
+
+  !! parallelized_inv1 (Hybrid_TPS_cg.f90:7-16): inverse iteration (inv1z + cg1, the
+  !! reference's SAVEd loop flag kept) for the eigenvector of Eig into x; the start vector
+  !! is seeded, or x itself when restart
+  subroutine parallelized_inv1(Eig, x, restart)
+    double precision, intent(inout) :: Eig
+    type(c_ptr), intent(in) :: x
+    logical, intent(in) :: restart
+    integer(c_int) :: iters
+    if (.not. restart) then
+      call parallelized_set_initial_vec_p(wwk(1))
+    else
+      call pned_check(pned_vec_copy(pned_context, x, wwk(1)), 'inv1')
+    end if
+    call pned_check(pned_inv1(pned_context, Eig, x, wwk(1), wwk(2), wwk(3), wwk(4), 5000_c_int, 500_c_int, 1_c_int, &
+                    iters), 'inv1')
+  end subroutine
+
+  !! check1 (Hybrid_TPS_check.f90:8-54): <x|H|x> and (Hx)_j/x_j appended to
+  !! acc_log_me00000.txt with the reference's format statements
+  subroutine check1(x)
+    type(c_ptr), intent(in) :: x
+    real(c_double) :: expect(2), ratios(2*4096)
+    integer(c_int) :: nr
+    integer :: i, u
+    call pned_check(pned_check1(pned_context, x, wwk(1), expect, ratios, 4096_c_int, nr), 'check1')
+    open(newunit=u, file='acc_log_me00000.txt', status='unknown', access='append')
+    write(u, *)
+    write(u, 200)
+200 format(' ---------------------------- Information from check1')
+    write(u, 210) dcmplx(expect(1), expect(2))
+210 format(' <x*H*x> =(', 1pe16.8, ',', 1pe16.8, ')')
+    write(u, 220)
+220 format(' H*x(j)/x(j) (j=min(ldim/3,13),ldim,max(1,ldim/20))')
+    write(u, 230) (dcmplx(ratios(2*i-1), ratios(2*i)), i = 1, nr)
+230 format(4('(', 1pe16.6e3, ',', 1pe16.6e3, ')'))
+    write(u, 240)
+240 format(' ---------------------------------------------------')
+    close(u)
+  end subroutine check1
+
+  !! parallelized_TPS (Hybrid_TPS_TPS.f90:45-81): one mTPQ run of pned_max_itr steps per
+  !! sample pned_first..pned_last (the reference reuses the momentum range as the sample
+  !! range), sample i starting from seed pned_seed + i; the operator must already be the
+  !! shifted one (pned_update_matrices after change_to_TPS_hamiltonian) and localHam_max its
+  !! shift L.  Each sample's alpha, beta, coef go to SalphaSbeta_me00000seed<i><dsf_comp>.
+  subroutine parallelized_TPS(nvec2, itr, ene, restart)
+    integer, intent(in) :: nvec2
+    integer, intent(inout) :: itr
+    double precision :: ene(:)
+    logical, intent(in) :: restart
+    integer :: i, k, m, ncoef
+    real(c_double), allocatable :: cf(:,:)
+    if (nvec2 < 0) then
+      print *, ' #(E06)# Wrong value given to nvec in lnc1'
+      print *, '         Only the eigenvalues are calculated'
+      stop
+    end if
+    write(*, *) 'start TPS'
+    ncoef = 7 + size(pned_sq_q, 2)
+    allocate(cf(ncoef, pned_max_itr))
+    do i = pned_first, pned_last
+      if (.not. restart) call parallelized_set_initial_vec_p(wwk(1), pned_seed + int(i, c_int64_t))
+      call pned_check(pned_tps(pned_context, wwk(1), wwk(2), wwk(3), wwk(4), localHam_max, &
+                      int(pned_max_itr, c_int), pned_alpha, pned_beta, cf), 'TPS')
+      pned_coef = 0d0
+      do k = 1, pned_max_itr
+        do m = 1, ncoef
+          pned_coef(k, m) = cf(m, k)
+        end do
+      end do
+      itr = max(pned_max_itr - 1, int(pned_ctx_dim(pned_context)))
+      call dump_alpha_beta_coef(pned_dsf_comp, i, itr + 1, wwk(2), wwk(1), wwk(3))
+    end do
+  end subroutine parallelized_TPS
+
+  !! calc_DSF (Hybrid_TPS_DSF.f90:446-512): for momenta pned_first..pned_last of the q_list,
+  !! the excitation (S^z(q) for 'zz', S^+(q) for 'xx') of x, sq.txt, the continued-fraction
+  !! Lanczos (s_maxitr steps), its files (dump_alpha_beta), the spectrum on i_range
+  !! frequencies normalised to S(q), and dsf_res_<comp>.datcl<i_color>lme<Iam>
+  subroutine calc_DSF(Iam, ene, x, i_color)
+    integer, intent(in) :: Iam, i_color
+    double precision, intent(in) :: ene
+    type(c_ptr), intent(in) :: x
+    integer :: i, u
+    double precision :: q(2), w(pned_i_range), dsf(pned_i_range)
+    real(c_double) :: cn
+    real(c_double), allocatable :: sa(:), sb(:)
+    integer(c_int) :: conv(pned_i_range)
+    complex(kind(0d0)) :: norm
+    character(len=64) :: ident, fres
+    if (Iam == 0) write(*, *) 'start : calc_DSF'
+    write(ident, '(A,I5.5,A,I5.5)') 'cl', i_color, 'lme', Iam
+    fres = 'dsf_res_'//pned_dsf_comp//'.dat'//trim(ident)
+    allocate(sa(pned_s_maxitr), sb(pned_s_maxitr + 1))
+    do i = pned_first, pned_last
+      q(1:2) = pned_q_list(1:2, i)
+      write(*, '(A,I3,2F12.6)') ' q = ', i, q(1), q(2)
+      if (pned_dsf_comp == 'zz') then
+        call pned_check(pned_intalv_zz(pned_context, x, wwk(1), q, pned_site_vec, cn), 'intalv_zz')
+      else if (pned_dsf_comp == 'xx') then
+        call pned_check(pned_intalv_xy(pned_context, x, wwk(1), q, pned_site_vec, 1_c_int, cn), 'intalv_xy')
+      else
+        write(*, *) 'Error: DSF comp.'
+        stop
+      end if
+      norm = dcmplx(cn, 0d0)
+      if (Iam == 0) then
+        open(newunit=u, file='sq.txt', status='unknown', access='append')
+        write(u, '(A,I3,2F12.6,F25.18)') ' q = ', i, q(1), q(2), dble(norm)
+        close(u)
+      end if
+      call pned_check(pned_dsflnc1z(pned_context, wwk(1), wwk(2), cn, int(pned_s_maxitr, c_int), sa, sb), 'dsflnc1z')
+      pned_salpha = sa(1:pned_s_maxitr)
+      pned_sbeta = sb(1:pned_s_maxitr)
+      call dump_alpha_beta(i, pned_s_maxitr - 1, norm, pned_dsf_comp, wwk(1), wwk(2))
+      call pned_check(pned_renbun(pned_context, int(pned_i_range, c_int), int(pned_s_maxitr, c_int), pned_split, &
+                      ene, pned_delt, sa, sb, cn, w, dsf, conv), 'renbun')
+      if (Iam == 0) call pned_check(pned_write_dsf_res(trim(fres)//c_null_char, merge(0_c_int, 1_c_int, &
+                                    i == pned_first), cn, q, int(pned_i_range, c_int), w, dsf, conv), 'write_dsf_res')
+    end do
+  end subroutine calc_DSF
 
   !! restrict the context to the S^z sector with n_down down spins (parity_type = 0;
   !! the index tables of mk_list12, Hybrid_TPS_parallel.f90:452-578)
@@ -426,57 +695,63 @@ contains
     end if
   end subroutine
 
-  !! dump_alpha_beta_coef (Hybrid_TPS_TPS.f90:507-538): the mTPQ norms and observables of
-  !! one sample, file SalphaSbeta_me<pme>seed<indx1><comp>
-  subroutine dump_alpha_beta_coef(comp, indx1, pme, alpha, beta, coef)
+  !! dump_alpha_beta_coef (Hybrid_TPS_TPS.f90:507-538): alpha, beta, coef (working_area) to
+  !! SalphaSbeta_me00000seed<indx1><comp>; the vectors are not written (the reference's
+  !! vector dump is commented out)
+  subroutine dump_alpha_beta_coef(comp, indx1, indx2, v1, v2, v3)
     character(len=2), intent(in) :: comp
-    integer, intent(in) :: indx1, pme
-    double precision, intent(in) :: alpha(:), beta(:), coef(:,:)
+    integer, intent(in) :: indx1, indx2
+    type(c_ptr), intent(in) :: v1, v2, v3
     character(len=128) :: fname
-    write(fname, '(a,i5.5,a,i5.5,a)') 'SalphaSbeta_me', pme, 'seed', indx1, trim(comp)
-    call pned_check(pned_dump_alpha_beta_coef(trim(fname)//c_null_char, int(ubound(alpha, 1), c_int), &
-                    int(ubound(coef, 2), c_int), alpha, beta, coef), 'dump_alpha_beta_coef')
+    write(fname, '(a,i5.5,a,i5.5,a)') 'SalphaSbeta_me', 0, 'seed', indx1, trim(comp)
+    call pned_check(pned_dump_alpha_beta_coef(trim(fname)//c_null_char, int(ubound(pned_alpha, 1), c_int), &
+                    int(ubound(pned_coef, 2), c_int), pned_alpha, pned_beta, pned_coef), 'dump_alpha_beta_coef')
   end subroutine
 
-  !! dump_alpha_beta (Hybrid_TPS_DSF.f90:353-379): continued-fraction file
-  !! SalphaSbeta_q<iq>me<pme><comp> and the vector file vec_q<iq>me<pme><comp>
-  subroutine dump_alpha_beta(iq, pme, indx, sq, comp, salpha, sbeta, v1, v2)
-    integer, intent(in) :: iq, pme, indx
+  !! dump_alpha_beta (Hybrid_TPS_DSF.f90:353-379): SalphaSbeta_q<iq>me00000<comp> (sq,
+  !! salpha, sbeta of working_area) and vec_q<iq>me00000<comp> (indx and two vectors)
+  subroutine dump_alpha_beta(iq, indx, sq, comp, v1, v2)
+    integer, intent(in) :: iq, indx
     complex(kind(0d0)), intent(in) :: sq
     character(len=2), intent(in) :: comp
-    double precision, intent(in) :: salpha(:), sbeta(:)
     type(c_ptr), intent(in) :: v1, v2
     character(len=128) :: fname
     real(c_double) :: s(2)
     type(c_ptr) :: vv(2)
     s = [dble(sq), dimag(sq)]
-    write(fname, '(a,i5.5,a,i5.5,a)') 'SalphaSbeta_q', iq, 'me', pme, trim(comp)
-    call pned_check(pned_dump_dsf_alpha_beta(trim(fname)//c_null_char, s, int(size(salpha), c_int), salpha, &
-                    int(size(sbeta), c_int), sbeta), 'dump_alpha_beta')
-    write(fname, '(a,i5.5,a,i5.5,a)') 'vec_q', iq, 'me', pme, trim(comp)
+    write(fname, '(a,i5.5,a,i5.5,a)') 'SalphaSbeta_q', iq, 'me', 0, trim(comp)
+    call pned_check(pned_dump_dsf_alpha_beta(trim(fname)//c_null_char, s, int(size(pned_salpha), c_int), &
+                    pned_salpha, int(size(pned_sbeta), c_int), pned_sbeta), 'dump_alpha_beta')
+    write(fname, '(a,i5.5,a,i5.5,a)') 'vec_q', iq, 'me', 0, trim(comp)
     vv = [v1, v2]
     call pned_check(pned_dump_vectors(trim(fname)//c_null_char, int(indx, c_int), 2_c_int, vv), 'dump_alpha_beta')
   end subroutine
 
-  !! read_alpha_beta2 (Hybrid_TPS_DSF.f90:382-419): the restart counterpart of dump_alpha_beta
-  subroutine read_alpha_beta2(iq, pme, indx, sq, comp, salpha, sbeta, v1, v2)
-    integer, intent(in) :: iq, pme
+  !! read_alpha_beta2 (Hybrid_TPS_DSF.f90:382-419): the restart counterpart of dump_alpha_beta;
+  !! salpha/sbeta get the file's first indx+1 entries, the rest zero
+  subroutine read_alpha_beta2(iq, indx, sq, comp, v1, v2)
+    integer, intent(in) :: iq
     integer, intent(inout) :: indx
     complex(kind(0d0)), intent(inout) :: sq
     character(len=2), intent(in) :: comp
-    double precision, intent(inout) :: salpha(:), sbeta(:)
     type(c_ptr), intent(in) :: v1, v2
     character(len=128) :: fname
     real(c_double) :: s(2)
+    real(c_double), allocatable :: ta(:), tb(:)
     integer(c_int) :: ix
     type(c_ptr) :: vv(2)
-    write(fname, '(a,i5.5,a,i5.5,a)') 'SalphaSbeta_q', iq, 'me', pme, trim(comp)
-    call pned_check(pned_read_dsf_alpha_beta(trim(fname)//c_null_char, s, int(size(salpha), c_int), salpha, &
-                    int(size(sbeta), c_int), sbeta), 'read_alpha_beta2')
-    sq = dcmplx(s(1), s(2))
-    write(fname, '(a,i5.5,a,i5.5,a)') 'vec_q', iq, 'me', pme, trim(comp)
+    write(fname, '(a,i5.5,a,i5.5,a)') 'vec_q', iq, 'me', 0, trim(comp)
     vv = [v1, v2]
     call pned_check(pned_read_vectors(trim(fname)//c_null_char, ix, 2_c_int, vv), 'read_alpha_beta2')
     indx = ix
+    allocate(ta(indx + 1), tb(indx + 1))
+    write(fname, '(a,i5.5,a,i5.5,a)') 'SalphaSbeta_q', iq, 'me', 0, trim(comp)
+    call pned_check(pned_read_dsf_alpha_beta(trim(fname)//c_null_char, s, int(indx + 1, c_int), ta, &
+                    int(indx + 1, c_int), tb), 'read_alpha_beta2')
+    sq = dcmplx(s(1), s(2))
+    pned_salpha = 0d0
+    pned_sbeta = 0d0
+    pned_salpha(1:indx + 1) = ta
+    pned_sbeta(1:indx + 1) = tb
   end subroutine
 end module pned_parallelized_diag

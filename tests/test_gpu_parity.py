@@ -389,16 +389,35 @@ def test_cocg_fused_shift_equals_generic_path():
     assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x2)
 
 
-def test_fortran_host_driver(tmp_path):
-    """The ISO_C_BINDING layer end to end: a Fortran host reads the data_mem tables and
-    runs lnc1 + p_mltply_v3 through pned_parallelized_diag."""
-    _say("fortran driver")
+def _fortran_records(path):
+    """Records of a Fortran unformatted sequential file (4-byte markers, no subrecords)."""
+    raw = open(path, "rb").read()
+    out, k = [], 0
+    while k < len(raw):
+        n = int(np.frombuffer(raw[k:k + 4], dtype=np.int32)[0])
+        out.append(raw[k + 4:k + 4 + n])
+        k += n + 8
+    return out
+
+
+def test_fortran_reference_call_sequence(tmp_path):
+    """The ISO_C_BINDING layer as a drop-in: a Fortran host runs the reference's call
+    sequence (S_one_half/Hybrid_TPS_main.f90:89-143) through the same-name wrappers --
+    parallelized_lnc1, parallelized_inv1, readwrite_vec_data, check1, calc_DSF, then
+    change_to_TPS_hamiltonian and parallelized_TPS -- and every printed value and result
+    file is checked against the oracle run on the same start vectors."""
+    _say("fortran reference call sequence")
     import os
+    import re
     import subprocess
-    exe = os.path.join(os.path.dirname(gpu.LIB_PATH), "fortran", "_build", "pned_lanczos_driver")
+    exe = os.path.join(os.path.dirname(gpu.LIB_PATH), "fortran", "_build", "pned_main_driver")
     assert os.path.exists(exe), "build the Fortran layer first (__graft_entry__.build())"
-    cl = lattice.reference_cluster(16)
-    op = model.build_operator(cl, 20, 126.0, field=0.1)
+    cl = lattice.reference_cluster(12)
+    op = model.build_operator(cl, 20, 54.0, field=0.1)
+    opt = model.build_operator(cl, 20, 54.0, field=0.1, tps=True)
+    seed, max_itr, s_maxitr, i_range, first, last, nvec = 20261202, 400, 40, 200, 2, 3, 8
+    split, delt = 2.5e-2, 0.05
+    sq_q = cl.sq_momenta()
     f = tmp_path / "tables.bin"
     with open(f, "wb") as fh:
         np.array([op.n, op.q, op.ibond], dtype=np.int32).tofile(fh)
@@ -406,17 +425,70 @@ def test_fortran_host_driver(tmp_path):
         np.asarray(op.ibond_type, dtype=np.int32).tofile(fh)
         op.fortran_xlocal().astype(np.complex128).tofile(fh)
         op.fortran_llocal().astype(np.int32).tofile(fh)
-    out = subprocess.run([exe, str(f)], capture_output=True, text=True, timeout=120)
-    assert out.returncode == 0, out.stdout + out.stderr
-    vals = dict(line.split(None, 1) for line in out.stdout.strip().splitlines())
+        opt.fortran_xlocal().astype(np.complex128).tofile(fh)
+        opt.fortran_llocal().astype(np.int32).tofile(fh)
+        np.array([opt.shift_L]).tofile(fh)
+        np.array([len(cl.q_list)], dtype=np.int32).tofile(fh)
+        np.ascontiguousarray(cl.q_list, dtype=np.float64).tofile(fh)
+        np.array([len(sq_q)], dtype=np.int32).tofile(fh)
+        np.ascontiguousarray(sq_q, dtype=np.float64).tofile(fh)
+        np.ascontiguousarray(cl.site_vec, dtype=np.float64).tofile(fh)
+        np.array([max_itr, s_maxitr, i_range, first, last, nvec], dtype=np.int32).tofile(fh)
+        np.array([split, delt]).tofile(fh)
+        np.array([seed], dtype=np.int64).tofile(fh)
+    out = subprocess.run([exe, str(f)], capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert out.returncode == 0 and "DONE" in out.stdout, out.stdout + out.stderr
+    vals = dict(line.split(None, 1) for line in out.stdout.splitlines() if line.split()[:1] in (["E0"], ["ITR"]))
+    oop = oracle.OracleOp(op)
+    d = op.dim
+    g = oracle.gaussian(d, seed)
+    itr, _, _, ene = oracle.lanczos(oop, g, max_itr=max_itr, nvec=nvec)
     e0 = float(vals["E0"])
-    x = oracle.gaussian(2 ** 16, 20261202)
-    itr, a, b, ene = oracle.lanczos(oracle.OracleOp(op), x, max_itr=1024, nvec=8)
     assert abs(e0 - ene[0]) <= 1e-10 * abs(ene[0])
-    pr = complex(*map(float, vals["PRDCT"].split()))
-    want = np.zeros_like(x)
-    p_ref = oracle.hv(oracle.OracleOp(op), x, want)
-    assert abs(pr - p_ref) <= 1e-12 * abs(p_ref)
+    assert int(vals["ITR"]) == itr
+    # inverse iteration from the same seeded start: basisme00000 holds E0 and x
+    rec = _fortran_records(tmp_path / "basisme00000")
+    assert np.frombuffer(rec[0], dtype=np.float64)[0] == e0
+    x = np.frombuffer(rec[1], dtype=np.complex128)
+    _, x_ref = oracle.inverse_iteration(oop, e0, g)
+    assert abs(abs(np.vdot(x_ref, x)) - 1.0) < 1e-10
+    # check1's log: <x|H|x> and the ratios (Hx)_j / x_j
+    log = (tmp_path / "acc_log_me00000.txt").read_text()
+    m = re.search(r"<x\*H\*x> =\(\s*([-+.\dEe]+),\s*([-+.\dEe]+)\)", log)
+    hx = np.zeros_like(x)
+    oracle.hv(oop, x, hx)
+    assert abs(float(m.group(1)) - np.vdot(x, hx).real) <= 2e-8 * abs(e0)     # 1pe16.8
+    nums = [complex(float(a), float(b)) for a, b in re.findall(r"\(\s*([-+.\dEe]+),\s*([-+.\dEe]+)\)",
+                                                                log.split("H*x(j)/x(j)")[1])]
+    js = np.array(list(range(min(d // 3, 13), d + 1, max(1, d // 20))))
+    assert np.allclose(nums, hx[js - 1] / x[js - 1], rtol=2e-6)                # 1pe16.6
+    # calc_DSF at momenta first..last: sq.txt, SalphaSbeta_q*, dsf_res_zz.dat*
+    sq_lines = (tmp_path / "sq.txt").read_text().splitlines()
+    blocks = (tmp_path / "dsf_res_zz.datcl00000lme00000").read_text().split("#")[1:]
+    assert len(sq_lines) == len(blocks) == last - first + 1
+    for k, iq in enumerate(range(first, last + 1)):
+        qv = cl.q_list[iq - 1]
+        vx = oracle.sz_q(cl.n, 2, cl.site_vec, qv, x)
+        cn = np.vdot(vx, vx).real / cl.n
+        assert abs(float(sq_lines[k].split()[-1]) - cn) <= 1e-15 + 1e-12 * cn
+        sq, sa, sb = gpu.read_dsf_alpha_beta(tmp_path / f"SalphaSbeta_q{iq:05d}me00000zz", s_maxitr, s_maxitr)
+        sa_ref, sb_ref = oracle.dsf_lanczos(oop, vx / np.sqrt(cn * cl.n), cn, s_maxitr)
+        assert np.allclose(sa[:20], sa_ref[:20], rtol=1e-10)
+        assert np.allclose(sb[:20], sb_ref[:20], rtol=1e-10)
+        w2, d2, c2 = oracle.renbun(sa_ref, sb_ref, e0, delt, i_range, split, s_maxitr)
+        d2 = oracle.normalize_dsf(d2, split, cn)
+        rows = np.array([[float(t) for t in ln.split()] for ln in blocks[k].splitlines()[1:] if ln.strip()])
+        assert rows.shape == (i_range, 5)
+        assert np.allclose(rows[:, 2], w2, atol=1e-10)
+        assert np.allclose(rows[:, 3], d2, rtol=1e-7, atol=2e-16 * np.abs(d2).max() + 1e-15)
+    # parallelized_TPS: one SalphaSbeta_me00000seed<i>zz per sample, start vector seed + i
+    oopt = oracle.OracleOp(opt)
+    for i in range(first, last + 1):
+        a, b, c = gpu.read_alpha_beta_coef(tmp_path / f"SalphaSbeta_me00000seed{i:05d}zz")
+        a_ref, b_ref, c_ref = oracle.tps(oopt, cl.site_vec, sq_q, opt.shift_L, max_itr, oracle.gaussian(d, seed + i))
+        assert np.allclose(a, a_ref, rtol=1e-11)
+        assert np.allclose(b, b_ref, rtol=1e-11)
+        assert np.allclose(c[:, :c_ref.shape[1]], c_ref, rtol=1e-9, atol=1e-12)
 
 
 @pytest.mark.parametrize("dtype,tol", [(gpu.C128, 1e-10), (gpu.C64, 2e-4)])

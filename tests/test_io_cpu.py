@@ -169,16 +169,20 @@ program bind
   use pned_parallelized_diag
   implicit none
   integer :: i, k
-  double precision :: alpha(12), beta(12), coef(12,9), out(4*5)
+  double precision :: out(4*5)
+  ! working_area arrays of the module, filled as the mTPQ loop fills them
+  allocate(pned_alpha(12), pned_beta(12), pned_coef(12, 9))
   do i = 1, 12
-    alpha(i) = -0.1d0 * i
-    beta(i) = 1d0 + 0.01d0 * i
+    pned_alpha(i) = -0.1d0 * i
+    pned_beta(i) = 1d0 + 0.01d0 * i
     do k = 1, 9
-      coef(i,k) = -0.5d0 + 0.02d0 * i - 0.001d0 * k * i
+      pned_coef(i,k) = -0.5d0 + 0.02d0 * i - 0.001d0 * k * i
     end do
   end do
-  call dump_alpha_beta_coef('tp', 1, 0, alpha, beta, coef)
-  call pned_check(pned_ctpq_eval(12_c_int, 9_c_int, 1_c_int, alpha, beta, coef, 0.5d0, 4d0, 4_c_int, out), 'ctpq')
+  ! the reference's argument list (comp, indx1, indx2, v1, v2, v3); the vectors are not written
+  call dump_alpha_beta_coef('tp', 1, 13, c_null_ptr, c_null_ptr, c_null_ptr)
+  call pned_check(pned_ctpq_eval(12_c_int, 9_c_int, 1_c_int, pned_alpha, pned_beta, pned_coef, 0.5d0, 4d0, &
+                  4_c_int, out), 'ctpq')
   write(*,'(ES25.17)') out
 end program bind
 """
