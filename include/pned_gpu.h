@@ -205,6 +205,18 @@ int pned_lanczos_update(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, const double 
 int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, int fixed_steps,
               double *alpha, double *beta, double *ene, int *itr);
 
+/* Resumable Lanczos (no separate reference entry; the restart branch of lnc1z,
+ * Hybrid_TPS_Lanczos.f90:87-93): the lnc1z loop from step `first` with the reference's
+ * explicit update, so that the loop state at any step is the reference's vectors.
+ * first <= 2 starts from the normalised v1; first > 2 continues from the loop state of the
+ * top of step `first` (v1, v0 and alpha/beta[0..first-2], e.g. after
+ * pned_interval_dump_read).  stop >= max(first,2) returns at the top of step `stop` with
+ * *itr = stop and the loop state in v1, v0 (what interval_dump writes); stop = 0 runs to
+ * convergence, max_itr or fixed_steps as pned_lnc1 does.  A run split into spans gives the
+ * same alpha/beta bit for bit as one span. */
+int pned_lnc1_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int first, int stop, int max_itr, int nvec,
+                   int fixed_steps, double *alpha, double *beta, double *ene, int *itr);
+
 /* Serving (no reference counterpart): one Lanczos step of every slot (column) of a batch.
  * mode[c]: 0 idle (column zeroed, alpha = beta = 0), 1 first step (v1 holds the start
  * vector, v0 is cleared; the lnc1z start, Hybrid_TPS_Lanczos.f90:78-85), 2 continue (the
@@ -274,6 +286,13 @@ int pned_tps_set_observables(pned_ctx *ctx, const double *site_vec, int nsq, con
 int pned_tps(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, pned_vec *vt, double lmax,
              int nsteps, double *alpha, double *beta, double *coef);
 
+/* pned_tps from step `first` (the TPSz restart branch, Hybrid_TPS_TPS.f90:352-358):
+ * first > 1 continues from the loop state of the top of step `first` (v0 = (L-H) v1 of the
+ * step before, v1, vg; rows 0..first-2 of alpha/beta/coef filled); stop >= first returns at
+ * the top of step `stop`.  Split runs give the rows of one span bit for bit. */
+int pned_tps_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, pned_vec *vt, double lmax, int first,
+                  int stop, int nsteps, double *alpha, double *beta, double *coef);
+
 /* Same outputs as pned_tps with the vector work fused (rotation instead of copies, the
  * 1/beta normalisation folded into the next H.v, one reduction pass per step).  v1 holds
  * the normalised start columns; va, vb are work vectors; on return all three hold work
@@ -299,6 +318,14 @@ int pned_intalv_xy(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double q
  * normalised excitation in v1; salpha[s_maxitr], sbeta[s_maxitr+1] (sbeta[0] = cnorm). */
 int pned_dsflnc1z(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double cnorm, int s_maxitr,
                   double *salpha, double *sbeta);
+/* dsflnc1z from step `first` with the reference's explicit update (restart after
+ * interval_dump_read, Hybrid_TPS_DSF.f90:213-225, and the COMP=zr/xr extension of
+ * calc_extended_DSF, :515-553, which reads SalphaSbeta_q* and vec_q* and continues from
+ * first = indx + 1): first > 2 continues from v1, v0 of the top of step `first` with
+ * salpha[0..first-2], sbeta[0..first-1]; runs to step s_maxitr - 1 or returns at the top of
+ * step `stop` >= max(first, 2); *icf = last step taken.  Split runs equal one span bit for bit. */
+int pned_dsflnc1z_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double cnorm, int first, int stop, int s_maxitr,
+                       double *salpha, double *sbeta, int *icf);
 /* exec_renbun (Hybrid_TPS_DSF.f90:263-334) on the GPU, one thread per omega, followed by
  * nomalization_DSF (337-351) when sq > 0.  w/dsf/conv have i_range entries.              */
 int pned_renbun(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,

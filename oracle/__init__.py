@@ -62,9 +62,9 @@ def lib():
         L.orc_sz_q.argtypes = [i32, i32, _dp, _dp, _zp, _zp]
         L.orc_intalv_xy.argtypes = [i32, _dp, _dp, i64, _up, _zp, i64, _up, _up, _zp]
         L.orc_intalv_pm.argtypes = [i32, i32, _dp, _dp, i32, _zp, _zp]
-        L.orc_sq_apply.argtypes = [i32, _dp, _dp, i32, _zp, _zp]
-        L.orc_zz_bond_apply.argtypes = [i32, i32, _ip, _ip, _zp, _zp]
-        L.orc_tps.argtypes = [i32, i32, _ip, _ip, _ip, _zp, _bp, _dp, i32, _dp, f64, i32,
+        L.orc_sq_apply.argtypes = [i32, i32, _dp, _dp, i32, _zp, _zp]
+        L.orc_zz_bond_apply.argtypes = [i32, i32, i32, _ip, _ip, _zp, _zp]
+        L.orc_tps.argtypes = [i32, i32, i32, _ip, _ip, _ip, _zp, _bp, _dp, i32, _dp, f64, i32,
                               _zp, _zp, _zp, _zp, _dp, _dp, _dp]
         L.orc_dsf_lanczos.argtypes = [i32, i32, i32, _ip, _ip, _zp, _bp, f64, i32, _zp, _zp, _dp, _dp]
         L.orc_dsf_lanczos.restype = i32
@@ -269,7 +269,7 @@ def tps(oop: OracleOp, site_vec, sq_q, lmax, nsteps, v_start):
     alpha = np.zeros(nsteps)
     beta = np.zeros(nsteps)
     coef = np.zeros(nsteps * (7 + nsq))
-    lib().orc_tps(oop.n, oop.ibond, oop.site_pair, oop.btype, oop.ipair, oop.xl, oop.mk,
+    lib().orc_tps(oop.n, oop.q, oop.ibond, oop.site_pair, oop.btype, oop.ipair, oop.xl, oop.mk,
                   np.ascontiguousarray(site_vec, dtype=np.float64).reshape(-1),
                   nsq, sq_q if nsq else np.zeros(2), float(lmax), nsteps, v1, v0, vg, vt, alpha, beta, coef)
     return alpha, beta, coef.reshape(nsteps, 7 + nsq)

@@ -634,11 +634,34 @@ void orc_intalv_pm(int n, int q, const double *site_vec, const double qv[2], int
     }
 }
 
-/* calc_sq_zz / calc_sq_pp  H/TPS.f90:146-251 (S=1/2): mode 0: (s_j-1/2)(s_0-1/2) e^{iqr_j};
- * mode 1: [s_j != s_0]/4 e^{iqr_j}. */
-void orc_sq_apply(int n, const double *site_vec, const double qv[2], int mode, const zc *x, zc *v)
+/* calc_sq_zz / calc_sq_pp  H/TPS.f90:146-251 (S=1/2 tree), S/TPS.f90:152-247 (spin-S tree).
+ * Both trees weight bit j of the state index (iand(ishft(istate,-j),1); in the spin-S tree
+ * these are bits of the base-q index, not its digits) against bit 0:
+ *   mode 0 (zz): (b_j - S)(b_0 - S) e^{iqr_j}, S = (q-1)/2;
+ *   mode 1 (pp): my_xor(b_j, b_0) e^{iqr_j}, my_xor = [b != b0]/4 in the S=1/2 tree
+ *   (H/TPS.f90:214-215) and sx + sy of the spin-S tree's ladder matrices
+ *   (S/params.f90:1479-1495), whose (0,1) and (1,0) entries are (1 - i) su(0,1)/2 and
+ *   (1 + i) sd(1,0)/2. */
+static void my_xor_table(int q, zc t[2][2])
 {
-    const int64_t d = (int64_t)1 << n;
+    t[0][0] = t[1][1] = 0.0;
+    if (q == 2) {
+        t[0][1] = t[1][0] = 0.25;
+        return;
+    }
+    const double ss = 0.5 * (q - 1);
+    const double su01 = sqrt((ss - (-ss)) * (ss + (-ss) + 1.0));       /* su(0,1), m = -S      */
+    const double sd10 = sqrt((ss + (1.0 - ss)) * (ss - (1.0 - ss) + 1.0)); /* sd(1,0), m = 1 - S */
+    t[0][1] = 0.5 * su01 - 0.5 * I * su01;
+    t[1][0] = 0.5 * sd10 + 0.5 * I * sd10;
+}
+
+void orc_sq_apply(int n, int q, const double *site_vec, const double qv[2], int mode, const zc *x, zc *v)
+{
+    const int64_t d = ipow64(q, n);
+    const double ss = 0.5 * (q - 1);
+    zc tab[2][2];
+    my_xor_table(q, tab);
     zc *ph = malloc(sizeof(zc) * n);
     for (int j = 0; j < n; ++j) {
         const double qn = qv[0] * site_vec[2 * j] + qv[1] * site_vec[2 * j + 1];
@@ -650,9 +673,7 @@ void orc_sq_apply(int n, const double *site_vec, const double qv[2], int mode, c
         const int s0 = (int)(i & 1);
         for (int j = 0; j < n; ++j) {
             const int sj = (int)((i >> j) & 1);
-            double w;
-            if (mode == 0) w = ((double)sj - 0.5) * ((double)s0 - 0.5);
-            else w = (sj != s0) ? 0.25 : 0.0;
+            const zc w = (mode == 0) ? ((double)sj - ss) * ((double)s0 - ss) : tab[sj][s0];
             acc += w * ph[j] * x[i];
         }
         v[i] = acc;
@@ -663,9 +684,9 @@ void orc_sq_apply(int n, const double *site_vec, const double qv[2], int mode, c
 /* calc_zz_bond_corr  H/TPS.f90:270-321: for z-type bonds (type 3), weight
  * (b(is1)-1/2)(b(is2)-1/2) where the bit positions are the 1-BASED ipair entries
  * (reference off-by-one, kept); a position >= n reads bit (pos-n) of the rank id 0. */
-void orc_zz_bond_apply(int n, int ibond, const int *ipair, const int *btype, const zc *x, zc *v)
+void orc_zz_bond_apply(int n, int q, int ibond, const int *ipair, const int *btype, const zc *x, zc *v)
 {
-    const int64_t d = (int64_t)1 << n;
+    const int64_t d = ipow64(q, n);   /* bits of the base-q index in the spin-S tree (S/TPS.f90:269-320) */
     #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < d; ++i) {
         double w = 0.0;
@@ -686,26 +707,26 @@ void orc_zz_bond_apply(int n, int ibond, const int *ipair, const int *btype, con
  * v1 holds the normalised start vector.  coef is row-major (steps x ncoef),        */
 /* ncoef = 7 + sq_num.  Returns 0.                                                   */
 /* ------------------------------------------------------------------------------ */
-static void eval_phys(int n, int ibond, const int *ipair, const int *btype, const double *site_vec,
+static void eval_phys(int n, int q, int ibond, const int *ipair, const int *btype, const double *site_vec,
                       int sq_num, const double *sq_q, int ict, double alp, double bet, double lmax,
                       const zc *vg, const zc *v1, const zc *v0, zc *vt, double *row)
 {
-    const int64_t d = (int64_t)1 << n;
+    const int64_t d = ipow64(q, n);
     row[0] = (double)(2 * ict) / alp;
     row[1] = lmax;
     row[2] = bet;
     row[3] = lmax - alp;
     const zc val1 = orc_dot(d, vg, v0), val2 = orc_dot(d, vg, v1);
     row[4] = creal(lmax * val2 - val1);
-    orc_zz_bond_apply(n, ibond, ipair, btype, v1, vt);
+    orc_zz_bond_apply(n, q, ibond, ipair, btype, v1, vt);
     row[5] = creal(orc_dot(d, v1, vt));
     row[6] = creal(orc_dot(d, vg, vt));
     double *sq = malloc(sizeof(double) * (4 * sq_num + 1));
     for (int i = 0; i < sq_num; ++i) {
-        orc_sq_apply(n, site_vec, sq_q + 2 * i, 0, v1, vt);
+        orc_sq_apply(n, q, site_vec, sq_q + 2 * i, 0, v1, vt);
         sq[4 * i + 0] = creal(orc_dot(d, v1, vt));
         sq[4 * i + 1] = creal(orc_dot(d, vg, vt));
-        orc_sq_apply(n, site_vec, sq_q + 2 * i, 1, v1, vt);
+        orc_sq_apply(n, q, site_vec, sq_q + 2 * i, 1, v1, vt);
         sq[4 * i + 2] = creal(orc_dot(d, v1, vt));
         sq[4 * i + 3] = creal(orc_dot(d, vg, vt));
     }
@@ -714,13 +735,13 @@ static void eval_phys(int n, int ibond, const int *ipair, const int *btype, cons
     free(sq);
 }
 
-int orc_tps(int n, int ibond, const int *site_pair, const int *btype, const int *ipair,
+int orc_tps(int n, int q, int ibond, const int *site_pair, const int *btype, const int *ipair,
             const zc *xl, const unsigned char *mk, const double *site_vec,
             int sq_num, const double *sq_q, double lmax, int nsteps,
             zc *v1, zc *v0, zc *vg, zc *vt, double *alpha, double *beta, double *coef)
 {
-    orc_op op = {n, 2, ibond, site_pair, btype, xl, mk};
-    const int64_t d = (int64_t)1 << n;
+    orc_op op = {n, q, ibond, site_pair, btype, xl, mk};
+    const int64_t d = ipow64(q, n);
     const int nc = 7 + sq_num;
     memset(v0, 0, sizeof(zc) * (size_t)d);
     memset(vg, 0, sizeof(zc) * (size_t)d);
@@ -730,7 +751,7 @@ int orc_tps(int n, int ibond, const int *site_pair, const int *btype, const int 
     double b1 = sqrt(creal(orc_dot(d, v0, v0)));
     alpha[0] = lmax - a1;
     beta[0] = b1;
-    eval_phys(n, ibond, ipair, btype, site_vec, sq_num, sq_q, 1, a1, b1, lmax, vg, v1, v0, vt, coef);
+    eval_phys(n, q, ibond, ipair, btype, site_vec, sq_num, sq_q, 1, a1, b1, lmax, vg, v1, v0, vt, coef);
     for (int i = 2; i <= nsteps; ++i) {
         const double xn = 1.0 / b1;
         #pragma omp parallel for schedule(static)
@@ -740,7 +761,7 @@ int orc_tps(int n, int ibond, const int *site_pair, const int *btype, const int 
         b1 = sqrt(creal(orc_dot(d, v0, v0)));
         alpha[i - 1] = lmax - a1;
         beta[i - 1] = b1;
-        eval_phys(n, ibond, ipair, btype, site_vec, sq_num, sq_q, i, a1, b1, lmax, vg, v1, v0, vt,
+        eval_phys(n, q, ibond, ipair, btype, site_vec, sq_num, sq_q, i, a1, b1, lmax, vg, v1, v0, vt,
                   coef + (size_t)(i - 1) * nc);
     }
     return 0;

@@ -29,10 +29,13 @@ struct LanczosSteps {
     bool lazy;
     pned_vec *X, *Y;
     double sx = 1.0, sy = 0.0;
-    LanczosSteps(pned_ctx *c, pned_vec *a, pned_vec *b)
+    // explicit: the reference's update on the true vectors at every step (no deferred
+    // normalisation), so the loop state can be checkpointed and resumed bit for bit
+    LanczosSteps(pned_ctx *c, pned_vec *a, pned_vec *b, bool explicit_update = false)
         : ctx(c), v1(a), v0(b),
-          lazy(a->nvec == 1 && ((c->q == 2 && c->sector_k < 0) ||
-                                (c->q >= 3 && c->q <= 5 && c->spin_tiled && a->dtype == PNED_C128))),
+          lazy(!explicit_update && a->nvec == 1 &&
+               ((c->q == 2 && c->sector_k < 0) ||
+                (c->q >= 3 && c->q <= 5 && c->spin_tiled && a->dtype == PNED_C128))),
           X(a), Y(b)
     {
     }
@@ -226,6 +229,82 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
         return PNED_OK;
     }
     *itr = last;
+    return fail(PNED_E_NOCONV, "#(W07) lnc1 did not converge within " + std::to_string(last) + " steps");
+}
+
+// parallelized_lnc1z (S_one_half/Hybrid_TPS_Lanczos.f90:48-163) from step `first`, with
+// the reference's explicit update: first <= 2 starts from the normalised vector in v1
+// (alpha(1), beta(1) by calc_alpha_beta, :78-85); first > 2 continues from the loop state a
+// checkpoint holds -- v1, v0 of the top of step `first` and alpha/beta(1..first-1) -- as
+// the restart branch does after interval_dump_read (:87-93).  The convergence reference
+// ebefor is recomputed from the last check before `first`, where the uninterrupted loop
+// set it (the reference leaves it undefined after a restart).  stop > first: return at the
+// top of step `stop` (where the reference's time limit dumps) with *itr = stop and the loop
+// state in v1, v0.  The same entry run in one span or split at any step gives the same
+// alpha/beta bit for bit.
+int pned_lnc1_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int first, int stop, int max_itr, int nvec,
+                   int fixed_steps, double *alpha, double *beta, double *ene, int *itr)
+{
+    if (!ctx || !v1 || !v0 || !alpha || !beta || !ene || !itr) return fail(PNED_E_ARG, "null argument");
+    if (nvec < 0) return fail(PNED_E_ARG, "#(E06) Wrong value given to nvec in lnc1");
+    if (v1->nvec != 1) return fail(PNED_E_ARG, "lnc1 runs on one column");
+    PNED_TRY(check_hv_pair(ctx, v1, v0));
+    const int last = fixed_steps > 0 ? fixed_steps : max_itr;
+    if (first < 1 || first > last || (stop > 0 && stop < std::max(first, 2)))
+        return fail(PNED_E_ARG, "lnc1 span: need 1 <= first <= last and stop >= max(first, 2) (or 0)");
+    double a1, b1;
+    LanczosSteps run(ctx, v1, v0, true);
+    int start = first;
+    if (first <= 2) {
+        PNED_TRY(run.first(a1, b1));
+        alpha[0] = a1;
+        beta[0] = b1;
+        start = 2;
+    } else {
+        a1 = alpha[first - 2];
+        b1 = beta[first - 2];
+    }
+    double ebefor = 0.0;
+    const double eps = 1e-25;
+    const int ne = std::max(nvec, 1);
+    if (fixed_steps <= 0) {
+        int prev = -1;   // the last step before `start` that set ebefor
+        for (int i = start - 1; i >= 2 && prev < 0; --i)
+            if ((i > 4 * ne && i % 5 == 0) || i == 4 * ne) prev = i;
+        if (prev > 0) {
+            PNED_TRY(pned_bisec(alpha, beta, prev, ene, ne, eps));
+            ebefor = ene[ne - 1];
+        }
+    }
+    for (int i = start; i <= last; ++i) {
+        if (i == stop) {
+            *itr = stop;
+            return PNED_OK;
+        }
+        PNED_TRY(run.next(a1, b1));
+        alpha[i - 1] = a1;
+        beta[i - 1] = b1;
+        if (b1 < 0.5e-30) {
+            *itr = i;
+            return fail(PNED_E_BREAKDOWN, "#(E07) Tridiagonalization unsuccessful in lnc1: beta too small at i=" +
+                                              std::to_string(i));
+        }
+        if (fixed_steps > 0) continue;
+        if (i > 4 * ne && i % 5 == 0) {
+            PNED_TRY(pned_bisec(alpha, beta, i, ene, ne, eps));
+            if (std::fabs((ebefor - ene[ne - 1]) / ene[ne - 1]) < 1e-13) {
+                *itr = i;
+                return PNED_OK;
+            }
+            ebefor = ene[ne - 1];
+        }
+        if (i == 4 * ne) {
+            PNED_TRY(pned_bisec(alpha, beta, 4 * ne, ene, ne, eps));
+            ebefor = ene[ne - 1];
+        }
+    }
+    *itr = last;
+    if (fixed_steps > 0) return pned_bisec(alpha, beta, last, ene, std::min(ne, last), eps);
     return fail(PNED_E_NOCONV, "#(W07) lnc1 did not converge within " + std::to_string(last) + " steps");
 }
 
@@ -496,15 +575,33 @@ int pned_tps_set_observables(pned_ctx *ctx, const double *site_vec, int nsq, con
 int pned_tps(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, pned_vec *vt, double lmax, int nsteps,
              double *alpha, double *beta, double *coef)
 {
+    return pned_tps_span(ctx, v1, v0, vg, vt, lmax, 1, 0, nsteps, alpha, beta, coef);
+}
+
+// The same from step `first`: first <= 1 starts from the normalised columns of v1; first > 1
+// continues from the loop state of the top of step `first` (the TPSz restart branch after
+// interval_dump_read, :352-358): v0 = (L-H) v1 of step first-1, v1, vg, and rows
+// 1..first-1 of alpha/beta/coef.  stop > first returns at the top of step `stop` (the
+// time-limit dump point) with that loop state.  One span or several: the same rows.
+int pned_tps_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, pned_vec *vt, double lmax, int first,
+                  int stop, int nsteps, double *alpha, double *beta, double *coef)
+{
     if (!ctx || !v1 || !v0 || !vg || !vt || !alpha || !beta || !coef) return fail(PNED_E_ARG, "null argument");
     if (ctx->site_vec.empty()) return fail(PNED_E_ARG, "call pned_tps_set_observables first");
+    if (first < 1 || first > nsteps || (stop > 0 && stop < first))
+        return fail(PNED_E_ARG, "tps span: need 1 <= first <= nsteps and stop >= first (or 0)");
     const int nc = v1->nvec, nsq = ctx->nsq, ncoef = 7 + nsq;
     const int width = 6 + 2 * nsq;
     std::vector<double> pr(2 * nc), n0(2 * nc), gv0(2 * nc), obs((size_t)width * nc), xn(nc);
-    PNED_TRY(pned_vec_zero(ctx, v0));
-    PNED_TRY(pned_vec_zero(ctx, vg));
     (void)vt;
-    for (int i = 1; i <= nsteps; ++i) {
+    if (first <= 1) {
+        PNED_TRY(pned_vec_zero(ctx, v0));
+        PNED_TRY(pned_vec_zero(ctx, vg));
+    } else {
+        for (int c = 0; c < nc; ++c) xn[c] = 1.0 / beta[(size_t)(first - 2) * nc + c];
+    }
+    for (int i = std::max(first, 1); i <= nsteps; ++i) {
+        if (i == stop) return PNED_OK;
         if (i > 1) {
             PNED_TRY(tps_shift_update(ctx, v1, v0, vg, xn.data()));
         }
@@ -639,6 +736,45 @@ int pned_dsflnc1z(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double cnorm, int s
         sbeta[i] = b1;
     }
     return run.finish(a1);
+}
+
+// parallelized_dsflnc1z (S_one_half/Hybrid_TPS_DSF.f90:195-261) from step `first` with
+// the reference's explicit update: first <= 2 starts from the normalised excitation in v1
+// (sbeta(1) = cnorm); first > 2 continues from the loop state of the top of step `first`
+// -- v1, v0 and salpha(1..first-1), sbeta(1..first) -- the interval_dump_read restart
+// (itr_counter(11) > 2) and the calc_extended_DSF continuation (itr_counter(11) = indx + 1
+// after read_alpha_beta2, :515-553) both use.  Runs to step s_maxitr - 1; stop > first
+// returns at the top of step `stop` (the time-limit dump point).  *icf = the last step
+// taken.  One span or several give the same salpha/sbeta bit for bit.
+int pned_dsflnc1z_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double cnorm, int first, int stop, int s_maxitr,
+                       double *salpha, double *sbeta, int *icf)
+{
+    if (!ctx || !v1 || !v0 || !salpha || !sbeta || !icf || s_maxitr < 2) return fail(PNED_E_ARG, "bad argument");
+    PNED_TRY(check_hv_pair(ctx, v1, v0));
+    if (first < 1 || first > s_maxitr || (stop > 0 && stop < std::max(first, 2)))
+        return fail(PNED_E_ARG, "dsflnc1z span: need 1 <= first <= s_maxitr and stop >= max(first, 2) (or 0)");
+    double a1, b1;
+    LanczosSteps run(ctx, v1, v0, true);
+    int start = first;
+    if (first <= 2) {
+        sbeta[0] = cnorm;
+        PNED_TRY(run.first(a1, b1));
+        salpha[0] = a1;
+        sbeta[1] = b1;
+        start = 2;
+    } else {
+        a1 = salpha[first - 2];
+        b1 = sbeta[first - 1];
+    }
+    int i = start;
+    for (; i <= s_maxitr - 1; ++i) {
+        if (i == stop) break;
+        PNED_TRY(run.next(a1, b1));
+        salpha[i - 1] = a1;
+        sbeta[i] = b1;
+    }
+    *icf = i - 1;
+    return PNED_OK;
 }
 
 int pned_renbun(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,

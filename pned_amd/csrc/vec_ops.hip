@@ -499,6 +499,10 @@ struct ObsArgs {
     int sq_q[OBS_MAX_SQ];                 // momentum slot
     double phr[8][32], phi[8][32];        // e^{i q.r_j} per momentum slot
     int8_t pos[32];                       // memory bit of site j
+    // structure-factor weights w(b_j, b_0) (re, im) by mode (0: zz, 1: pp) of the two bits:
+    // S=1/2 tree (b - 1/2)(b0 - 1/2) and [b != b0]/4; spin-S tree (b - S)(b0 - S) and the
+    // 2x2 corner of sx + sy (S_arbitrary/Hybrid_TPS9_TPS.f90:175-177, 224-243)
+    double wtab[2][2][2][2];
 };
 
 template <typename Real, int NOUT>
@@ -537,11 +541,9 @@ __global__ __launch_bounds__(RED_TPB) void k_tps_obs(const typename V2<Real>::T 
             double cr = 0.0, ci = 0.0;
             for (int j = 0; j < a.n; ++j) {
                 const int sj = (int)((s >> a.pos[j]) & 1u);
-                double w;
-                if (a.sq_mode[k] == 0) w = ((double)sj - 0.5) * ((double)s0 - 0.5);
-                else w = (sj != s0) ? 0.25 : 0.0;
-                cr = fma(w, a.phr[qs][j], cr);
-                ci = fma(w, a.phi[qs][j], ci);
+                const double *w = a.wtab[a.sq_mode[k]][sj][s0];
+                cr = fma(w[0], a.phr[qs][j], fma(-w[1], a.phi[qs][j], cr));
+                ci = fma(w[0], a.phi[qs][j], fma(w[1], a.phr[qs][j], ci));
             }
             // W x
             const double wr = cr * xr - ci * xi, wi = cr * xi + ci * xr;
@@ -621,11 +623,9 @@ __global__ __launch_bounds__(RED_TPB) void k_tps_fused(const typename V2<Real>::
                 double cr = 0.0, ci = 0.0;
                 for (int j = 0; j < a.n; ++j) {
                     const int sj = (int)((s >> a.pos[j]) & 1u);
-                    double wj;
-                    if (a.sq_mode[k] == 0) wj = ((double)sj - 0.5) * ((double)s0 - 0.5);
-                    else wj = (sj != s0) ? 0.25 : 0.0;
-                    cr = fma(wj, a.phr[qs][j], cr);
-                    ci = fma(wj, a.phi[qs][j], ci);
+                    const double *wj = a.wtab[a.sq_mode[k]][sj][s0];
+                    cr = fma(wj[0], a.phr[qs][j], fma(-wj[1], a.phi[qs][j], cr));
+                    ci = fma(wj[0], a.phi[qs][j], fma(wj[1], a.phr[qs][j], ci));
                 }
                 wt[sl * nw + 1 + 2 * k] = cr;
                 wt[sl * nw + 2 + 2 * k] = ci;
@@ -1216,13 +1216,35 @@ int sz_q_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *qv, 
     return PNED_OK;
 }
 
+// The reference computes the observables from the binary bits of the state index in both
+// trees (iand(ishft(istate,-j),1)): in the spin-S tree these are bits of the base-q index,
+// not its digits (S_arbitrary/Hybrid_TPS9_TPS.f90:175-177, 238-243, 305-308), kept for parity.
 static int build_obs_args(pned_ctx *ctx, ObsArgs &a)
 {
-    if (ctx->q != 2) return fail(PNED_E_UNSUPPORTED, "mTPQ observables are defined for S=1/2");
+    if (ctx->q > 2 && (uint64_t)ctx->dim > 0xFFFFFFFFull)
+        return fail(PNED_E_UNSUPPORTED, "mTPQ observables: at most 2^32 states");
     a = ObsArgs{};
     a.n = ctx->n;
     a.nzz = 0;
     for (int j = 0; j < 32; ++j) a.pos[j] = ctx->lay.pos[j];
+    const double ss = 0.5 * (ctx->q - 1);
+    for (int b = 0; b < 2; ++b)
+        for (int b0 = 0; b0 < 2; ++b0) {
+            a.wtab[0][b][b0][0] = ((double)b - ss) * ((double)b0 - ss);
+            a.wtab[0][b][b0][1] = 0.0;
+            a.wtab[1][b][b0][0] = a.wtab[1][b][b0][1] = 0.0;
+        }
+    if (ctx->q == 2) {
+        a.wtab[1][0][1][0] = a.wtab[1][1][0][0] = 0.25;   // my_xor, Hybrid_TPS_TPS.f90:214-215
+    } else {
+        // my_xor = sx + sy with sx = (su + sd)/2, sy = -i (su - sd)/2, su(0,1) and sd(1,0)
+        // the ladder factors of m = -S (Hybrid_TPS9_params.f90:1479-1495)
+        const double su01 = std::sqrt((ss + ss) * (ss - ss + 1.0)), sd10 = std::sqrt((ss + 1.0 - ss) * (ss + ss));
+        a.wtab[1][0][1][0] = 0.5 * su01;
+        a.wtab[1][0][1][1] = -0.5 * su01;
+        a.wtab[1][1][0][0] = 0.5 * sd10;
+        a.wtab[1][1][0][1] = 0.5 * sd10;
+    }
     for (int b = 0; b < ctx->ibond; ++b) {
         if (ctx->btype[b] != 3) continue;
         // H/TPS.f90:290-316 shifts by the 1-based site number: bit 'ipair'; past the last

@@ -181,6 +181,16 @@ module pned_gpu
       real(c_double), intent(out) :: salpha(*), sbeta(*)
       integer(c_int) :: rc
     end function
+    function pned_dsflnc1z_span(ctx, v1, v0, cnorm, first, stop, s_maxitr, salpha, sbeta, icf) &
+        bind(C, name="pned_dsflnc1z_span") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, v1, v0
+      real(c_double), value :: cnorm
+      integer(c_int), value :: first, stop, s_maxitr
+      real(c_double), intent(inout) :: salpha(*), sbeta(*)
+      integer(c_int), intent(out) :: icf
+      integer(c_int) :: rc
+    end function
     function pned_renbun(ctx, i_range, s_maxitr, split, ene, eta, salpha, sbeta, sq, w, dsf, conv) &
         bind(C, name="pned_renbun") result(rc)
       import :: c_int, c_ptr, c_double
@@ -671,6 +681,61 @@ contains
                                     i == pned_first), cn, q, int(pned_i_range, c_int), w, dsf, conv), 'write_dsf_res')
     end do
   end subroutine calc_DSF
+
+  !! calc_extended_DSF (Hybrid_TPS_DSF.f90:515-553), the COMP=zr/xr continuation: for each
+  !! momentum, read_alpha_beta2 restores sq, salpha/sbeta(1..indx+1) and the two vectors of an
+  !! earlier run, the Lanczos continues from step indx + 1 to pned_s_maxitr - 1
+  !! (pned_set_dsf_steps raises it), then the files, the spectrum and the result block as
+  !! in calc_DSF.  The result file name follows open_res_files: dsf_res_zz.dat / dsf_res_xx.dat
+  !! for those components, dsf_comp_error.dat otherwise (Hybrid_TPS_output.f90:216-231).
+  subroutine calc_extended_DSF(Iam, ene, x, i_color)
+    integer, intent(in) :: Iam, i_color
+    double precision, intent(in) :: ene
+    type(c_ptr), intent(in) :: x
+    integer :: i, icn
+    integer(c_int) :: icf
+    double precision :: q(2), w(pned_i_range), dsf(pned_i_range)
+    real(c_double), allocatable :: sa(:), sb(:)
+    integer(c_int) :: conv(pned_i_range)
+    complex(kind(0d0)) :: norm
+    character(len=64) :: ident, fres
+    write(ident, '(A,I5.5,A,I5.5)') 'cl', i_color, 'lme', Iam
+    if (pned_dsf_comp == 'zz' .or. pned_dsf_comp == 'xx') then
+      fres = 'dsf_res_'//pned_dsf_comp//'.dat'//trim(ident)
+    else
+      fres = 'dsf_comp_error.dat'//trim(ident)
+    end if
+    allocate(sa(pned_s_maxitr), sb(pned_s_maxitr + 1))
+    do i = pned_first, pned_last
+      q(1:2) = pned_q_list(1:2, i)
+      write(*, '(A,I3,2F12.6)') ' q = ', i, q(1), q(2)
+      call read_alpha_beta2(i, icn, norm, pned_dsf_comp, wwk(1), wwk(2))
+      sa = 0d0
+      sb = 0d0
+      sa(1:pned_s_maxitr) = pned_salpha
+      sb(1:pned_s_maxitr) = pned_sbeta
+      call pned_check(pned_dsflnc1z_span(pned_context, wwk(1), wwk(2), dble(norm), int(icn + 1, c_int), 0_c_int, &
+                      int(pned_s_maxitr, c_int), sa, sb, icf), 'dsflnc1z')
+      pned_salpha = sa(1:pned_s_maxitr)
+      pned_sbeta = sb(1:pned_s_maxitr)
+      call dump_alpha_beta(i, int(icf), norm, pned_dsf_comp, wwk(1), wwk(2))
+      call pned_check(pned_renbun(pned_context, int(pned_i_range, c_int), int(pned_s_maxitr, c_int), pned_split, &
+                      ene, pned_delt, sa, sb, dble(norm), w, dsf, conv), 'renbun')
+      if (Iam == 0) call pned_check(pned_write_dsf_res(trim(fres)//c_null_char, merge(0_c_int, 1_c_int, &
+                                    i == pned_first), dble(norm), q, int(pned_i_range, c_int), w, dsf, conv), &
+                                    'write_dsf_res')
+    end do
+  end subroutine calc_extended_DSF
+
+  !! a new s_maxitr (itr_counter(12)) for the DSF Lanczos, e.g. before calc_extended_DSF
+  subroutine pned_set_dsf_steps(s_maxitr)
+    integer, intent(in) :: s_maxitr
+    pned_s_maxitr = s_maxitr
+    if (allocated(pned_salpha)) deallocate(pned_salpha, pned_sbeta)
+    allocate(pned_salpha(s_maxitr), pned_sbeta(s_maxitr))
+    pned_salpha = 0d0
+    pned_sbeta = 0d0
+  end subroutine pned_set_dsf_steps
 
   !! restrict the context to the S^z sector with n_down down spins (parity_type = 0;
   !! the index tables of mk_list12, Hybrid_TPS_parallel.f90:452-578)

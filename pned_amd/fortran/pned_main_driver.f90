@@ -1,6 +1,7 @@
 !! pned_main_driver.f90 -- a Fortran host that runs the reference's call sequence
 !! (S_one_half/Hybrid_TPS_main.f90:89-143) through pned_parallelized_diag on the GPU:
 !!   parallelized_lnc1 -> parallelized_inv1 -> readwrite_vec_data -> check1 -> calc_DSF,
+!!   calc_extended_DSF (twice the DSF steps from the files calc_DSF wrote),
 !!   then change_to_TPS_hamiltonian (tables given) -> parallelized_TPS.
 !! Input (unformatted stream file, argument 1): n, q, ibond, ipair, ibond_type,
 !! xlocal_Ham, llocal_Ham (as int), the shifted TPS tables and L, then nq, q_list(2,nq),
@@ -45,8 +46,13 @@ program pned_main_driver
   ! *** Precision check (Hybrid_TPS_main.f90:119-120)
   call readwrite_vec_data(0, .true., ene(1), xvec)
   call check1(xvec)
-  ! *** DSF (Hybrid_TPS_main.f90:139-143)
+  ! *** DSF (Hybrid_TPS_main.f90:139-143), then the COMP=zr/xr-style continuation of the same
+  ! files to twice the steps (calc_extended_DSF); the first run's files are kept as first_*
   call calc_DSF(0, ene(1), xvec, 0)
+  call execute_command_line('for f in dsf_res_zz.datcl00000lme00000 SalphaSbeta_q*me00000zz vec_q*me00000zz; '// &
+                            'do cp "$f" "first_$f"; done')
+  call pned_set_dsf_steps(2 * s_maxitr)
+  call calc_extended_DSF(0, ene(1), xvec, 0)
   ! *** mTPQ (Hybrid_TPS_main.f90:108-114): change_to_TPS_hamiltonian, then parallelized_TPS
   call pned_update_matrices(q, xt, llt /= 0)
   localHam_max = L

@@ -100,6 +100,9 @@ def lib():
     L.pned_scale.argtypes = [P, P, _dp]
     L.pned_lanczos_update.argtypes = [P, P, P, _dp, _dp]
     L.pned_lnc1.argtypes = [P, P, P, i32, i32, i32, _dp, _dp, _dp, C.POINTER(i32)]
+    L.pned_lnc1_span.argtypes = [P, P, P, i32, i32, i32, i32, i32, _dp, _dp, _dp, C.POINTER(i32)]
+    L.pned_tps_span.argtypes = [P, P, P, P, P, f64, i32, i32, i32, _dp, _dp, _dp]
+    L.pned_dsflnc1z_span.argtypes = [P, P, P, f64, i32, i32, i32, _dp, _dp, C.POINTER(i32)]
     L.pned_bisec.argtypes = [_dp, _dp, i32, _dp, i32, f64]
     L.pned_vec12.argtypes = [_dp, _dp, i32, _dp, i32, _dp]
     L.pned_ritz_vector.argtypes = [P, P, P, P, i32, _dp, _dp, _dp]
@@ -411,6 +414,36 @@ class Context:
             rc = 0
         check(rc)
         return itr.value, alpha[: itr.value], beta[: itr.value], ene
+
+    def lnc1_span(self, v1: Vector, v0: Vector, alpha, beta, first=1, stop=0, max_itr=1024, nvec=8, fixed_steps=0,
+                  allow_noconv=False):
+        """lnc1z from step ``first`` with the reference's explicit update, into the caller's
+        alpha/beta arrays (steps 1..first-1 already there when first > 2); returns (itr, ene).
+        stop > 0: return at the top of step ``stop`` (the checkpoint point)."""
+        ene = np.zeros(max(nvec, 1))
+        itr = C.c_int()
+        rc = lib().pned_lnc1_span(self.h, v1.h, v0.h, first, stop, max_itr, nvec, fixed_steps, alpha, beta, ene,
+                                  C.byref(itr))
+        if rc == E_NOCONV and allow_noconv:
+            rc = 0
+        check(rc)
+        return itr.value, ene
+
+    def tps_span(self, v1: Vector, v0: Vector, vg: Vector, vt: Vector, lmax: float, alpha, beta, coef, first=1,
+                 stop=0):
+        """pned_tps from step ``first`` into the caller's flat arrays (nsteps*nvec,
+        nsteps*nvec, nsteps*nvec*ncoef); stop > 0 returns at the top of step ``stop``."""
+        nc = v1.nvec
+        nsteps = len(alpha) // nc
+        check(lib().pned_tps_span(self.h, v1.h, v0.h, vg.h, vt.h, lmax, first, stop, nsteps, alpha, beta, coef))
+
+    def dsflnc1z_span(self, v1: Vector, v0: Vector, cnorm: float, salpha, sbeta, first=1, stop=0):
+        """dsflnc1z from step ``first`` into the caller's salpha (s_maxitr) and sbeta
+        (s_maxitr + 1); returns the last step taken."""
+        icf = C.c_int()
+        check(lib().pned_dsflnc1z_span(self.h, v1.h, v0.h, cnorm, first, stop, len(salpha), salpha, sbeta,
+                                       C.byref(icf)))
+        return icf.value
 
     def ritz_vector(self, v1: Vector, v0: Vector, x: Vector, alpha, beta, coef):
         n = len(coef)

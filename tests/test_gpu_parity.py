@@ -286,6 +286,28 @@ def test_tps_parity_n12_batched():
         assert np.allclose(co[:, c, :], c_ref, rtol=1e-9, atol=1e-12)
 
 
+@pytest.mark.parametrize("spec", [8, 12])
+def test_tps_parity_spin1(spec):
+    """Spin-1 mTPQ (S_arbitrary/Hybrid_TPS9_TPS.f90:45-409): the partially shifted operator
+    (S_arbitrary/Hybrid_TPS9_params.f90:1660-1686), the tiled spin-S H.v and the
+    observables on the bits of the base-q index, against the oracle for two samples."""
+    _say("tps spin1", spec)
+    cl = lattice.reference_cluster(spec)
+    op = model.build_operator(cl, 20, 108.0, q=3, field=0.1, tps=True)
+    ctx = gpu.Context(op)
+    ctx.tps_set_observables(cl.site_vec, cl.sq_momenta())
+    v1, v0, vg, vt = (ctx.vector() for _ in range(4))
+    steps = 12
+    for seed in (31, 32):
+        ctx.set_initial_vec(v1, seed)
+        x = v1.download()
+        al, be, co = ctx.tps(v1, v0, vg, vt, op.shift_L, steps)
+        a_ref, b_ref, c_ref = oracle.tps(oracle.OracleOp(op), cl.site_vec, cl.sq_momenta(), op.shift_L, steps, x)
+        assert np.allclose(np.ravel(al), a_ref, rtol=1e-11)
+        assert np.allclose(np.ravel(be), b_ref, rtol=1e-11)
+        assert np.allclose(np.asarray(co).reshape(steps, -1), c_ref, rtol=1e-9, atol=1e-12)
+
+
 def test_dsf_parity_n12():
     _say("dsf n12")
     cl = lattice.reference_cluster(12)
@@ -464,14 +486,16 @@ def test_fortran_reference_call_sequence(tmp_path):
     assert np.allclose(nums, hx[js - 1] / x[js - 1], rtol=2e-6)                # 1pe16.6
     # calc_DSF at momenta first..last: sq.txt, SalphaSbeta_q*, dsf_res_zz.dat*
     sq_lines = (tmp_path / "sq.txt").read_text().splitlines()
-    blocks = (tmp_path / "dsf_res_zz.datcl00000lme00000").read_text().split("#")[1:]
+    blocks = (tmp_path / "first_dsf_res_zz.datcl00000lme00000").read_text().split("#")[1:]
+    ext_blocks = (tmp_path / "dsf_res_zz.datcl00000lme00000").read_text().split("#")[1:]
     assert len(sq_lines) == len(blocks) == last - first + 1
+    pctx = gpu.Context(op)
     for k, iq in enumerate(range(first, last + 1)):
         qv = cl.q_list[iq - 1]
         vx = oracle.sz_q(cl.n, 2, cl.site_vec, qv, x)
         cn = np.vdot(vx, vx).real / cl.n
         assert abs(float(sq_lines[k].split()[-1]) - cn) <= 1e-15 + 1e-12 * cn
-        sq, sa, sb = gpu.read_dsf_alpha_beta(tmp_path / f"SalphaSbeta_q{iq:05d}me00000zz", s_maxitr, s_maxitr)
+        _, sa, sb = gpu.read_dsf_alpha_beta(tmp_path / f"first_SalphaSbeta_q{iq:05d}me00000zz", s_maxitr, s_maxitr)
         sa_ref, sb_ref = oracle.dsf_lanczos(oop, vx / np.sqrt(cn * cl.n), cn, s_maxitr)
         assert np.allclose(sa[:20], sa_ref[:20], rtol=1e-10)
         assert np.allclose(sb[:20], sb_ref[:20], rtol=1e-10)
@@ -481,6 +505,21 @@ def test_fortran_reference_call_sequence(tmp_path):
         assert rows.shape == (i_range, 5)
         assert np.allclose(rows[:, 2], w2, atol=1e-10)
         assert np.allclose(rows[:, 3], d2, rtol=1e-7, atol=2e-16 * np.abs(d2).max() + 1e-15)
+        # calc_extended_DSF continued those files to 2 s_maxitr steps: the library's own
+        # continuation (pned_dsflnc1z_span from indx + 1, bitwise equal to one long span and
+        # held to the oracle in tests/test_gpu_resume.py) from the same files, bit for bit
+        s2 = 2 * s_maxitr
+        _, ea, eb = gpu.read_dsf_alpha_beta(tmp_path / f"SalphaSbeta_q{iq:05d}me00000zz", s2, s2)
+        w1, w0 = pctx.vector(), pctx.vector()
+        indx = pctx.read_vectors(tmp_path / f"first_vec_q{iq:05d}me00000zz", [w1, w0])
+        ca, cb = np.zeros(s2), np.zeros(s2 + 1)
+        ca[: indx + 1], cb[: indx + 1] = sa[: indx + 1], sb[: indx + 1]
+        pctx.dsflnc1z_span(w1, w0, cn, ca, cb, first=indx + 1)
+        assert np.array_equal(ea, ca) and np.array_equal(eb, cb[:s2])
+        w3, d3, c3 = pctx.renbun(ca, cb, e0, delt, i_range, split, s2, sq=cn)
+        rows = np.array([[float(t) for t in ln.split()] for ln in ext_blocks[k].splitlines()[1:] if ln.strip()])
+        assert np.allclose(rows[:, 3], d3, rtol=1e-14, atol=1e-15 * np.abs(d3).max())
+        assert np.array_equal(rows[:, 4].astype(int), c3)
     # parallelized_TPS: one SalphaSbeta_me00000seed<i>zz per sample, start vector seed + i
     oopt = oracle.OracleOp(opt)
     for i in range(first, last + 1):

@@ -27,6 +27,7 @@ import math
 import os
 
 import numpy as np
+import scipy.sparse as sp
 import pytest
 import scipy.linalg as sla
 
@@ -335,6 +336,73 @@ def test_tps_observables_vs_dense_operators(spec, phi, h):
     zz_true = sum(szd[a - 1] * szd[b - 1] for (a, b, t) in c.bonds if t == 3)
     assert np.abs(zz_true - zdiag).max() > 0.1
 
+
+def test_tps_spin1_vs_dense_operators():
+    """The spin-S mTPQ (S_arbitrary/Hybrid_TPS9_TPS.f90:45-409) at spin 1, N=8, against
+    dense operators built from the textbook spin-1 matrices:
+      * the shifted operator of change_to_TPS_hamiltonian's spin-S form
+        (S_arbitrary/Hybrid_TPS9_params.f90:1660-1686), which adds l_b only to the local
+        indices 0..3 of each bond (local index q*d(max site) + d(min site)):
+        A = sum_b l_b P_b - H, P_b the projector on those four local states;
+        alpha_k = L - <v|A|v>, beta_k = ||A v||;
+      * coef columns 5..7+sq_num with the tree's weights on the BITS of the base-q state
+        index (not its digits): zz bonds (b_p1 - 1/2)(b_p2 - 1/2) at the 1-based ipair
+        entries (0 past bit n-1), structure factors (b_j - S)(b_0 - S) and
+        my_xor(b_j, b_0), where my_xor = sx + sy of the tree's transposed ladder matrices =
+        (S^x - S^y)[b_j, b_0] of the textbook ones."""
+    c = lattice.reference_cluster(8)
+    n, q = c.n, 3
+    D = q ** n
+    phi, h = 108.0, 0.1
+    J, K = dense.model20_couplings(phi)
+    H = dense.kh_dense(c, J, K, K, K, q=q, h=h)
+    op_tps = model.build_operator(c, 20, phi, q=q, field=h, tps=True)
+    op = model.build_operator(c, 20, phi, q=q, field=h)
+    lb = op_tps.shift_L / c.ibond
+    idx = np.arange(D)
+    digits = [(idx // q ** k) % q for k in range(n)]
+    shift = np.zeros(D)
+    for (a, b, _) in c.bonds:
+        lo, hi = min(a, b) - 1, max(a, b) - 1
+        shift += lb * ((q * digits[hi] + digits[lo]) <= 3)
+    A = sp.diags(shift) - H
+    psi = oracle.gaussian(D, 23)
+    steps = 6
+    sq = c.sq_momenta()
+    nsq = len(sq)
+    alpha, beta, coef = oracle.tps(oracle.OracleOp(op_tps), c.site_vec, sq, op_tps.shift_L, steps, psi)
+    bit = [((idx >> p) & 1).astype(float) for p in range(n)]
+    f = [bit[p] - 0.5 if p < n else -0.5 * np.ones(D) for p in range(n + 1)]
+    zdiag = sum(f[a] * f[b] for (a, b, t) in c.bonds if t == 3)
+    Sx, Sy, _ = dense.site_ops(q)
+    mx = Sx - Sy
+    ph = [np.exp(1j * (qk @ c.site_vec[j])) for qk in sq for j in range(n)]
+    Aq = [sum(np.exp(1j * (qk @ c.site_vec[j])) * (bit[j] - 1.0) * (bit[0] - 1.0) for j in range(n)) for qk in sq]
+    Bq = [sum(np.exp(1j * (qk @ c.site_vec[j])) * mx[bit[j].astype(int), bit[0].astype(int)] for j in range(n))
+          for qk in sq]
+    del ph
+    vg = np.zeros(D, dtype=complex)
+    v1 = psi.copy()
+    L = op_tps.shift_L
+    for k in range(steps):
+        Av = A @ v1
+        assert abs(alpha[k] - (L - np.vdot(v1, Av).real)) < 1e-11 * L, k
+        assert abs(beta[k] - np.linalg.norm(Av)) < 1e-11 * np.linalg.norm(Av), k
+        row = coef[k]
+        assert abs(row[4] - (L * np.vdot(vg, v1) - np.vdot(vg, Av)).real) < 1e-11, k
+        assert abs(row[5] - np.vdot(v1, zdiag * v1).real) < 1e-12, k
+        assert abs(row[6] - np.vdot(vg, zdiag * v1).real) < 1e-12, k
+        four = []
+        for a, b in zip(Aq, Bq):
+            four += [np.vdot(v1, a * v1).real, np.vdot(vg, a * v1).real,
+                     np.vdot(v1, b * v1).real, np.vdot(vg, b * v1).real]
+        assert np.allclose(row[7:7 + nsq], four[:nsq], rtol=0, atol=1e-12), k
+        vg, v1 = v1, Av / np.linalg.norm(Av)
+    # the unshifted H.v of the same tables is the textbook H (the shift is the only change)
+    x = oracle.gaussian(D, 5)
+    y = np.zeros(D, dtype=complex)
+    oracle.hv(oracle.OracleOp(op), x, y)
+    assert np.linalg.norm(y - H @ x) < 1e-12 * np.linalg.norm(H @ x)
 
 @pytest.mark.parametrize("spec,q", [(8, 2), (12, 2), (8, 3)])
 def test_sz_q_is_the_dense_sz_sum(spec, q):
