@@ -217,6 +217,19 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
 int pned_lnc1_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int first, int stop, int max_itr, int nvec,
                    int fixed_steps, double *alpha, double *beta, double *ene, int *itr);
 
+/* Lanczos with full reorthogonalisation (no reference counterpart; the reference's unused
+ * orthg, Hybrid_TPS_basic.f90:219-303, is the closest): basis[0] holds the normalised start
+ * vector, basis[1..] receive the Lanczos vectors (two passes of classical Gram-Schmidt
+ * against all earlier ones per step), w is work; all of one shape, complex128 or -- the
+ * precision form -- complex64, with every dot, norm, alpha and beta in FP64.  fixed_steps
+ * > 0 runs that many steps; else the lnc1z convergence rule up to nbasis - 1 steps. */
+int pned_lnc1_reorth(pned_ctx *ctx, pned_vec *const *basis, int nbasis, pned_vec *w, int nvec, int fixed_steps,
+                     double *alpha, double *beta, double *ene, int *itr);
+/* the DSF form of it: s_maxitr - 1 steps from the normalised excitation in basis[0];
+ * salpha[s_maxitr], sbeta[s_maxitr] with sbeta[0] = cnorm (as pned_dsflnc1z) */
+int pned_dsflnc1z_reorth(pned_ctx *ctx, pned_vec *const *basis, int nbasis, pned_vec *w, double cnorm, int s_maxitr,
+                         double *salpha, double *sbeta);
+
 /* Serving (no reference counterpart): one Lanczos step of every slot (column) of a batch.
  * mode[c]: 0 idle (column zeroed, alpha = beta = 0), 1 first step (v1 holds the start
  * vector, v0 is cleared; the lnc1z start, Hybrid_TPS_Lanczos.f90:78-85), 2 continue (the

@@ -3,6 +3,7 @@
 // back to the host, where the tridiagonal algebra runs.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <complex>
 #include <cstring>
 
@@ -79,6 +80,12 @@ struct LanczosSteps {
 
 extern "C" {
 
+// The Sturm bisection and inverse iteration below keep the reference's steps (TITPACK's
+// bisec/vec12): lnc1z stops when the nvec-th Ritz value changes by < 1e-13 relative between
+// checks, a threshold within reach of the bisection's own rounding, so the step at which a
+// run stops is a function of these exact operations.  A LAPACK-style Sturm count (LDL^T
+// pivots, eigenvalues to ~1e-14) stopped a run of the N=12 reference cluster at step 120
+// where the reference's arithmetic stops at 125.
 // bisec (S_one_half/Hybrid_TPS_basic.f90:25-107): Sturm-sequence bisection, ne lowest
 int pned_bisec(const double *alpha, const double *beta, int ndim, double *E, int ne, double eps)
 {
@@ -306,6 +313,95 @@ int pned_lnc1_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int first, int sto
     *itr = last;
     if (fixed_steps > 0) return pned_bisec(alpha, beta, last, ene, std::min(ne, last), eps);
     return fail(PNED_E_NOCONV, "#(W07) lnc1 did not converge within " + std::to_string(last) + " steps");
+}
+
+// Lanczos with full reorthogonalisation (no reference counterpart; the reference's orthg,
+// S_one_half/Hybrid_TPS_basic.f90:219-303, is never called).  basis[0] holds the normalised
+// start vector q_1; step i forms r = H q_i - alpha_i q_i - beta_{i-1} q_{i-1} in w, removes
+// its components along q_1..q_i by two passes of classical Gram-Schmidt (all dots of a pass
+// first, FP64, then r -= sum_j c_j q_j), and stores q_{i+1} = r / beta_i, beta_i = ||r||,
+// in basis[i].  Every dot, norm, alpha and beta is FP64 whatever the vectors' storage, so
+// with complex64 vectors only the stored components and the H.v products round to single
+// precision.  fixed > 0: exactly that many steps; else the lnc1z rule (nvec-th Ritz value
+// unchanged to 1e-13 relative between checks every 5 steps after 4 nvec), at most
+// nbasis - 1 steps.  alpha[i-1] = alpha_i, beta[i-1] = beta_i.
+static int lanczos_reorth(pned_ctx *ctx, pned_vec *const *basis, int nbasis, pned_vec *w, int fixed, int nvec,
+                          double *alpha, double *beta, double *ene, int *itr)
+{
+    if (!ctx || !basis || !w || !alpha || !beta || !ene || !itr || nbasis < 2) return fail(PNED_E_ARG, "bad argument");
+    for (int j = 0; j < nbasis; ++j) {
+        if (!basis[j]) return fail(PNED_E_ARG, "null basis vector");
+        PNED_TRY(check_hv_pair(ctx, basis[j], w));
+        if (basis[j]->nvec != 1) return fail(PNED_E_ARG, "reorthogonalised Lanczos runs on one column");
+    }
+    const int last = fixed > 0 ? fixed : nbasis - 1;
+    if (last > nbasis - 1) return fail(PNED_E_ARG, "basis holds fewer than steps + 1 vectors");
+    const int ne = std::max(nvec, 1);
+    const double one[2] = {1.0, 0.0};
+    double ebefor = 0.0;
+    std::vector<double> c(2 * (size_t)(last + 1));
+    for (int i = 1; i <= last; ++i) {
+        pned_vec *qi = basis[i - 1];
+        double pr[2];
+        PNED_TRY(pned_hv_fresh(ctx, qi, w, 0.0, pr));
+        const double ai = pr[0];
+        const double ma[2] = {-ai, 0.0};
+        PNED_TRY(pned_axpby(ctx, ma, qi, one, w));
+        if (i > 1) {
+            const double mb[2] = {-beta[i - 2], 0.0};
+            PNED_TRY(pned_axpby(ctx, mb, basis[i - 2], one, w));
+        }
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int j = 0; j < i; ++j) PNED_TRY(pned_calc_vec_inprdct(ctx, basis[j], w, &c[2 * j]));
+            for (int j = 0; j < i; ++j) {
+                const double mc[2] = {-c[2 * j], -c[2 * j + 1]};
+                PNED_TRY(pned_axpby(ctx, mc, basis[j], one, w));
+            }
+        }
+        double nn[2];
+        PNED_TRY(pned_calc_vec_inprdct(ctx, w, w, nn));
+        const double bi = std::sqrt(nn[0]);
+        alpha[i - 1] = ai;
+        beta[i - 1] = bi;
+        if (bi < 0.5e-30) {
+            *itr = i;
+            return fail(PNED_E_BREAKDOWN, "#(E07) Tridiagonalization unsuccessful: beta too small at i=" +
+                                              std::to_string(i));
+        }
+        PNED_TRY(pned_vec_copy(ctx, w, basis[i]));
+        const double sc[2] = {1.0 / bi, 0.0};
+        PNED_TRY(pned_scale(ctx, basis[i], sc));
+        if (fixed > 0) continue;
+        if (i >= 4 * ne && i % 5 == 0) {
+            PNED_TRY(pned_bisec(alpha, beta, i, ene, ne, 1e-25));
+            if (i > 4 * ne && std::fabs((ebefor - ene[ne - 1]) / ene[ne - 1]) < 1e-13) {
+                *itr = i;
+                return PNED_OK;
+            }
+            ebefor = ene[ne - 1];
+        }
+    }
+    *itr = last;
+    if (fixed > 0) return pned_bisec(alpha, beta, last, ene, std::min(ne, last), 1e-25);
+    return fail(PNED_E_NOCONV, "reorthogonalised Lanczos did not converge within " + std::to_string(last) + " steps");
+}
+
+int pned_lnc1_reorth(pned_ctx *ctx, pned_vec *const *basis, int nbasis, pned_vec *w, int nvec, int fixed_steps,
+                     double *alpha, double *beta, double *ene, int *itr)
+{
+    return lanczos_reorth(ctx, basis, nbasis, w, fixed_steps, nvec, alpha, beta, ene, itr);
+}
+
+int pned_dsflnc1z_reorth(pned_ctx *ctx, pned_vec *const *basis, int nbasis, pned_vec *w, double cnorm, int s_maxitr,
+                         double *salpha, double *sbeta)
+{
+    if (!salpha || !sbeta || s_maxitr < 2) return fail(PNED_E_ARG, "bad argument");
+    std::vector<double> b((size_t)s_maxitr), e(1);
+    int itr = 0;
+    PNED_TRY(lanczos_reorth(ctx, basis, nbasis, w, s_maxitr - 1, 1, salpha, b.data(), e.data(), &itr));
+    sbeta[0] = cnorm;
+    for (int i = 1; i <= s_maxitr - 1; ++i) sbeta[i] = b[(size_t)i - 1];
+    return PNED_OK;
 }
 
 // Ritz vector of the Lanczos run: x = sum_k coef[k] q_k, regenerating q_k with the stored

@@ -102,6 +102,8 @@ def lib():
     L.pned_lnc1.argtypes = [P, P, P, i32, i32, i32, _dp, _dp, _dp, C.POINTER(i32)]
     L.pned_lnc1_span.argtypes = [P, P, P, i32, i32, i32, i32, i32, _dp, _dp, _dp, C.POINTER(i32)]
     L.pned_tps_span.argtypes = [P, P, P, P, P, f64, i32, i32, i32, _dp, _dp, _dp]
+    L.pned_lnc1_reorth.argtypes = [P, C.POINTER(P), i32, P, i32, i32, _dp, _dp, _dp, C.POINTER(i32)]
+    L.pned_dsflnc1z_reorth.argtypes = [P, C.POINTER(P), i32, P, f64, i32, _dp, _dp]
     L.pned_dsflnc1z_span.argtypes = [P, P, P, f64, i32, i32, i32, _dp, _dp, C.POINTER(i32)]
     L.pned_bisec.argtypes = [_dp, _dp, i32, _dp, i32, f64]
     L.pned_vec12.argtypes = [_dp, _dp, i32, _dp, i32, _dp]
@@ -444,6 +446,26 @@ class Context:
         check(lib().pned_dsflnc1z_span(self.h, v1.h, v0.h, cnorm, first, stop, len(salpha), salpha, sbeta,
                                        C.byref(icf)))
         return icf.value
+
+    def lnc1_reorth(self, basis, w: Vector, nvec=1, fixed_steps=0, allow_noconv=False):
+        """Lanczos with full reorthogonalisation over the given basis vectors (basis[0] =
+        normalised start); returns (itr, alpha, beta, ene)."""
+        nb = len(basis)
+        arr = (C.c_void_p * nb)(*[v.h for v in basis])
+        alpha, beta, ene = np.zeros(nb), np.zeros(nb), np.zeros(max(nvec, 1))
+        itr = C.c_int()
+        rc = lib().pned_lnc1_reorth(self.h, arr, nb, w.h, nvec, fixed_steps, alpha, beta, ene, C.byref(itr))
+        if rc == E_NOCONV and allow_noconv:
+            rc = 0
+        check(rc)
+        return itr.value, alpha[: itr.value], beta[: itr.value], ene
+
+    def dsflnc1z_reorth(self, basis, w: Vector, cnorm: float, s_maxitr: int):
+        nb = len(basis)
+        arr = (C.c_void_p * nb)(*[v.h for v in basis])
+        sa, sb = np.zeros(s_maxitr), np.zeros(s_maxitr)
+        check(lib().pned_dsflnc1z_reorth(self.h, arr, nb, w.h, cnorm, s_maxitr, sa, sb))
+        return sa, sb
 
     def ritz_vector(self, v1: Vector, v0: Vector, x: Vector, alpha, beta, coef):
         n = len(coef)

@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from pned_amd import gpu, lattice, model
 
@@ -39,15 +40,46 @@ def test_no_gpu_is_an_error_not_a_fallback():
         raise AssertionError("context created without a GPU")
 
 
-def test_host_tridiagonal_solvers_match_oracle():
+@pytest.mark.parametrize("n,seed", [(80, 1), (400, 2), (7, 3)])
+def test_host_tridiagonal_solvers(n, seed):
+    """The library's bisection and inverse iteration (the reference's TITPACK steps) equal
+    the oracle's restatement bit for bit, and agree with LAPACK
+    (scipy.linalg.eigh_tridiagonal): eigenvalues within 64 ulps of the spectral radius,
+    eigenvectors up to sign."""
     import oracle
-    rng = np.random.default_rng(1)
-    a = rng.standard_normal(80)
-    b = rng.random(80) + 0.05
-    E = gpu.bisec(a, b, 8)
-    assert np.array_equal(E, oracle.bisec(a, b, 8))
-    V = gpu.vec12(a, b, E, 3)
-    assert np.allclose(V, oracle.vec12(a, b, E, 3), atol=1e-14)
+    import scipy.linalg as sla
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal(n)
+    b = rng.random(n) + 0.05
+    ne = min(8, n)
+    E = gpu.bisec(a, b, ne)
+    w, vecs = sla.eigh_tridiagonal(a, b[: n - 1])
+    scale = np.max(np.abs(w))
+    ulp = np.finfo(float).eps * scale    # a Sturm count resolves eigenvalues to O(n eps ||T||)
+    assert np.max(np.abs(E - w[:ne])) <= 64 * ulp
+    assert np.array_equal(E, oracle.bisec(a, b, ne))
+    nv = min(3, n)
+    V = gpu.vec12(a, b, E, nv)
+    assert np.allclose(V, oracle.vec12(a, b, E, nv), atol=1e-14)
+    T = np.diag(a) + np.diag(b[: n - 1], 1) + np.diag(b[: n - 1], -1)
+    for k in range(nv):
+        assert abs(np.linalg.norm(V[k]) - 1.0) < 1e-14
+        assert np.linalg.norm(T @ V[k] - E[k] * V[k]) < 1e-12 * scale
+        assert abs(abs(V[k] @ vecs[:, k]) - 1.0) < 1e-12
+
+
+def test_host_tridiagonal_degenerate_pair():
+    """Two decoupled identical blocks: every eigenvalue twice; the two inverse-iteration
+    vectors of a pair come out orthogonal."""
+    rng = np.random.default_rng(4)
+    h = rng.standard_normal(20)
+    c = rng.random(19) + 0.1
+    a = np.concatenate([h, h])
+    b = np.concatenate([c, [0.0], c, [0.0]])
+    E = gpu.bisec(a, b, 4)
+    assert abs(E[0] - E[1]) < 1e-14 and abs(E[2] - E[3]) < 1e-14
+    V = gpu.vec12(a, b, E, 2)
+    assert abs(V[0] @ V[1]) < 1e-10   # TITPACK orthogonalises the start vectors, then iterates
 
 
 def test_ctpq_product_matches_oracle():
