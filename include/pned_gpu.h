@@ -111,6 +111,19 @@ int pned_s12_plan_export(int n_sites, int ibond, const int32_t *ipair, const int
                          int32_t *pass_info, int32_t *pass_bits, double *terms,
                          uint32_t *term_gmask, double *diag, int max_pass, int max_term,
                          const int32_t *site_bit /* memory layout (see below), NULL = identity */);
+/* diagnostic, needs no GPU: the spin-S (q = 3..5) tile passes the planner builds for these
+ * tables.  tile_sites <= 0: the largest tile (q^K <= 8192 states); low_sites 0: the kernel's
+ * choice (3 forced low sites when that costs no pass, else 2).  Outputs (caller-sized):
+ *   pass_sites[npass*32] = tile sites (0-based, lowest first), -1 padded
+ *   bond_pass[ibond]     = pass applying each bond
+ *   cls[npass*64*(3+2*q*q)] = per pass and class: {bond, delta_min_site, delta_max_site,
+ *                          coefficient re/im by output local state} (as doubles, unused rows 0)
+ *   ncls[npass]          = off-diagonal classes per pass
+ * Returns the pass count in *npass and the forced low sites in *low_used. */
+int pned_spin_plan_export(int n_sites, int q, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                          const double *xlocal_ham, const int32_t *llocal_ham, int tile_sites, int low_sites,
+                          int *npass, int *low_used, int32_t *pass_sites, int32_t *bond_pass, double *cls,
+                          int32_t *ncls, int max_pass);
 /* ---- memory layout of full-space S=1/2 vectors (no reference counterpart) ----
  * Device memory holds basis state s at memory index swap(s), where swap exchanges up to 4
  * disjoint pairs of bits: site k (0-based) lives in memory bit site_bit[k], and

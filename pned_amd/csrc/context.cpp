@@ -896,6 +896,8 @@ static int ctx_create(int n_sites, int q_spin, int ibond, const int32_t *ipair, 
     const size_t qq = (size_t)q_spin * q_spin;
     ctx->xl.assign(xlocal_ham, xlocal_ham + 2 * qq * qq * 9);
     ctx->ll.assign(llocal_ham, llocal_ham + qq * qq * 9);
+    // the tiled spin-S kernel keeps 32-bit tile offsets; larger spaces use the gather kernel
+    if (d > (int64_t(1) << 32)) ctx->spin_tiled = 0;
     ctx->site_pair.resize(2 * ibond);
     for (int b = 0; b < ibond; ++b) {
         const int a = ipair[2 * b] - 1, c = ipair[2 * b + 1] - 1;
@@ -1060,6 +1062,8 @@ int pned_ctx_set_matrices(pned_ctx *ctx, const double *xlocal_ham, const int32_t
 int pned_ctx_spin_kernel(pned_ctx *ctx, int tiled)
 {
     if (!ctx) return fail(PNED_E_ARG, "null context");
+    if (tiled && ctx->dim > (int64_t(1) << 32))
+        return fail(PNED_E_UNSUPPORTED, "tiled spin-S kernel: at most 2^32 states (32-bit tile offsets)");
     ctx->spin_tiled = tiled ? 1 : 0;
     return PNED_OK;
 }

@@ -8,7 +8,10 @@
 // state) the nonzero input states are precomputed on the host, so the inner loop visits
 // only the elements the reference does not skip (llocal_Ham).
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <cstring>
+#include <utility>
 
 #include "internal.h"
 
@@ -214,42 +217,52 @@ int launch_hv_spin(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperb
 }
 
 // ------------------------------------------------------------------------------------
-// Tiled spin-S H.v.  A pass owns K tile sites (the lowest L digits always among them, so a
-// wave's lanes read contiguous runs); every bond with both sites in the tile is applied in
-// that pass from the tile staged in LDS.  The tile is the q^K states sharing the digits
-// of the other sites; tile-local index j = sum digit_k q^k over the tile sites in order.
-// Each pass reads v1 once and read-modify-writes v0 once (pass 0 may start from zero).
+// Tiled spin-S H.v.  A pass owns K tile sites (the lowest L sites always among them, so a
+// wave's lanes read contiguous runs of q^L states); every bond with both sites in the tile
+// is applied in that pass from the tile staged in LDS.  The tile is the q^K states sharing
+// the digits of the other sites; tile-local index j = sum digit_k q^k over the tile sites in
+// order.  Each pass reads v1 once and read-modify-writes v0 once (pass 0 may start from
+// zero).
 //
-// Per pass the host lists, for every (bond, output local state), the nonzero elements as
-// (partner offset in tile units, coefficient); the kernel stages that list in LDS once per
-// block.  A thread owns R tile slots j = tid + TPB r: their digit words and memory offsets
-// are computed once per launch and stay in registers, the next tile's v1 is prefetched
-// into registers while the current one is computed, and the slots are accumulated in
-// sweeps of S so only S accumulators and v0 values are live.
+// A bond's elements that the reference's llocal_Ham does not skip are grouped by their
+// partner shift (input minus output digit on each of the two sites).  Within such a class
+// the partner's tile offset is the same for every state; only the coefficient depends on
+// the state's two digits: a q^2-entry table, zero where the element is skipped or the
+// partner digit would leave [0, q).  A thread owns R tile slots j = tid + TPB r for the whole
+// launch, so each slot's digits are fixed: the diagonals of all the pass's bonds are summed
+// into one coefficient per slot at launch start, and per tile the slots visit the
+// off-diagonal classes under uniform (scalar) loop control, one coefficient and one partner
+// read from LDS each, S slots at a time for independent reads.  The next tile's v1 is
+// prefetched into registers while the current one is computed.  Tiles are handed out so that
+// the blocks of one XCD (blockIdx mod 8) work on neighbouring tiles at the same time: the
+// runs of neighbouring tiles share cache lines at their ends.
 // ------------------------------------------------------------------------------------
 constexpr int SP_MAX_TILE = 8192;     // states per tile (128 KB of complex128)
-constexpr int SP_MAX_BOND = 64;
+constexpr int SP_MAX_BOND = 64;       // bonds per pass: one record per lane of a wave
 constexpr int SP_BASE_TAB = 4096;
 constexpr int SP_MAX_QQ = 25;         // q <= 5
-constexpr int SP_MAX_ENT = 4096;      // nonzero entries per pass
+constexpr int SP_MAX_CLS = 64;        // off-diagonal (bond, partner shift) classes per pass: one per lane
+constexpr int SP_MAX_COEF = SP_MAX_CLS * SP_MAX_QQ;
 
 struct SpinPassDev {
-    uint64_t off[SP_MAX_TILE];        // memory offset of tile-local state j
+    uint32_t off[SP_MAX_TILE];        // memory offset of tile-local state j (dim <= 2^32)
     uint64_t blo[SP_BASE_TAB];        // tile base, low outer digits
     uint64_t bhi[SP_BASE_TAB];        // tile base, high outer digits
-    int32_t bond_pa[SP_MAX_BOND];     // tile position of the min site
-    int32_t bond_pb[SP_MAX_BOND];     // tile position of the max site
-    int32_t row_start[SP_MAX_BOND * SP_MAX_QQ + 1];   // entries of (bond, istat) row
-    int32_t ent_delta[SP_MAX_ENT];    // partner - j in tile units (0 = diagonal)
-    double ent_c[2 * SP_MAX_ENT];     // coefficient (re, im)
+    // bond record: digit-word shift (4 * tile position) of the min site (bits 0-4) and of the
+    // max site (5-9), first off-diagonal class (10-15), one past the last (16-22)
+    int32_t bond_rec[SP_MAX_BOND];
+    int32_t cls_delta[SP_MAX_CLS];    // (partner - j) in bytes of a complex128 tile entry
+    int32_t diag_kept[SP_MAX_BOND * SP_MAX_QQ];   // diagonal element present: the shift applies
+    double diag[2 * SP_MAX_BOND * SP_MAX_QQ];     // bond diagonal by output local state
+    double cls_c[2 * SP_MAX_COEF];    // class coefficient by output local state (0 where none)
 };
 
 struct SpinPassArgs {
     uint64_t tile_count;
     int tsz;                          // q^K
-    int K, nbond, diag_first, fresh;
-    int nent;                         // entries of this pass
+    int K, nbond, ncls, diag_first, fresh;
     int nlo_mod;                      // tile_count split: tix % nlo_mod, tix / nlo_mod
+    int xcd;                          // 1: neighbouring tiles on one XCD (grid a multiple of 8)
     double shift;                     // -eperbd per bond diagonal
     double xs;                        // v0 += xs (H v1); <v1|.> then carries one factor xs
     double ys;                        // first pass: factor on the v0 input
@@ -257,30 +270,47 @@ struct SpinPassArgs {
     int dotu;                         // 1: accumulate sum v1 * (H v1) without the conjugate
 };
 
-template <int Q, int TPB, int R, int S>
+// Slots are visited in sweeps of S; a sweep is skipped by the waves that hold no state of the
+// tile in its first slot, and within a sweep every slot is computed (a slot beyond the tile
+// reads clamped, in-bounds entries and is not stored), so the class loop has no branches.
+// S is chosen so that only the last sweep can be partial.
+template <int Q, int TPB, int R, int S, bool CPLX>
 __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ v1, double2 *__restrict__ v0,
                                                     const SpinPassDev *__restrict__ pd, SpinPassArgs a,
                                                     double *__restrict__ partial)
 {
     constexpr int QQ = Q * Q;
+    constexpr int CW = CPLX ? 16 : 8;     // bytes per staged coefficient
     extern __shared__ unsigned char spin_smem[];
     double2 *tile = reinterpret_cast<double2 *>(spin_smem);
-    // staged row table: starts, deltas, coefficients
-    const int nrow = a.nbond * QQ;
-    int32_t *rs = reinterpret_cast<int32_t *>(spin_smem + sizeof(double2) * (size_t)a.tsz);
-    int32_t *dl = rs + ((nrow + 1 + 3) & ~3);
-    double2 *cf = reinterpret_cast<double2 *>(dl + ((a.nent + 3) & ~3));
+    unsigned char *cfb = spin_smem + sizeof(double2) * (size_t)a.tsz;
     const int tid = threadIdx.x;
-    for (int i = tid; i <= nrow; i += TPB) rs[i] = pd->row_start[i];
-    for (int i = tid; i < a.nent; i += TPB) {
-        dl[i] = pd->ent_delta[i];
-        cf[i] = make_double2(pd->ent_c[2 * i], pd->ent_c[2 * i + 1]);
+    const int lane = tid & 63;
+    {
+        const int ncoef = a.ncls * QQ;
+        double *cf = reinterpret_cast<double *>(cfb);
+        for (int i = tid; i < ncoef; i += TPB) {
+            if (CPLX) {
+                cf[2 * i] = pd->cls_c[2 * i];
+                cf[2 * i + 1] = pd->cls_c[2 * i + 1];
+            } else {
+                cf[i] = pd->cls_c[2 * i];
+            }
+        }
     }
+    // the pass's bond and class records, lane b / c of every wave
+    const int brec = (lane < a.nbond) ? pd->bond_rec[lane] : 0;
+    const int crec = (lane < a.ncls) ? pd->cls_delta[lane] : 0;
     const bool rd0 = !(a.diag_first && a.fresh);
     const double ysc = a.diag_first ? a.ys : 1.0;
-    // per-slot digit words (4 bits per tile digit) and memory offsets, fixed for the launch
+    const uint32_t tmax = (uint32_t)(a.tsz - 1) * 16u;
+    // first thread of this wave: slot r holds a state of the tile for some lane of the wave
+    // iff wfirst + TPB r < tsz (a uniform test)
+    const int wfirst = __builtin_amdgcn_readfirstlane(tid & ~63);
+    // per slot for the launch: digit word (4 bits per tile digit), memory offset, diagonal
     uint32_t dw[R];
-    uint64_t offj[R];
+    uint32_t offj[R];
+    double gr[R], gi[CPLX ? R : 1];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int j = tid + TPB * r;
@@ -292,7 +322,21 @@ __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ 
         }
         dw[r] = w;
         offj[r] = (j < a.tsz) ? pd->off[j] : 0;
+        double sr = 0.0, si = (CPLX && a.diag_first) ? a.zi : 0.0;
+        if (j < a.tsz) {
+            for (int b = 0; b < a.nbond; ++b) {
+                const int rec = __builtin_amdgcn_readlane(brec, b);
+                const int o = b * QQ + Q * (int)((w >> ((rec >> 5) & 31)) & 15u) + (int)((w >> (rec & 31)) & 15u);
+                sr += pd->diag[2 * o] + (pd->diag_kept[o] ? a.shift : 0.0);
+                si += pd->diag[2 * o + 1];
+            }
+            if (a.diag_first) sr += a.zr;
+        }
+        gr[r] = sr;
+        if (CPLX) gi[r] = si;
     }
+    // pass 0 of a real pass: the imaginary part of the complex diagonal shift
+    const double zi = (a.diag_first && !CPLX) ? a.zi : 0.0;
     double pr_re = 0.0, pr_im = 0.0;
     double xr[R], xi[R];
     auto tile_base = [&](uint64_t tix) {
@@ -308,16 +352,17 @@ __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ 
             }
         }
     };
-    uint64_t tix = blockIdx.x;
+    const uint64_t G = gridDim.x;
+    uint64_t tix = a.xcd ? (uint64_t)(blockIdx.x & 7u) * (G >> 3) + (blockIdx.x >> 3) : (uint64_t)blockIdx.x;
     if (tix < a.tile_count) fetch_x(tile_base(tix));
-    __syncthreads();   // staged table visible
-    for (; tix < a.tile_count; tix += gridDim.x) {
+    __syncthreads();   // staged coefficients visible
+    for (; tix < a.tile_count; tix += G) {
         const uint64_t base = tile_base(tix);
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (tid + TPB * r < a.tsz) tile[tid + TPB * r] = make_double2(xr[r], xi[r]);
         __syncthreads();
-        const uint64_t nx = tix + gridDim.x;
+        const uint64_t nx = tix + G;
         // v0 of the first sweep before the next tile's v1 (in-order vector memory counters)
         double yr[S], yi[S];
         auto fetch_y = [&](int r0) {
@@ -326,7 +371,7 @@ __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ 
                 const int r = r0 + u;
                 yr[u] = 0.0;
                 yi[u] = 0.0;
-                if (rd0 && tid + TPB * r < a.tsz) {
+                if (r < R && rd0 && tid + TPB * r < a.tsz) {
                     const double2 t = v0[base + offj[r]];
                     yr[u] = t.x;
                     yi[u] = t.y;
@@ -338,41 +383,69 @@ __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ 
 #pragma unroll
         for (int r0 = 0; r0 < R; r0 += S) {
             if (r0 > 0) fetch_y(r0);
+            if (wfirst + TPB * r0 >= a.tsz) continue;   // uniform: no state of this sweep in the wave
+            double ar[S], ai[S];
+            uint32_t jb[S];   // byte offset of the slot's state in the tile (clamped)
+#pragma unroll
+            for (int u = 0; u < S; ++u) {
+                const int r = (r0 + u < R) ? r0 + u : R - 1;   // (slots past R are never used)
+                jb[u] = min((uint32_t)(tid + TPB * r) * 16u, tmax);
+                const double2 x = *reinterpret_cast<const double2 *>(spin_smem + jb[u]);
+                const double g_i = CPLX ? gi[r] : zi;
+                ar[u] = fma(gr[r], x.x, -g_i * x.y);
+                ai[u] = fma(gr[r], x.y, g_i * x.x);
+            }
+#pragma unroll 1
+            for (int b = 0; b < a.nbond; ++b) {
+                const int rec = __builtin_amdgcn_readlane(brec, b);
+                const int sa = rec & 31, sb = (rec >> 5) & 31;
+                const int c0 = (rec >> 10) & 63, c1 = (rec >> 16) & 127;
+                uint32_t ob[S];   // byte offset of the slot's output local state in a class table
+#pragma unroll
+                for (int u = 0; u < S; ++u) {
+                    const int r = (r0 + u < R) ? r0 + u : R - 1;
+                    ob[u] = (uint32_t)(Q * (int)((dw[r] >> sb) & 15u) + (int)((dw[r] >> sa) & 15u)) * CW;
+                }
+#pragma unroll 1
+                for (int c = c0; c < c1; ++c) {
+                    const uint32_t dl = (uint32_t)__builtin_amdgcn_readlane(crec, c);
+                    const unsigned char *cb = cfb + (size_t)c * QQ * CW;
+                    // all of the sweep's reads first, so they are in flight together
+                    double2 p[S], cc[S];
+#pragma unroll
+                    for (int u = 0; u < S; ++u) {
+                        if (r0 + u >= R) break;   // resolved at compile time
+                        p[u] = *reinterpret_cast<const double2 *>(spin_smem + min(jb[u] + dl, tmax));
+                        if (CPLX) cc[u] = *reinterpret_cast<const double2 *>(cb + ob[u]);
+                        else cc[u].x = *reinterpret_cast<const double *>(cb + ob[u]);
+                    }
+#pragma unroll
+                    for (int u = 0; u < S; ++u) {
+                        if (r0 + u >= R) break;
+                        if (CPLX) {
+                            ar[u] = fma(cc[u].x, p[u].x, fma(-cc[u].y, p[u].y, ar[u]));
+                            ai[u] = fma(cc[u].x, p[u].y, fma(cc[u].y, p[u].x, ai[u]));
+                        } else {
+                            ar[u] = fma(cc[u].x, p[u].x, ar[u]);
+                            ai[u] = fma(cc[u].x, p[u].y, ai[u]);
+                        }
+                    }
+                }
+            }
 #pragma unroll
             for (int u = 0; u < S; ++u) {
                 const int r = r0 + u;
-                const int j = tid + TPB * r;
-                if (j >= a.tsz) continue;
-                double ar = 0.0, ai = 0.0;
-                for (int b = 0; b < a.nbond; ++b) {
-                    const int d1 = (int)((dw[r] >> (4 * pd->bond_pa[b])) & 15u);
-                    const int d2 = (int)((dw[r] >> (4 * pd->bond_pb[b])) & 15u);
-                    const int row = b * QQ + Q * d2 + d1;
-                    const int e1 = rs[row + 1];
-                    for (int e = rs[row]; e < e1; ++e) {
-                        const int d = dl[e];
-                        const double2 c = cf[e];
-                        const double cr = (d == 0) ? c.x + a.shift : c.x;
-                        const double2 p = tile[j + d];
-                        ar = fma(cr, p.x, fma(-c.y, p.y, ar));
-                        ai = fma(cr, p.y, fma(c.y, p.x, ai));
-                    }
-                }
-                const double2 x = tile[j];
-                if (a.diag_first) {
-                    ar = fma(a.zr, x.x, fma(-a.zi, x.y, ar));
-                    ai = fma(a.zr, x.y, fma(a.zi, x.x, ai));
-                }
-                ar *= a.xs;
-                ai *= a.xs;
+                if (r >= R || tid + TPB * r >= a.tsz) continue;
+                const double2 x = tile[tid + TPB * r];
+                const double hr = ar[u] * a.xs, hi = ai[u] * a.xs;
                 if (a.dotu) {
-                    pr_re += x.x * ar - x.y * ai;
-                    pr_im += x.x * ai + x.y * ar;
+                    pr_re += x.x * hr - x.y * hi;
+                    pr_im += x.x * hi + x.y * hr;
                 } else {
-                    pr_re += x.x * ar + x.y * ai;
-                    pr_im += x.x * ai - x.y * ar;
+                    pr_re += x.x * hr + x.y * hi;
+                    pr_im += x.x * hi - x.y * hr;
                 }
-                v0[base + offj[r]] = make_double2(fma(ysc, yr[u], ar), fma(ysc, yi[u], ai));
+                v0[base + offj[r]] = make_double2(fma(ysc, yr[u], hr), fma(ysc, yi[u], hi));
             }
         }
         __syncthreads();
@@ -394,8 +467,13 @@ __global__ __launch_bounds__(TPB) void hv_spin_pass(const double2 *__restrict__ 
     }
 }
 
-// greedy cover of the bonds by tile-site sets (lowest L sites forced)
-static std::vector<std::vector<int>> spin_cover(int n, const std::vector<int32_t> &sp, int ibond, int K, int L)
+// ---- planner (host only) ------------------------------------------------------------
+
+namespace {
+
+// greedy cover of the bonds by tile-site sets (lowest L sites forced): returns one site set
+// per pass
+std::vector<std::vector<int>> spin_greedy_cover(int n, const std::vector<int32_t> &sp, int ibond, int K, int L)
 {
     std::vector<char> done(ibond, 0);
     int remaining = ibond;
@@ -425,75 +503,403 @@ static std::vector<std::vector<int>> spin_cover(int n, const std::vector<int32_t
             in[best] = 1;
             ++cnt;
         }
-        std::vector<int> bonds;
+        bool progress = false;
         for (int b = 0; b < ibond; ++b)
-            if (!done[b] && in[sp[2 * b]] && in[sp[2 * b + 1]]) { done[b] = 1; --remaining; bonds.push_back(b); }
-        if (bonds.empty()) return {};
+            if (!done[b] && in[sp[2 * b]] && in[sp[2 * b + 1]]) { done[b] = 1; --remaining; progress = true; }
+        if (!progress) return {};
         std::vector<int> sites;
         for (int st = 0; st < n; ++st)
             if (in[st]) sites.push_back(st);
         out.push_back(sites);
-        out.push_back(bonds);
     }
     return out;
 }
 
-// Spreads the bonds over the passes whose tiles hold both of their sites: bonds with the
-// fewest covering passes are placed first, each on the pass with the fewest nonzero
-// elements so far (w[b]: elements of bond b's matrix that are not skipped), so that no pass
-// carries many more (bond, state) elements than the others. Passes left without bonds are
-// dropped.
-static void spin_balance(std::vector<std::vector<int>> &cov, const std::vector<int32_t> &sp, int n,
-                         const std::vector<int> &w)
+uint64_t sp_mix(uint64_t &s)
 {
-    const int np = (int)cov.size() / 2;
-    std::vector<std::vector<char>> has(np, std::vector<char>(n, 0));
-    std::vector<int> all;
-    for (int p = 0; p < np; ++p) {
-        for (int st : cov[2 * p]) has[p][st] = 1;
-        for (int b : cov[2 * p + 1]) all.push_back(b);
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// P tile-site sets (each holding sites 0..L-1 and K-L others) covering every bond, by
+// annealing from random sets: a move exchanges one free site of one set for a site outside
+// it; the cost is the number of bonds no set covers (Metropolis at a falling temperature).
+// Returns the sets, or nothing when the search ends without a cover.
+std::vector<std::vector<int>> spin_anneal_cover(int n, const std::vector<int32_t> &sp, int ibond, int K, int L,
+                                                int P, uint64_t seed, int moves)
+{
+    if (K >= n || K - L < 1) return {};
+    std::vector<std::vector<char>> in(P, std::vector<char>(n, 0));
+    for (int p = 0; p < P; ++p) {
+        for (int k = 0; k < L; ++k) in[p][k] = 1;
+        std::vector<int> fr;
+        for (int st = L; st < n; ++st) fr.push_back(st);
+        for (int k = 0; k < K - L; ++k) {
+            const int i = k + (int)(sp_mix(seed) % (uint64_t)(fr.size() - k));
+            std::swap(fr[k], fr[i]);
+            in[p][fr[k]] = 1;
+        }
     }
-    std::vector<std::vector<int>> choice(all.size());
-    for (size_t i = 0; i < all.size(); ++i)
+    std::vector<int> cov(ibond, 0);
+    for (int p = 0; p < P; ++p)
+        for (int b = 0; b < ibond; ++b) cov[b] += in[p][sp[2 * b]] && in[p][sp[2 * b + 1]];
+    int unc = 0;
+    for (int b = 0; b < ibond; ++b) unc += cov[b] == 0;
+    double T = 1.0;
+    for (int it = 0; it < moves && unc > 0; ++it) {
+        const int p = (int)(sp_mix(seed) % (uint64_t)P);
+        int so, si;
+        do { so = L + (int)(sp_mix(seed) % (uint64_t)(n - L)); } while (!in[p][so]);
+        do { si = L + (int)(sp_mix(seed) % (uint64_t)(n - L)); } while (in[p][si]);
+        // change of the uncovered count when site so leaves set p and si joins it
+        int d = 0;
+        for (int b = 0; b < ibond; ++b) {
+            const int x = sp[2 * b], y = sp[2 * b + 1];
+            const bool before = in[p][x] && in[p][y];
+            const bool after = (x != so && y != so) && (in[p][x] || x == si) && (in[p][y] || y == si);
+            if (before == after) continue;
+            if (before && cov[b] == 1) ++d;
+            if (after && cov[b] == 0) --d;
+        }
+        const double u = (double)(sp_mix(seed) >> 11) * (1.0 / 9007199254740992.0);
+        if (d <= 0 || u < std::exp(-d / T)) {
+            for (int b = 0; b < ibond; ++b) {
+                const int x = sp[2 * b], y = sp[2 * b + 1];
+                const bool before = in[p][x] && in[p][y];
+                const bool after = (x != so && y != so) && (in[p][x] || x == si) && (in[p][y] || y == si);
+                cov[b] += (int)after - (int)before;
+            }
+            in[p][so] = 0;
+            in[p][si] = 1;
+            unc += d;
+        }
+        T = std::max(0.05, T * 0.9999);
+    }
+    if (unc > 0) return {};
+    std::vector<std::vector<int>> out(P);
+    for (int p = 0; p < P; ++p)
+        for (int st = 0; st < n; ++st)
+            if (in[p][st]) out[p].push_back(st);
+    return out;
+}
+
+constexpr uint64_t SPIN_COVER_SEED = 0x5EED0003ull;
+constexpr int SPIN_COVER_TRIES = 4;
+constexpr int SPIN_COVER_MOVES = 60000;
+
+// the fewest passes found: the greedy cover, then annealed covers with one set fewer until
+// a search fails
+std::vector<std::vector<int>> spin_cover(int n, const std::vector<int32_t> &sp, int ibond, int K, int L)
+{
+    auto best = spin_greedy_cover(n, sp, ibond, K, L);
+    if (best.empty()) return best;
+    uint64_t seed = SPIN_COVER_SEED ^ ((uint64_t)K << 8) ^ (uint64_t)L;
+    for (int P = (int)best.size() - 1; P >= 1; --P) {
+        std::vector<std::vector<int>> got;
+        for (int t = 0; t < SPIN_COVER_TRIES && got.empty(); ++t)
+            got = spin_anneal_cover(n, sp, ibond, K, L, P, sp_mix(seed), SPIN_COVER_MOVES);
+        if (got.empty()) break;
+        best.swap(got);
+    }
+    return best;
+}
+
+// Spreads the bonds over the passes whose tiles hold both of their sites: bonds with the
+// fewest covering passes are placed first, each on the pass with the least work so far
+// (w[b]: the bond's cost in the kernel), so that no pass carries much more than the others.
+// Passes left without bonds are dropped.  Returns per pass its sites and bonds.
+std::vector<std::pair<std::vector<int>, std::vector<int>>> spin_balance(const std::vector<std::vector<int>> &cov,
+                                                                        const std::vector<int32_t> &sp, int n,
+                                                                        int ibond, const std::vector<int> &w)
+{
+    const int np = (int)cov.size();
+    std::vector<std::vector<char>> has(np, std::vector<char>(n, 0));
+    for (int p = 0; p < np; ++p)
+        for (int st : cov[p]) has[p][st] = 1;
+    std::vector<std::vector<int>> choice(ibond);
+    for (int b = 0; b < ibond; ++b)
         for (int p = 0; p < np; ++p)
-            if (has[p][sp[2 * all[i]]] && has[p][sp[2 * all[i] + 1]]) choice[i].push_back(p);
-    std::vector<size_t> order(all.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+            if (has[p][sp[2 * b]] && has[p][sp[2 * b + 1]]) choice[b].push_back(p);
+    std::vector<int> order(ibond);
+    for (int b = 0; b < ibond; ++b) order[b] = b;
     std::stable_sort(order.begin(), order.end(),
-                     [&](size_t a, size_t b) { return choice[a].size() < choice[b].size(); });
+                     [&](int a, int b) { return choice[a].size() < choice[b].size(); });
     std::vector<std::vector<int>> bonds(np);
     std::vector<long> load(np, 0);
-    for (size_t i : order) {
-        int best = choice[i][0];
-        for (int p : choice[i])
+    for (int b : order) {
+        int best = choice[b][0];
+        for (int p : choice[b])
             if (load[p] < load[best]) best = p;
-        bonds[best].push_back(all[i]);
-        load[best] += w[all[i]];
+        bonds[best].push_back(b);
+        load[best] += w[b];
     }
-    std::vector<std::vector<int>> out;
+    std::vector<std::pair<std::vector<int>, std::vector<int>>> out;
     for (int p = 0; p < np; ++p) {
         if (bonds[p].empty()) continue;
         std::sort(bonds[p].begin(), bonds[p].end());
-        out.push_back(cov[2 * p]);
-        out.push_back(bonds[p]);
+        out.emplace_back(cov[p], bonds[p]);
     }
-    cov.swap(out);
+    return out;
 }
 
-// LDS bytes of one pass: tile, then the row table (starts, deltas, coefficients)
-static size_t spin_lds_bytes(int tsz, int nrow, int nent, int tpb)
+// Splits a pass whose bonds hold more off-diagonal classes or bonds than one wave holds
+// records for (64 each) into passes over the same tile sites.
+std::vector<std::pair<std::vector<int>, std::vector<int>>> spin_split(
+    const std::vector<std::pair<std::vector<int>, std::vector<int>>> &passes, const std::vector<int32_t> &btype,
+    const std::vector<std::vector<std::pair<int, int>>> &tcls)
 {
-    const size_t t = sizeof(double2) * (size_t)tsz + sizeof(int32_t) * (size_t)((nrow + 1 + 3) & ~3) +
-                     sizeof(int32_t) * (size_t)((nent + 3) & ~3) + sizeof(double2) * (size_t)nent;
+    std::vector<std::pair<std::vector<int>, std::vector<int>>> out;
+    for (const auto &p : passes) {
+        std::vector<int> cur;
+        int nc = 0;
+        for (int b : p.second) {
+            const int k = (int)tcls[btype[b] - 1].size();
+            if (!cur.empty() && (nc + k > SP_MAX_CLS || (int)cur.size() == SP_MAX_BOND)) {
+                out.emplace_back(p.first, cur);
+                cur.clear();
+                nc = 0;
+            }
+            cur.push_back(b);
+            nc += k;
+        }
+        if (!cur.empty()) out.emplace_back(p.first, cur);
+    }
+    return out;
+}
+
+// off-diagonal partner shifts of a bond type: (in - out) digit of the min and max site
+// among the elements that are not skipped
+std::vector<std::pair<int, int>> spin_classes(int q, const std::vector<int32_t> &ll, int t)
+{
+    const int qq = q * q;
+    std::vector<std::pair<int, int>> cls;
+    for (int out = 0; out < qq; ++out)
+        for (int in = 0; in < qq; ++in) {
+            if (in == out) continue;
+            if (ll[(size_t)in + (size_t)qq * out + (size_t)qq * qq * t]) continue;
+            const std::pair<int, int> k(in % q - out % q, in / q - out / q);
+            if (std::find(cls.begin(), cls.end(), k) == cls.end()) cls.push_back(k);
+        }
+    std::sort(cls.begin(), cls.end());
+    return cls;
+}
+
+// LDS bytes of one pass: tile, then the class coefficients
+size_t spin_lds_bytes(int tsz, int ncls, int qq, bool cplx, int tpb)
+{
+    const size_t t = sizeof(double2) * (size_t)tsz + (size_t)ncls * qq * (cplx ? 16 : 8);
     return std::max(t, sizeof(double) * 2 * (size_t)tpb);
 }
 
+struct SpinPlan {
+    int K = 0, L = 0;
+    std::vector<SpinPassDev> dev;
+    std::vector<std::vector<int>> sites, bonds;
+    std::vector<int> tsz, nlo, ncls, cplx;
+    std::vector<std::vector<std::array<int, 3>>> cls_key;   // per pass and class: bond, digit shifts
+};
+
+// Builds the passes for tile sites K (at most, fewer when the tables do not fit) and L forced
+// low sites (0: 3 if that costs no pass over 2, else 2).
+int spin_make_plan(int n, int q, int ibond, const std::vector<int32_t> &sp, const std::vector<int32_t> &btype,
+                   const std::vector<double> &xl, const std::vector<int32_t> &ll, int Kmax, int Lreq, SpinPlan &plan)
+{
+    const int qq = q * q;
+    if (q < 3 || q > 5) return fail(PNED_E_UNSUPPORTED, "tiled spin-S kernel: q = 3..5");
+    std::vector<std::vector<std::pair<int, int>>> tcls(9);
+    for (int t = 0; t < 9; ++t) tcls[t] = spin_classes(q, ll, t);
+    std::vector<int> w(ibond);
+    for (int b = 0; b < ibond; ++b) w[b] = 1 + 4 * (int)tcls[btype[b] - 1].size();
+    std::vector<uint64_t> pw(n);
+    {
+        uint64_t x = 1;
+        for (int k = 0; k < n; ++k) { pw[k] = x; x *= (uint64_t)q; }
+    }
+    for (int K = Kmax; K >= 1; --K) {
+        std::vector<std::vector<int>> cov;
+        int L = 0;
+        if (Lreq > 0) {
+            L = std::min(Lreq, K);
+            cov = spin_cover(n, sp, ibond, K, L);
+        } else {
+            auto c2 = spin_cover(n, sp, ibond, K, std::min(2, K));
+            auto c3 = (K >= 3) ? spin_cover(n, sp, ibond, K, 3) : std::vector<std::vector<int>>{};
+            if (!c3.empty() && (c2.empty() || c3.size() <= c2.size())) { cov = c3; L = 3; }
+            else { cov = c2; L = std::min(2, K); }
+        }
+        // tiles too small to hold every bond next to the forced sites: force fewer
+        while (cov.empty() && L > 0) cov = spin_cover(n, sp, ibond, K, --L);
+        if (cov.empty()) return fail(PNED_E_UNSUPPORTED, "spin-S tile planner made no progress");
+        auto passes = spin_split(spin_balance(cov, sp, n, ibond, w), btype, tcls);
+        const int np = (int)passes.size();
+        plan = SpinPlan{};
+        plan.K = K;
+        plan.L = L;
+        plan.dev.assign(np, SpinPassDev{});
+        bool fits = true;
+        for (int p = 0; p < np && fits; ++p) {
+            const std::vector<int> &sites = passes[p].first, &bonds = passes[p].second;
+            SpinPassDev &d = plan.dev[p];
+            if ((int)bonds.size() > SP_MAX_BOND) return fail(PNED_E_UNSUPPORTED, "too many bonds in a spin-S pass");
+            const int Kp = (int)sites.size();
+            int ts = 1;
+            for (int k = 0; k < Kp; ++k) ts *= q;
+            for (int j = 0; j < ts; ++j) {
+                uint64_t o = 0;
+                int y = j;
+                for (int k = 0; k < Kp; ++k) { o += (uint64_t)(y % q) * pw[sites[k]]; y /= q; }
+                d.off[j] = (uint32_t)o;
+            }
+            std::vector<int> outer;
+            for (int st = 0; st < n; ++st)
+                if (std::find(sites.begin(), sites.end(), st) == sites.end()) outer.push_back(st);
+            const int no = (int)outer.size();
+            int nl = 0;
+            int64_t lm = 1;
+            while (nl < no && lm * q <= SP_BASE_TAB) { lm *= q; ++nl; }
+            int64_t hm = 1;
+            for (int k = nl; k < no; ++k) hm *= q;
+            if (hm > SP_BASE_TAB) return fail(PNED_E_UNSUPPORTED, "too many outer sites for the spin-S tables");
+            for (int64_t t2 = 0; t2 < lm; ++t2) {
+                uint64_t o = 0;
+                int64_t y = t2;
+                for (int k = 0; k < nl; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
+                d.blo[t2] = o;
+            }
+            for (int64_t t2 = 0; t2 < hm; ++t2) {
+                uint64_t o = 0;
+                int64_t y = t2;
+                for (int k = nl; k < no; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
+                d.bhi[t2] = o;
+            }
+            int nc = 0;
+            bool cplx = false;
+            std::vector<std::array<int, 3>> keys;
+            for (size_t bi = 0; bi < bonds.size(); ++bi) {
+                const int bond = bonds[bi];
+                const int sa = sp[2 * bond], sb = sp[2 * bond + 1];
+                const int pa = (int)(std::find(sites.begin(), sites.end(), sa) - sites.begin());
+                const int pbp = (int)(std::find(sites.begin(), sites.end(), sb) - sites.begin());
+                int wa = 1, wb = 1;
+                for (int k = 0; k < pa; ++k) wa *= q;
+                for (int k = 0; k < pbp; ++k) wb *= q;
+                const int t = btype[bond] - 1;
+                const int nct = (int)tcls[t].size();
+                if (nc + nct > SP_MAX_CLS) return fail(PNED_E_UNSUPPORTED, "spin-S pass tables too large");
+                d.bond_rec[bi] = (4 * pa) | ((4 * pbp) << 5) | (nc << 10) | ((nc + nct) << 16);
+                for (int out = 0; out < qq; ++out) {
+                    const size_t f = (size_t)out + (size_t)qq * out + (size_t)qq * qq * t;
+                    const size_t e = bi * qq + out;
+                    d.diag_kept[e] = ll[f] ? 0 : 1;
+                    d.diag[2 * e] = ll[f] ? 0.0 : xl[2 * f];
+                    d.diag[2 * e + 1] = ll[f] ? 0.0 : xl[2 * f + 1];
+                    cplx = cplx || d.diag[2 * e + 1] != 0.0;   // the real kernel drops it
+                }
+                for (const auto &k : tcls[t]) {
+                    keys.push_back({bond, k.first, k.second});
+                    d.cls_delta[nc] = 16 * (k.first * wa + k.second * wb);
+                    for (int out = 0; out < qq; ++out) {
+                        const int d1 = out % q + k.first, d2 = out / q + k.second;
+                        double cr = 0.0, ci = 0.0;
+                        if (d1 >= 0 && d1 < q && d2 >= 0 && d2 < q) {
+                            const int in = d1 + q * d2;
+                            const size_t f = (size_t)in + (size_t)qq * out + (size_t)qq * qq * t;
+                            if (!ll[f]) {
+                                cr = xl[2 * f];
+                                ci = xl[2 * f + 1];
+                            }
+                        }
+                        d.cls_c[2 * ((size_t)nc * qq + out)] = cr;
+                        d.cls_c[2 * ((size_t)nc * qq + out) + 1] = ci;
+                        cplx = cplx || ci != 0.0;
+                    }
+                    ++nc;
+                }
+            }
+            plan.sites.push_back(sites);
+            plan.bonds.push_back(bonds);
+            plan.tsz.push_back(ts);
+            plan.nlo.push_back((int)lm);
+            plan.ncls.push_back(nc);
+            plan.cplx.push_back(cplx ? 1 : 0);
+            plan.cls_key.push_back(keys);
+            fits = spin_lds_bytes(ts, nc, qq, cplx, 1024) <= 160 * 1024;
+        }
+        if (fits) return PNED_OK;
+    }
+    return fail(PNED_E_UNSUPPORTED, "spin-S tables do not fit LDS");
+}
+
+int spin_kmax(int n, int q)
+{
+    int K = 0;
+    int64_t p = 1;
+    while (K < n && p * q <= SP_MAX_TILE) { p *= q; ++K; }
+    return K;
+}
+
+}  // namespace
+
+extern "C" int pned_spin_plan_export(int n_sites, int q, int ibond, const int32_t *ipair, const int32_t *ibond_type,
+                                     const double *xlocal_ham, const int32_t *llocal_ham, int tile_sites,
+                                     int low_sites, int *npass, int *low_used, int32_t *pass_sites,
+                                     int32_t *bond_pass, double *cls, int32_t *ncls, int max_pass)
+{
+    if (!ipair || !ibond_type || !xlocal_ham || !llocal_ham || !npass || !low_used || !pass_sites || !bond_pass ||
+        !cls || !ncls)
+        return fail(PNED_E_ARG, "null argument");
+    if (q < 3 || q > 5 || n_sites < 2 || n_sites > 32 || ibond < 1) return fail(PNED_E_ARG, "q = 3..5, 2..32 sites");
+    const int qq = q * q;
+    std::vector<int32_t> sp(2 * ibond), bt(ibond_type, ibond_type + ibond);
+    for (int b = 0; b < ibond; ++b) {
+        const int a = ipair[2 * b] - 1, c = ipair[2 * b + 1] - 1;
+        if (a < 0 || a >= n_sites || c < 0 || c >= n_sites) return fail(PNED_E_ARG, "ipair out of range");
+        sp[2 * b] = std::min(a, c);
+        sp[2 * b + 1] = std::max(a, c);
+    }
+    std::vector<double> xl(xlocal_ham, xlocal_ham + 2 * (size_t)qq * qq * 9);
+    std::vector<int32_t> ll(llocal_ham, llocal_ham + (size_t)qq * qq * 9);
+    int K = spin_kmax(n_sites, q);
+    if (tile_sites > 0 && tile_sites < K) K = tile_sites;
+    SpinPlan plan;
+    PNED_TRY(spin_make_plan(n_sites, q, ibond, sp, bt, xl, ll, K, low_sites, plan));
+    const int np = (int)plan.dev.size();
+    *npass = np;
+    *low_used = plan.L;
+    if (np > max_pass) return fail(PNED_E_ARG, "output arrays too small");
+    const int row = 3 + 2 * qq;
+    for (int p = 0; p < np; ++p) {
+        for (int k = 0; k < 32; ++k) pass_sites[p * 32 + k] = k < (int)plan.sites[p].size() ? plan.sites[p][k] : -1;
+        ncls[p] = plan.ncls[p];
+        const SpinPassDev &d = plan.dev[p];
+        for (size_t bi = 0; bi < plan.bonds[p].size(); ++bi) {
+            const int bond = plan.bonds[p][bi];
+            bond_pass[bond] = p;
+        }
+        for (int c = 0; c < plan.ncls[p]; ++c) {
+            double *o = cls + ((size_t)p * SP_MAX_CLS + c) * row;
+            o[0] = plan.cls_key[p][c][0];
+            o[1] = plan.cls_key[p][c][1];
+            o[2] = plan.cls_key[p][c][2];
+            for (int e = 0; e < 2 * qq; ++e) o[3 + e] = d.cls_c[2 * (size_t)c * qq + e];
+        }
+    }
+    return PNED_OK;
+}
+
 template <int Q, int TPB, int R>
-static void launch_spin_pass(pned_ctx *ctx, int grid, size_t lds, const pned_vec *v1, pned_vec *v0,
+static void launch_spin_pass(pned_ctx *ctx, int grid, size_t lds, bool cplx, const pned_vec *v1, pned_vec *v0,
                              const SpinPassDev *pd, const SpinPassArgs &a, double *part)
 {
-    constexpr int S = (R < 4) ? R : 4;
-    auto k = hv_spin_pass<Q, TPB, R, S>;
+    // slots per sweep: only the last sweep may be partial (the instances' tiles fill all but
+    // the last slot), and 7 slots of 1024 threads keep 2 (real) or 1 (complex coefficients)
+    // so that 128 VGPRs hold every slot's state
+    constexpr int S = (R == 7) ? 2 : (R == 4 && Q == 4) ? 4 : (R > 1) ? R - 1 : 1;
+    constexpr int SC = (R == 7) ? 1 : S;
+    auto k = cplx ? hv_spin_pass<Q, TPB, R, SC, true> : hv_spin_pass<Q, TPB, R, S, false>;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k, dim3(grid), dim3(TPB), lds, ctx->stream, (const double2 *)v1->data, (double2 *)v0->data,
                        pd, a, part);
@@ -508,125 +914,25 @@ int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double
 int launch_hv_spin_tiled_shift(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
                                double *prdct_host, double xs, double ys, double zr, double zi, bool dotu)
 {
-    const int q = ctx->q, n = ctx->n, qq = q * q;
-    const int L0 = (ctx->tune_low_bits > 0) ? ctx->tune_low_bits : 2;
+    const int q = ctx->q, qq = q * q;
     if (ctx->spin_pd == nullptr) {
-        // tile sites: the largest K with q^K <= 8192 whose passes fit LDS with their tables
-        int K = 0;
-        {
-            int64_t p = 1;
-            while (K < n && p * q <= SP_MAX_TILE) { p *= q; ++K; }
-        }
+        int K = spin_kmax(ctx->n, q);
         if (ctx->tune_tile_bits > 0 && ctx->tune_tile_bits < K) K = ctx->tune_tile_bits;
-        std::vector<SpinPassDev> host;
-        std::vector<int> nb, tsz, nlo, nent;
-        for (; K >= 1; --K) {
-            const int L = std::min(L0, K);
-            auto cov = spin_cover(n, ctx->site_pair, ctx->ibond, K, L);
-            if (cov.empty()) return fail(PNED_E_UNSUPPORTED, "spin-S tile planner made no progress");
-            {
-                std::vector<int> w(ctx->ibond, 0);
-                for (int b = 0; b < ctx->ibond; ++b) {
-                    const size_t t = (size_t)(ctx->btype[b] - 1);
-                    for (int e = 0; e < qq * qq; ++e) w[b] += ctx->ll[(size_t)e + (size_t)qq * qq * t] ? 0 : 1;
-                }
-                spin_balance(cov, ctx->site_pair, n, w);
-            }
-            const int np = (int)cov.size() / 2;
-            host.assign(np, SpinPassDev{});
-            nb.assign(np, 0);
-            tsz.assign(np, 0);
-            nlo.assign(np, 0);
-            nent.assign(np, 0);
-            bool fits = true;
-            std::vector<uint64_t> pw(n);
-            {
-                uint64_t x = 1;
-                for (int k = 0; k < n; ++k) { pw[k] = x; x *= (uint64_t)q; }
-            }
-            for (int p = 0; p < np && fits; ++p) {
-                const std::vector<int> &sites = cov[2 * p], &bonds = cov[2 * p + 1];
-                SpinPassDev &d = host[p];
-                if ((int)bonds.size() > SP_MAX_BOND) return fail(PNED_E_UNSUPPORTED, "too many bonds in a spin-S pass");
-                const int Kp = (int)sites.size();
-                int ts = 1;
-                for (int k = 0; k < Kp; ++k) ts *= q;
-                for (int j = 0; j < ts; ++j) {
-                    uint64_t o = 0;
-                    int y = j;
-                    for (int k = 0; k < Kp; ++k) { o += (uint64_t)(y % q) * pw[sites[k]]; y /= q; }
-                    d.off[j] = o;
-                }
-                std::vector<int> outer;
-                for (int st = 0; st < n; ++st)
-                    if (std::find(sites.begin(), sites.end(), st) == sites.end()) outer.push_back(st);
-                const int no = (int)outer.size();
-                int nl = 0;
-                int64_t lm = 1;
-                while (nl < no && lm * q <= SP_BASE_TAB) { lm *= q; ++nl; }
-                int64_t hm = 1;
-                for (int k = nl; k < no; ++k) hm *= q;
-                if (hm > SP_BASE_TAB) return fail(PNED_E_UNSUPPORTED, "too many outer sites for the spin-S tables");
-                for (int64_t t2 = 0; t2 < lm; ++t2) {
-                    uint64_t o = 0;
-                    int64_t y = t2;
-                    for (int k = 0; k < nl; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
-                    d.blo[t2] = o;
-                }
-                for (int64_t t2 = 0; t2 < hm; ++t2) {
-                    uint64_t o = 0;
-                    int64_t y = t2;
-                    for (int k = nl; k < no; ++k) { o += (uint64_t)(y % q) * pw[outer[k]]; y /= q; }
-                    d.bhi[t2] = o;
-                }
-                // (bond, output local state) rows of nonzero elements, as the reference's
-                // H_dot_v visits them (llocal_Ham skips), partners as tile offsets
-                int ne = 0;
-                for (size_t bi = 0; bi < bonds.size(); ++bi) {
-                    const int bond = bonds[bi];
-                    const int sa = ctx->site_pair[2 * bond], sb = ctx->site_pair[2 * bond + 1];
-                    const int pa = (int)(std::find(sites.begin(), sites.end(), sa) - sites.begin());
-                    const int pbp = (int)(std::find(sites.begin(), sites.end(), sb) - sites.begin());
-                    int wa = 1, wb = 1;
-                    for (int k = 0; k < pa; ++k) wa *= q;
-                    for (int k = 0; k < pbp; ++k) wb *= q;
-                    d.bond_pa[bi] = pa;
-                    d.bond_pb[bi] = pbp;
-                    const int t = ctx->btype[bond] - 1;
-                    for (int out = 0; out < qq; ++out) {
-                        d.row_start[bi * qq + out] = ne;
-                        const int d1 = out % q, d2 = out / q;
-                        for (int in = 0; in < qq; ++in) {
-                            const size_t f = (size_t)in + (size_t)qq * out + (size_t)qq * qq * t;
-                            if (ctx->ll[f]) continue;
-                            if (ne >= SP_MAX_ENT) return fail(PNED_E_UNSUPPORTED, "spin-S pass table too large");
-                            d.ent_delta[ne] = (in % q - d1) * wa + (in / q - d2) * wb;
-                            d.ent_c[2 * ne] = ctx->xl[2 * f];
-                            d.ent_c[2 * ne + 1] = ctx->xl[2 * f + 1];
-                            ++ne;
-                        }
-                    }
-                }
-                d.row_start[bonds.size() * qq] = ne;
-                nb[p] = (int)bonds.size();
-                tsz[p] = ts;
-                nlo[p] = (int)lm;
-                nent[p] = ne;
-                fits = spin_lds_bytes(ts, (int)bonds.size() * qq, ne, 1024) <= 160 * 1024;
-            }
-            if (fits) break;
-        }
-        if (K < 1) return fail(PNED_E_UNSUPPORTED, "spin-S tables do not fit LDS");
-        const int np = (int)host.size();
+        SpinPlan plan;
+        PNED_TRY(spin_make_plan(ctx->n, q, ctx->ibond, ctx->site_pair, ctx->btype, ctx->xl, ctx->ll, K,
+                                ctx->tune_low_bits, plan));
+        const int np = (int)plan.dev.size();
         PNED_HIP(hipStreamSynchronize(ctx->stream));
         PNED_HIP(hipMalloc(&ctx->spin_pd, sizeof(SpinPassDev) * np));
-        PNED_HIP(hipMemcpy(ctx->spin_pd, host.data(), sizeof(SpinPassDev) * np, hipMemcpyHostToDevice));
-        ctx->spin_K = K;
+        PNED_HIP(hipMemcpy(ctx->spin_pd, plan.dev.data(), sizeof(SpinPassDev) * np, hipMemcpyHostToDevice));
+        ctx->spin_K = plan.K;
         ctx->spin_np = np;
-        ctx->spin_nb = nb;
-        ctx->spin_tsz = tsz;
-        ctx->spin_nlo = nlo;
-        ctx->spin_nent = nent;
+        ctx->spin_nb.clear();
+        for (const auto &b : plan.bonds) ctx->spin_nb.push_back((int)b.size());
+        ctx->spin_tsz = plan.tsz;
+        ctx->spin_nlo = plan.nlo;
+        ctx->spin_ncls = plan.ncls;
+        ctx->spin_cplx = plan.cplx;
     }
     const int np = ctx->spin_np;
     const int grid_cap = 2048;
@@ -640,7 +946,7 @@ int launch_hv_spin_tiled_shift(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, 
         a.K = 0;
         for (int t = 1; t < a.tsz; t *= q) ++a.K;
         a.nbond = ctx->spin_nb[p];
-        a.nent = ctx->spin_nent[p];
+        a.ncls = ctx->spin_ncls[p];
         a.diag_first = (p == 0);
         a.fresh = fresh ? 1 : 0;
         a.nlo_mod = ctx->spin_nlo[p];
@@ -651,23 +957,22 @@ int launch_hv_spin_tiled_shift(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, 
         a.zi = zi;
         a.dotu = dotu ? 1 : 0;
         const int grid = (int)std::min<uint64_t>(a.tile_count, (uint64_t)grid_cap);
+        a.xcd = (grid % 8 == 0) ? 1 : 0;
         const SpinPassDev *pd = reinterpret_cast<const SpinPassDev *>(ctx->spin_pd) + p;
         double *part = ctx->d_partial + 2 * (size_t)rows;
-        // 1024 threads for tiles above 2048 states, else 256; R slots per thread (power of 2)
+        // 1024 threads for tiles above 2048 states, else 256; R = ceil(tsz / threads) slots
         const int tpb = (a.tsz > 2048) ? 1024 : 256;
-        int R = 1;
-        while (R * tpb < a.tsz) R *= 2;
-        const size_t lds = spin_lds_bytes(a.tsz, a.nbond * qq, a.nent, tpb);
+        const int R = (a.tsz + tpb - 1) / tpb;
+        const bool cplx = ctx->spin_cplx[p] != 0;
+        const size_t lds = spin_lds_bytes(a.tsz, a.ncls, qq, cplx, tpb);
 #define PNED_SPIN_CASE(QV, TV, RV)                                                                                   \
         if (q == QV && tpb == TV && R == RV) {                                                                      \
-            launch_spin_pass<QV, TV, RV>(ctx, grid, lds, v1, v0, pd, a, part);                                      \
+            launch_spin_pass<QV, TV, RV>(ctx, grid, lds, cplx, v1, v0, pd, a, part);                                \
         } else
-        PNED_SPIN_CASE(3, 1024, 8) PNED_SPIN_CASE(3, 1024, 4) PNED_SPIN_CASE(3, 256, 8) PNED_SPIN_CASE(3, 256, 4)
-        PNED_SPIN_CASE(3, 256, 2) PNED_SPIN_CASE(3, 256, 1)
-        PNED_SPIN_CASE(4, 1024, 8) PNED_SPIN_CASE(4, 1024, 4) PNED_SPIN_CASE(4, 256, 8) PNED_SPIN_CASE(4, 256, 4)
-        PNED_SPIN_CASE(4, 256, 2) PNED_SPIN_CASE(4, 256, 1)
-        PNED_SPIN_CASE(5, 1024, 8) PNED_SPIN_CASE(5, 1024, 4) PNED_SPIN_CASE(5, 256, 8) PNED_SPIN_CASE(5, 256, 4)
-        PNED_SPIN_CASE(5, 256, 2) PNED_SPIN_CASE(5, 256, 1)
+        // every tile size q^K (q^K <= 8192) maps to one of these
+        PNED_SPIN_CASE(3, 1024, 7) PNED_SPIN_CASE(3, 1024, 3) PNED_SPIN_CASE(3, 256, 3) PNED_SPIN_CASE(3, 256, 1)
+        PNED_SPIN_CASE(4, 1024, 4) PNED_SPIN_CASE(4, 256, 4) PNED_SPIN_CASE(4, 256, 1)
+        PNED_SPIN_CASE(5, 1024, 4) PNED_SPIN_CASE(5, 256, 3) PNED_SPIN_CASE(5, 256, 1)
         { return fail(PNED_E_UNSUPPORTED, "no tiled spin-S kernel instance for this q / tile"); }
 #undef PNED_SPIN_CASE
         PNED_HIP(hipGetLastError());

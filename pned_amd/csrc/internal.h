@@ -238,7 +238,7 @@ struct pned_ctx {
     int spin_tiled = 1;              // tiled kernel (q = 3..5); 0 = per-state gather kernel
     void *spin_pd = nullptr;         // tiled-pass tables
     int spin_K = -1, spin_np = 0;
-    std::vector<int> spin_nb, spin_tsz, spin_nlo, spin_nent;
+    std::vector<int> spin_nb, spin_tsz, spin_nlo, spin_ncls, spin_cplx;
     // scratch for reductions: per-block partials and final results (pinned host mirror)
     double *d_partial = nullptr;
     size_t partial_cap = 0;          // in doubles

@@ -128,6 +128,8 @@ def lib():
                                        C.POINTER(i32), C.POINTER(i32), _ip, _ip, _dp,
                                        np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), _dp, i32, i32,
                                        C.c_void_p]
+    L.pned_spin_plan_export.argtypes = [i32, i32, i32, _ip, _ip, _dp, _ip, i32, i32, C.POINTER(i32),
+                                        C.POINTER(i32), _ip, _ip, _dp, _ip, i32]
     L.pned_s12_layout_choose.argtypes = [i32, i32, _ip, _ip, _dp, _ip, _ip]
     L.pned_ctx_layout.argtypes = [P, _ip]
     L.pned_ctx_set_layout.argtypes = [P, C.c_void_p]
@@ -596,6 +598,33 @@ def s12_plan(op, colbits=0, tile_bits=12, low_bits=4, site_bit=None):
     return {"passes": passes, "tmask": tt[:, 0].astype(np.int64), "selmask": tt[:, 1].astype(np.int64),
             "c0": tt[:, 2] + 1j * tt[:, 3], "c1": tt[:, 4] + 1j * tt[:, 5], "gmask": gmask[:T].astype(np.int64),
             "diag": diag[: 5 * G].reshape(G, 5), "c0diag": diag[5 * G] + 1j * diag[5 * G + 1]}
+
+
+def spin_plan(op, tile_sites=0, low_sites=0):
+    """The tiled spin-S kernel's passes for an operator with q = 3..5 (no GPU needed): per
+    pass its tile sites (lowest first) and off-diagonal classes (bond, digit shift of the
+    min and max site, coefficient by output local state); the pass of each bond; the forced
+    low sites."""
+    mp, qq = 64, op.q * op.q
+    row = 3 + 2 * qq
+    npass, low = C.c_int(), C.c_int()
+    sites = np.zeros(32 * mp, dtype=np.int32)
+    bond_pass = np.full(op.ibond, -1, dtype=np.int32)
+    cls = np.zeros(mp * 64 * row)
+    ncls = np.zeros(mp, dtype=np.int32)
+    check(lib().pned_spin_plan_export(op.n, op.q, op.ibond, np.ascontiguousarray(op.ipair, dtype=np.int32),
+                                      np.ascontiguousarray(op.ibond_type, dtype=np.int32),
+                                      np.ascontiguousarray(op.fortran_xlocal().view(np.float64)),
+                                      op.fortran_llocal(), int(tile_sites), int(low_sites), C.byref(npass),
+                                      C.byref(low), sites, bond_pass, cls, ncls, mp))
+    passes = []
+    for p in range(npass.value):
+        st = [int(x) for x in sites[32 * p: 32 * p + 32] if x >= 0]
+        rows = cls[p * 64 * row: (p * 64 + int(ncls[p])) * row].reshape(-1, row)
+        passes.append({"sites": st, "bonds": [int(b) for b in np.flatnonzero(bond_pass == p)],
+                       "classes": [{"bond": int(r[0]), "da": int(r[1]), "db": int(r[2]),
+                                    "coef": r[3::2] + 1j * r[4::2]} for r in rows]})
+    return {"passes": passes, "low_sites": low.value, "bond_pass": bond_pass}
 
 
 def tps_arrays(alpha, beta, coef, nvec, ncoef):

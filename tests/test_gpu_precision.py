@@ -70,7 +70,7 @@ def _reorth_lanczos_fp64(oop, q1, steps):
     return np.array(a), np.array(b), np.array(Q)
 
 
-@pytest.mark.parametrize("dtype,a_tol,d_tol", [(gpu.C128, 1e-12, 1e-9), (gpu.C64, 1e-5, 1e-4)])
+@pytest.mark.parametrize("dtype,a_tol,d_tol", [(gpu.C128, 1e-12, 1e-9), (gpu.C64, 2e-6, 1e-4)])
 def test_reorthogonalised_dsf(dtype, a_tol, d_tol):
     """S^z(q) continued fraction of the ground state (N=12) with the reorthogonalised
     recursion, against the same recursion in FP64 on the CPU.  The individual coefficients
@@ -108,10 +108,15 @@ def test_reorthogonalised_dsf(dtype, a_tol, d_tol):
     sa, sb = ctx.dsflnc1z_reorth(basis, w, cn, s_maxitr)
     _, dsf, _ = ctx.renbun(sa, sb, e_ref[0], eta, i_range, split, s_maxitr, sq=cn)
     ea = np.max(np.abs(sa[:5] - ra[:5]))
+    # floor for the first coefficients: the same FP64 recursion started from the excitation
+    # rounded to the storage type (each later Lanczos vector is rounded the same way)
+    ex_s = exc.astype(np.complex64 if dtype == gpu.C64 else np.complex128).astype(np.complex128)
+    ra_s, _, _ = _reorth_lanczos_fp64(oop, ex_s / np.linalg.norm(ex_s), 5)
+    floor = np.max(np.abs(ra_s[:5] - ra[:5]))
     ed = np.max(np.abs(dsf - d_r)) / np.max(np.abs(d_r))
     eref = np.max(np.abs(dsf - d_ref)) / np.max(np.abs(d_ref))
     print(f"[result] dtype {dtype}, reorthogonalised DSF ({s_maxitr} steps): vs the FP64 reorthogonalised "
-          f"recursion max |d salpha_1..5| {ea:.2e}, max |d S(q,w)|/max S {ed:.2e}; vs the reference's FP64 "
+          f"recursion max |d salpha_1..5| {ea:.2e} (start rounded to the storage type alone: {floor:.2e}), max |d S(q,w)|/max S {ed:.2e}; vs the reference's FP64 "
           f"recursion without reorthogonalisation max |d S(q,w)|/max S {eref:.2e}", flush=True)
-    assert ea <= a_tol
+    assert ea <= max(a_tol, 20 * floor)
     assert ed <= d_tol
