@@ -748,6 +748,45 @@ bool default_large(const pned_ctx *ctx, int colbits)
     return (double)ctx->dim * (double)(1ull << colbits) > (double)(1ull << 25);
 }
 
+// The lane program of a diagonal (see DiagDev): groups sorted by weight class, so the
+// kernel sums a class's popcounts as integers and converts them once.
+static void build_diag_program(DiagDev &dh)
+{
+    dh.nprog = 0;
+    std::memset(dh.prog_a, 0, sizeof(dh.prog_a));
+    std::memset(dh.prog_b, 0, sizeof(dh.prog_b));
+    if (dh.ngroups < 1 || dh.ngroups > 48) return;
+    std::vector<std::pair<double, double>> cls;
+    std::vector<int> cof(dh.ngroups);
+    for (int g = 0; g < dh.ngroups; ++g) {
+        const std::pair<double, double> w(dh.wr[g], dh.wi[g]);
+        auto it = std::find(cls.begin(), cls.end(), w);
+        cof[g] = (int)(it - cls.begin());
+        if (it == cls.end()) cls.push_back(w);
+        if (dh.shift[g] < 0 || dh.shift[g] > 63) return;
+    }
+    if (cls.size() > 8) return;
+    int e = 0;
+    for (int c = 0; c < (int)cls.size(); ++c) {
+        int last = -1;
+        for (int g = 0; g < dh.ngroups; ++g)
+            if (cof[g] == c) {
+                dh.prog_a[e] = (uint32_t)dh.shift[g] | (dh.kind[g] != 0 ? 64u : 0u) | ((uint32_t)c << 8);
+                dh.prog_b[e] = dh.mask[g];
+                last = e++;
+            }
+        dh.prog_a[last] |= 128u;
+        uint64_t wr, wi;
+        std::memcpy(&wr, &cls[c].first, 8);
+        std::memcpy(&wi, &cls[c].second, 8);
+        dh.prog_a[48 + c] = (uint32_t)wr;
+        dh.prog_a[56 + c] = (uint32_t)(wr >> 32);
+        dh.prog_b[48 + c] = (uint32_t)wi;
+        dh.prog_b[56 + c] = (uint32_t)(wi >> 32);
+    }
+    dh.nprog = e;
+}
+
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
 {
     if (ctx->plan_colbits == colbits && ctx->plan_dtype == dtype) return PNED_OK;
@@ -792,6 +831,7 @@ int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
     }
     dh.c0r = pl.c0r;
     dh.c0i = pl.c0i;
+    build_diag_program(dh);
     if (!ctx->d_diag) PNED_HIP(hipMalloc((void **)&ctx->d_diag, sizeof(DiagDev)));
     PNED_HIP(hipMemcpy(ctx->d_diag, &dh, sizeof(DiagDev), hipMemcpyHostToDevice));
     ctx->plan_colbits = colbits;
@@ -1030,6 +1070,7 @@ static int refit_plan(pned_ctx *ctx, const double *xl, const int32_t *ll)
     }
     dh.c0r = pl.c0r;
     dh.c0i = pl.c0i;
+    build_diag_program(dh);
     PNED_HIP(hipMemcpy(ctx->d_diag, &dh, sizeof(DiagDev), hipMemcpyHostToDevice));
     ctx->pd_threads = -1;   // the term-major table carries the coefficients
     return PNED_OK;

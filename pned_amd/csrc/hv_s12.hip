@@ -360,6 +360,12 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
         lai = (double)colsc * a.lz_prd[2 * (tid & (a.ncol - 1)) + 1];
     }
 
+    // the diagonal as a lane program (see DiagDev): group e in lane e, class weights in
+    // lanes 48..63; one operator only (batch contexts weight per column)
+    const int dprog = (a.diag && a.nop <= 1) ? diagp->nprog : 0;
+    const uint32_t dpa = dprog ? diagp->prog_a[tid & 63] : 0u;
+    const uint32_t dpb = dprog ? diagp->prog_b[tid & 63] : 0u;
+
     double pr_re = 0.0, pr_im = 0.0;
     Real xr[R], xi[R];
     auto fetch_x = [&](uint64_t b) {
@@ -401,7 +407,48 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
                 if (rd0) fetch_y(tb0, r0, ycr, yci);
             }
             A ar[S], ai[S];
-            if (a.diag) {
+            if (dprog) {
+                // all S slots at once, group-outer: per group two readlanes, per slot a shift,
+                // an xor and an and with a popcount; per weight class one convert and FMA pair
+                uint32_t st[S], cnt[S];
+                double dr[S], di[S];
+                const double d0r = diagp->c0r + a.shift, d0i = diagp->c0i;
+                static_for<S>([&](auto I) {
+                    st[I] = (uint32_t)((tb0 | pd->hi[r0 + I]) >> a.colbits);
+                    cnt[I] = 0u;
+                    dr[I] = d0r;
+                    di[I] = d0i;
+                });
+                for (int e = 0; e < dprog; ++e) {
+                    const uint32_t pa = (uint32_t)__builtin_amdgcn_readlane((int)dpa, e);
+                    const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)dpb, e);
+                    const uint32_t sh = pa & 63u;
+                    if (pa & 64u) {
+                        static_for<S>([&](auto I) { cnt[I] += (uint32_t)__popc((st[I] ^ (st[I] >> sh)) & m); });
+                    } else {
+                        static_for<S>([&](auto I) { cnt[I] += (uint32_t)__popc(st[I] & m); });
+                    }
+                    if (pa & 128u) {
+                        const int c = (int)((pa >> 8) & 7u);
+                        const uint64_t wrb = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)dpa, 48 + c) |
+                                             ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)dpa, 56 + c) << 32);
+                        const uint64_t wib = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)dpb, 48 + c) |
+                                             ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)dpb, 56 + c) << 32);
+                        const double wr = __longlong_as_double((long long)wrb);
+                        const double wi = __longlong_as_double((long long)wib);
+                        static_for<S>([&](auto I) {
+                            dr[I] = fma((double)cnt[I], wr, dr[I]);
+                            di[I] = fma((double)cnt[I], wi, di[I]);
+                            cnt[I] = 0u;
+                        });
+                    }
+                }
+                static_for<S>([&](auto I) {
+                    const V x = tile[tid + TPB * (r0 + I)];
+                    ar[I] = (A)(dr[I] * (double)x.x - di[I] * (double)x.y);
+                    ai[I] = (A)(dr[I] * (double)x.y + di[I] * (double)x.x);
+                });
+            } else if (a.diag) {
                 const DiagDev &dg = *diagp;
                 static_for<S>([&](auto I) {
                     const uint32_t st = (uint32_t)((tb0 | pd->hi[r0 + I]) >> a.colbits);

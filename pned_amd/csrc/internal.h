@@ -166,6 +166,14 @@ struct DiagDev {             // flattened diagonal for the kernel (constant per 
     uint32_t mask[48];
     double wr[48], wi[48];
     double c0r, c0i;
+    // The same groups as a lane program for the term-major kernel, read with readlane so the
+    // per-state evaluation waits on no memory.  Lane e < nprog of prog_a: shift (bits 0-5),
+    // pair parity (bit 6), last group of its weight class (bit 7), class (bits 8-10); of
+    // prog_b: the mask.  Groups are ordered by class.  Lanes 48 + c and 56 + c of prog_a hold
+    // the low and high word of class c's real weight, of prog_b of its imaginary weight.
+    // nprog = 0: not available (more than 48 groups or 8 weight classes).
+    int nprog;
+    uint32_t prog_a[64], prog_b[64];
 };
 
 // ------------------------------------------------------------------------------------
