@@ -497,13 +497,48 @@ struct ObsArgs {
     int sq_mode[OBS_MAX_SQ];              // 0: zz weight, 1: pp weight
     int sq_left[OBS_MAX_SQ];              // 0: <v1|, 1: <vg|
     int sq_q[OBS_MAX_SQ];                 // momentum slot
-    double phr[8][32], phi[8][32];        // e^{i q.r_j} per momentum slot
+    double phr[8][32], phi[8][32];        // e^{i q.r_j} per momentum slot (entry k uses slot k / 4)
     int8_t pos[32];                       // memory bit of site j
-    // structure-factor weights w(b_j, b_0) (re, im) by mode (0: zz, 1: pp) of the two bits:
-    // S=1/2 tree (b - 1/2)(b0 - 1/2) and [b != b0]/4; spin-S tree (b - S)(b0 - S) and the
-    // 2x2 corner of sx + sy (S_arbitrary/Hybrid_TPS9_TPS.f90:175-177, 224-243)
-    double wtab[2][2][2][2];
+    // Entry k's weight of a state, W_k(s) = sum_j w_k(s_j, s_0) e^{i q.r_j}, with w_k(b, b0)
+    // by mode (zz or pp) of the two bits: S=1/2 tree (b - 1/2)(b0 - 1/2) and [b != b0]/4;
+    // spin-S tree (b - S)(b0 - S) and the 2x2 corner of sx + sy
+    // (S_arbitrary/Hybrid_TPS9_TPS.f90:175-177, 224-243).  As w_k depends on site j only
+    // through its bit, W_k(s) = A_k(s_0) + D_k(s_0) Q(s) with Q(s) the sum of the phases of the
+    // sites whose bit is set: sa[k][b0] = w_k(0, b0) sum_j e^{i q.r_j}, sd[k][b0] =
+    // w_k(1, b0) - w_k(0, b0), each (re, im).
+    double sa[OBS_MAX_SQ][2][2], sd[OBS_MAX_SQ][2][2];
 };
+
+// W_k(s) of the entries k < nsq (at most MAXK) of one state, handed to emit(k, re, im)
+template <int MAXK, typename F>
+__device__ __forceinline__ void sq_weights(const ObsArgs &a, uint32_t s, int nsq, F &&emit)
+{
+    constexpr int NSLOT = (MAXK + 3) / 4;
+    const int nslot = (nsq + 3) / 4;
+    double qr[NSLOT], qi[NSLOT];
+#pragma unroll
+    for (int m = 0; m < NSLOT; ++m) {
+        qr[m] = 0.0;
+        qi[m] = 0.0;
+    }
+    for (int j = 0; j < a.n; ++j) {
+        const double sj = (double)((s >> a.pos[j]) & 1u);
+#pragma unroll
+        for (int m = 0; m < NSLOT; ++m) {
+            if (m >= nslot) break;
+            qr[m] = fma(sj, a.phr[m][j], qr[m]);
+            qi[m] = fma(sj, a.phi[m][j], qi[m]);
+        }
+    }
+    const bool b0 = (s >> a.pos[0]) & 1u;
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k) {
+        if (k >= nsq) break;
+        const double ar = b0 ? a.sa[k][1][0] : a.sa[k][0][0], ai = b0 ? a.sa[k][1][1] : a.sa[k][0][1];
+        const double dr = b0 ? a.sd[k][1][0] : a.sd[k][0][0], di = b0 ? a.sd[k][1][1] : a.sd[k][0][1];
+        emit(k, fma(dr, qr[k / 4], fma(-di, qi[k / 4], ar)), fma(dr, qi[k / 4], fma(di, qr[k / 4], ai)));
+    }
+}
 
 template <typename Real, int NOUT>
 __global__ __launch_bounds__(RED_TPB) void k_tps_obs(const typename V2<Real>::T *__restrict__ vg,
@@ -535,23 +570,15 @@ __global__ __launch_bounds__(RED_TPB) void k_tps_obs(const typename V2<Real>::T 
         acc[2] = fma(wzz, n1, acc[2]);
         acc[4] = fma(wzz, gr * xr + gi * xi, acc[4]);
         acc[5] = fma(wzz, gr * xi - gi * xr, acc[5]);
-        const int s0 = (int)((s >> a.pos[0]) & 1u);
-        for (int k = 0; k < a.nsq && 6 + 2 * k + 1 < NOUT; ++k) {
-            const int qs = a.sq_q[k];
-            double cr = 0.0, ci = 0.0;
-            for (int j = 0; j < a.n; ++j) {
-                const int sj = (int)((s >> a.pos[j]) & 1u);
-                const double *w = a.wtab[a.sq_mode[k]][sj][s0];
-                cr = fma(w[0], a.phr[qs][j], fma(-w[1], a.phi[qs][j], cr));
-                ci = fma(w[0], a.phi[qs][j], fma(w[1], a.phr[qs][j], ci));
-            }
+        constexpr int MAXK = (NOUT - 6) / 2;
+        sq_weights<MAXK>(a, s, a.nsq < MAXK ? a.nsq : MAXK, [&](int k, double cr, double ci) {
             // W x
             const double wr = cr * xr - ci * xi, wi = cr * xi + ci * xr;
             double lr = xr, li = xi;
             if (a.sq_left[k]) { lr = gr; li = gi; }
             acc[6 + 2 * k] = fma(lr, wr, fma(li, wi, acc[6 + 2 * k]));
             acc[6 + 2 * k + 1] = fma(lr, wi, fma(-li, wr, acc[6 + 2 * k + 1]));
-        }
+        });
     }
     (void)nout;
     // per-column block reduction of NOUT values
@@ -605,31 +632,20 @@ __global__ __launch_bounds__(RED_TPB) void k_tps_fused(const typename V2<Real>::
     for (size_t ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
         const size_t s0c = ch * (size_t)cs;
         const int ns = (int)((nstates - s0c < (size_t)cs) ? (nstates - s0c) : (size_t)cs);
-        // phase A: per-state weights
-        for (int task = tid; task < ns * (1 + nsq); task += RED_TPB) {
-            const int sl = task % ns, kind = task / ns;
+        // phase A: per-state weights, one state per thread
+        for (int sl = tid; sl < ns; sl += RED_TPB) {
             const uint32_t s = (uint32_t)(s0c + sl);
-            if (kind == 0) {
-                double wzz = 0.0;
-                for (int b = 0; b < a.nzz; ++b) {
-                    const double s3 = (s & a.zz_mask1[b]) ? 0.5 : -0.5;
-                    const double s4 = (s & a.zz_mask2[b]) ? 0.5 : -0.5;
-                    wzz = fma(s3, s4, wzz);
-                }
-                wt[sl * nw] = wzz;
-            } else {
-                const int k = kind - 1, qs = a.sq_q[k];
-                const int s0 = (int)((s >> a.pos[0]) & 1u);
-                double cr = 0.0, ci = 0.0;
-                for (int j = 0; j < a.n; ++j) {
-                    const int sj = (int)((s >> a.pos[j]) & 1u);
-                    const double *wj = a.wtab[a.sq_mode[k]][sj][s0];
-                    cr = fma(wj[0], a.phr[qs][j], fma(-wj[1], a.phi[qs][j], cr));
-                    ci = fma(wj[0], a.phi[qs][j], fma(wj[1], a.phr[qs][j], ci));
-                }
+            double wzz = 0.0;
+            for (int b = 0; b < a.nzz; ++b) {
+                const double s3 = (s & a.zz_mask1[b]) ? 0.5 : -0.5;
+                const double s4 = (s & a.zz_mask2[b]) ? 0.5 : -0.5;
+                wzz = fma(s3, s4, wzz);
+            }
+            wt[sl * nw] = wzz;
+            sq_weights<(NOUT - 10) / 2>(a, s, nsq, [&](int k, double cr, double ci) {
                 wt[sl * nw + 1 + 2 * k] = cr;
                 wt[sl * nw + 2 + 2 * k] = ci;
-            }
+            });
         }
         __syncthreads();
         // phase B: the chunk's elements
@@ -1227,23 +1243,25 @@ static int build_obs_args(pned_ctx *ctx, ObsArgs &a)
     a.n = ctx->n;
     a.nzz = 0;
     for (int j = 0; j < 32; ++j) a.pos[j] = ctx->lay.pos[j];
+    // w[mode][b][b0] (re, im): the two-bit weights of the structure-factor modes
+    double wtab[2][2][2][2];
     const double ss = 0.5 * (ctx->q - 1);
     for (int b = 0; b < 2; ++b)
         for (int b0 = 0; b0 < 2; ++b0) {
-            a.wtab[0][b][b0][0] = ((double)b - ss) * ((double)b0 - ss);
-            a.wtab[0][b][b0][1] = 0.0;
-            a.wtab[1][b][b0][0] = a.wtab[1][b][b0][1] = 0.0;
+            wtab[0][b][b0][0] = ((double)b - ss) * ((double)b0 - ss);
+            wtab[0][b][b0][1] = 0.0;
+            wtab[1][b][b0][0] = wtab[1][b][b0][1] = 0.0;
         }
     if (ctx->q == 2) {
-        a.wtab[1][0][1][0] = a.wtab[1][1][0][0] = 0.25;   // my_xor, Hybrid_TPS_TPS.f90:214-215
+        wtab[1][0][1][0] = wtab[1][1][0][0] = 0.25;   // my_xor, Hybrid_TPS_TPS.f90:214-215
     } else {
         // my_xor = sx + sy with sx = (su + sd)/2, sy = -i (su - sd)/2, su(0,1) and sd(1,0)
         // the ladder factors of m = -S (Hybrid_TPS9_params.f90:1479-1495)
         const double su01 = std::sqrt((ss + ss) * (ss - ss + 1.0)), sd10 = std::sqrt((ss + 1.0 - ss) * (ss + ss));
-        a.wtab[1][0][1][0] = 0.5 * su01;
-        a.wtab[1][0][1][1] = -0.5 * su01;
-        a.wtab[1][1][0][0] = 0.5 * sd10;
-        a.wtab[1][1][0][1] = 0.5 * sd10;
+        wtab[1][0][1][0] = 0.5 * su01;
+        wtab[1][0][1][1] = -0.5 * su01;
+        wtab[1][1][0][0] = 0.5 * sd10;
+        wtab[1][1][0][1] = 0.5 * sd10;
     }
     for (int b = 0; b < ctx->ibond; ++b) {
         if (ctx->btype[b] != 3) continue;
@@ -1268,12 +1286,25 @@ static int build_obs_args(pned_ctx *ctx, ObsArgs &a)
         if (qi + 1 > nslots) nslots = qi + 1;
     }
     if (nslots > 8) return fail(PNED_E_UNSUPPORTED, "too many momentum slots");
+    double p0[8][2] = {};
     for (int s = 0; s < nslots; ++s)
         for (int j = 0; j < ctx->n; ++j) {
             const double qn = ctx->sq_q[2 * s] * ctx->site_vec[2 * j] + ctx->sq_q[2 * s + 1] * ctx->site_vec[2 * j + 1];
             a.phr[s][j] = std::cos(qn);
             a.phi[s][j] = std::sin(qn);
+            p0[s][0] += a.phr[s][j];
+            p0[s][1] += a.phi[s][j];
         }
+    for (int k = 0; k < nsq; ++k) {
+        const int md = a.sq_mode[k], q = a.sq_q[k];
+        for (int b0 = 0; b0 < 2; ++b0) {
+            const double *w0 = wtab[md][0][b0], *w1 = wtab[md][1][b0];
+            a.sa[k][b0][0] = w0[0] * p0[q][0] - w0[1] * p0[q][1];
+            a.sa[k][b0][1] = w0[0] * p0[q][1] + w0[1] * p0[q][0];
+            a.sd[k][b0][0] = w1[0] - w0[0];
+            a.sd[k][b0][1] = w1[1] - w0[1];
+        }
+    }
     return PNED_OK;
 }
 

@@ -551,6 +551,45 @@ def test_tps_fused_matches_oracle(dtype, tol):
         assert np.allclose(co[:, c, :], c_ref, rtol=max(tol, 1e-9), atol=1e-9 if dtype == gpu.C128 else 1e-4), c
 
 
+def test_tps_fused_c64_n24_r64():
+    """The batched form at its configured size: 3x4 torus (N=24), R=64 samples, mTPQ steps of
+    (l - H/N) with the bond-norm shift.  The fused complex64 run against the fused complex128
+    run on the same 64 start vectors (every column's E/N at every step within 1e-6), and the
+    first steps of two columns of each against the oracle's mTPQ restatement of the same start
+    vector (the oracle takes about 5 s per step at this size)."""
+    _say("tps fused c64 N=24 R=64")
+    cl = lattice.torus(3, 4)
+    lb = model.bond_norm_shift(cl, 20, 90.0)
+    op = model.build_operator(cl, 20, 90.0, tps=True, tps_lb=lb)
+    ctx = gpu.Context(op)
+    ctx.tps_set_observables(cl.site_vec, cl.sq_momenta())
+    R, k = 64, 20
+    seeds = [20261202 + 7919 * c for c in range(R)]
+    runs = {}
+    for dtype in (gpu.C64, gpu.C128):
+        v1, va, vb = (ctx.vector(R, dtype) for _ in range(3))
+        for c in range(R):
+            ctx.set_initial_column(v1, c, seeds[c])
+        runs[dtype] = ctx.tps_fused(v1, va, vb, op.shift_L, k)
+        for v in (v1, va, vb):
+            v.free()
+    a64, a128 = runs[gpu.C64][0], runs[gpu.C128][0]
+    e_err = np.max(np.abs(a64 - a128)) / cl.n
+    print(f"[result] N=24 R=64 fused complex64 vs complex128: max |dE/N| {e_err:.2e} over {k} steps", flush=True)
+    assert e_err <= 1e-6
+    oop = oracle.OracleOp(op)
+    ko = 3
+    for c in (0, 37):
+        a_ref, b_ref, c_ref = oracle.tps(oop, cl.site_vec, cl.sq_momenta(), op.shift_L, ko,
+                                         oracle.gaussian(op.dim, seeds[c]))
+        for dtype, tol in ((gpu.C128, 1e-10), (gpu.C64, 2e-4)):
+            al, be, co = runs[dtype]
+            assert np.allclose(al[:ko, c], a_ref, rtol=tol), (dtype, c)
+            assert np.allclose(be[:ko, c], b_ref, rtol=tol), (dtype, c)
+            assert np.allclose(co[:ko, c, :], c_ref, rtol=max(tol, 1e-9),
+                               atol=1e-9 if dtype == gpu.C128 else 1e-4), (dtype, c)
+
+
 def test_spin1_tiled_vs_gather_n18():
     """cfg4 size (3^18 states): the tiled spin-S kernel against the per-state gather kernel
     (both pinned to the oracle at N=8, 12 by test_hv_parity)."""
