@@ -168,6 +168,39 @@ def test_lanczos_parity_n16():
     assert np.allclose(beta[:20], b2[:20], rtol=1e-12, atol=0)
 
 
+def test_lanczos_n24_300_steps():
+    """The fixed-step Lanczos workload at its size: the reference 24-site cluster (2^24
+    states), phi = 54 deg, 300 fixed steps.  The first 30 alpha/beta against the oracle's
+    Lanczos from the same start vector, and the Ritz vector of the lowest Ritz value of the
+    300 steps: residual |Hx - E0 x| / |x| at round-off and Rayleigh quotient equal to E0."""
+    _say("lanczos n24 300 steps")
+    op = model.build_operator(lattice.reference_cluster(24), 20, 54.0)
+    ctx = gpu.Context(op)
+    v1, v0, x = ctx.vector(), ctx.vector(), ctx.vector()
+    seed = 20261202
+    ctx.set_initial_vec(v1, seed)
+    xs = v1.download()
+    itr, alpha, beta, ene = ctx.lnc1(v1, v0, fixed_steps=300, nvec=1)
+    assert itr == 300
+    _, a2, b2, _ = oracle.lanczos(oracle.OracleOp(op), xs, fixed_steps=30, nvec=1)
+    # alpha_1 is ~4e-4 here: bound by the coefficients' scale, not each one's size
+    scale = np.max(np.abs(np.concatenate([a2[:30], b2[:30]])))
+    assert np.allclose(alpha[:30], a2[:30], rtol=1e-11, atol=1e-12 * scale)
+    assert np.allclose(beta[:30], b2[:30], rtol=1e-11, atol=1e-12 * scale)
+    coef = gpu.vec12(alpha, beta, [ene[0]], 1)[0]
+    ctx.set_initial_vec(v1, seed)
+    ctx.ritz_vector(v1, v0, x, alpha, beta, coef)
+    res = ctx.residual(x, ene[0], v0)
+    X = x.download()
+    HX = np.zeros_like(X)
+    oracle.hv(oracle.OracleOp(op), X, HX)
+    rq = np.vdot(X, HX).real / np.vdot(X, X).real
+    print(f"[result] N=24 300-step Lanczos: E0/N {ene[0] / 24:.13f}, Ritz residual {res:.2e}, "
+          f"oracle Rayleigh quotient rel diff {abs(rq - ene[0]) / abs(ene[0]):.2e}", flush=True)
+    assert res <= 1e-10
+    assert abs(rq - ene[0]) <= 1e-12 * abs(ene[0])
+
+
 @pytest.mark.parametrize("spec,phi", [(8, 108.0), (12, 108.0)])
 def test_lanczos_parity_spin1(spec, phi):
     """Spin-1 Lanczos (tiled spin-S kernel, deferred normalisation) against the oracle."""

@@ -148,3 +148,32 @@ def test_sequential_server_matches_oracle():
         x = oracle.gaussian(op.dim, q.seed)
         itr, a, b, ene = oracle.lanczos(oracle.OracleOp(op), x, max_itr=1500, nvec=1)
         assert abs(r.e0 - ene[0]) <= 1e-10 * abs(ene[0]), (q, r.e0, ene[0])
+
+
+def test_server_large_sizes_match_plain_lanczos():
+    """The sizes that carry the serving work (3x4 N=24 and 2x7 N=28, one slot per pool) in a
+    mix with small ones: every served ground state has residual < 1e-10, and its E0 equals
+    the plain Lanczos (lnc1, pinned to the oracle at N=16 and N=24) from the same start
+    vector to 1e-10."""
+    _say("server with N=24 and N=28 requests")
+    reqs = serving.make_requests(3, ["3x4"], seed=17) + serving.make_requests(1, ["2x7"], seed=19) + \
+        serving.make_requests(4, ["2x4", "3x3"], seed=23)
+    for i, q in enumerate(reqs):
+        q.rid = i
+    srv = serving.Server(hbm_budget_bytes=120e9, tol=1e-10)
+    res = srv.run(reqs)
+    assert len(res) == len(reqs)
+    got = {r.rid: r for r in res}
+    for q in reqs:
+        r = got[q.rid]
+        assert r.resid < 1e-10, (q, r.resid)
+        op = serving.operator_for(q)
+        ctx = gpu.Context(op)
+        v1, v0 = ctx.vector(), ctx.vector()
+        ctx.set_initial_vec(v1, q.seed)
+        itr, a, b, ene = ctx.lnc1(v1, v0, max_itr=1500, nvec=1)
+        _say(f"{q.spec}: served E0 {r.e0:.13f}, plain lnc1 {ene[0]:.13f}")
+        assert abs(r.e0 - ene[0]) <= 1e-10 * abs(ene[0]), (q, r.e0, ene[0])
+        for v in (v1, v0):
+            v.free()
+        ctx.close()
