@@ -382,9 +382,14 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
             yi[I] = t.y;
         });
     };
-    uint64_t tix = blockIdx.x;
-    if (tix < a.tile_count) fetch_x(outer_base(tix) | offLo);
-    for (; tix < a.tile_count; tix += gridDim.x) {
+    // each block takes a contiguous range of tiles: consecutive tiles of a block differ in the
+    // lowest outer bits, so its runs follow each other in memory (measured at N=32: a few per
+    // cent on the passes whose tiles hold few low memory bits, against tiles strided by the grid)
+    const uint64_t per = (a.tile_count + gridDim.x - 1) / gridDim.x;
+    uint64_t tix = (uint64_t)blockIdx.x * per;
+    const uint64_t tend = (tix + per < a.tile_count) ? tix + per : a.tile_count;
+    if (tix < tend) fetch_x(outer_base(tix) | offLo);
+    for (; tix < tend; ++tix) {
         const uint64_t ob = outer_base(tix);
         const uint64_t tb0 = ob | offLo;
         static_for<R>([&](auto I) {
@@ -398,8 +403,8 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
         // not wait for the prefetch (vector memory counters retire in order)
         Real ycr[S], yci[S];
         if (rd0) fetch_y(tb0, 0, ycr, yci);
-        const uint64_t nx = tix + gridDim.x;
-        if (nx < a.tile_count) fetch_x(outer_base(nx) | offLo);
+        const uint64_t nx = tix + 1;
+        if (nx < tend) fetch_x(outer_base(nx) | offLo);
 
         static_for<NS>([&](auto HI) {
             constexpr int r0 = HI * S;
