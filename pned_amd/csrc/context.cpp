@@ -428,6 +428,67 @@ static bool random_cover(const std::vector<uint64_t> &um, int nb, int L, int K, 
     return false;
 }
 
+// Memory locality of a cover: a tile that holds at least two of the memory-index bits
+// colbits+3 .. colbits+7 reads at least 512 contiguous-ish bytes of every 4 KB it touches;
+// tiles without such bits read one 128-byte run per 4 KB and stream slower (measured at
+// N=32: 46 ms per pass against 39-40 ms).  Score = sum over tiles of min(2, such bits).
+static int locality_score(const std::vector<uint64_t> &tiles, int colbits)
+{
+    const uint64_t low = 0x1Full << (colbits + 3);
+    int s = 0;
+    for (uint64_t t : tiles) s += std::min(2, __builtin_popcountll(t & low));
+    return s;
+}
+
+// Anneals a complete cover towards a higher locality score: moves exchange one free bit of
+// one tile; the cost is 10 per uncovered mask minus the score (so the walk may pass through
+// incomplete covers), and the best complete cover seen is kept.
+static void refine_locality(const std::vector<uint64_t> &masks, int nb, int L, int colbits,
+                            std::vector<uint64_t> &tiles, Lcg &rng, int iters)
+{
+    const int P = (int)tiles.size();
+    const size_t nm = masks.size();
+    std::vector<int> cov(nm, 0);
+    for (size_t i = 0; i < nm; ++i)
+        for (int p = 0; p < P; ++p) cov[i] += ((masks[i] & ~tiles[p]) == 0);
+    int unc = 0;
+    for (size_t i = 0; i < nm; ++i) unc += cov[i] == 0;
+    int score = locality_score(tiles, colbits), best = unc == 0 ? score : -1;
+    std::vector<uint64_t> best_tiles = tiles;
+    std::vector<int> in_bits, out_bits;
+    for (int it = 0; it < iters && best < 2 * P; ++it) {
+        const double temp = std::max(0.05, 2.0 * std::pow(0.9997, (double)it));
+        const int p = (int)(rng.next() % (uint32_t)P);
+        const uint64_t T = tiles[p];
+        in_bits.clear();
+        out_bits.clear();
+        for (int b = L; b < nb; ++b) (T >> b & 1 ? in_bits : out_bits).push_back(b);
+        if (in_bits.empty() || out_bits.empty()) break;
+        const int a = in_bits[rng.next() % in_bits.size()];
+        const int c = out_bits[rng.next() % out_bits.size()];
+        const uint64_t T2 = (T & ~(1ull << a)) | (1ull << c);
+        int du = 0;
+        for (size_t i = 0; i < nm; ++i) {
+            const int was = (masks[i] & ~T) == 0, now = (masks[i] & ~T2) == 0;
+            if (was == now) continue;
+            du += ((cov[i] - was + now) == 0) - (cov[i] == 0);
+        }
+        tiles[p] = T2;
+        const int sc = locality_score(tiles, colbits);
+        const double dcost = 10.0 * du - (sc - score);
+        const double u = (double)(rng.next() & 0xFFFFFF) / 16777216.0;
+        if (dcost <= 0 || u < std::exp(-dcost / temp)) {
+            for (size_t i = 0; i < nm; ++i) cov[i] += ((masks[i] & ~T2) == 0) - ((masks[i] & ~T) == 0);
+            unc += du;
+            score = sc;
+            if (unc == 0 && score > best) { best = score; best_tiles = tiles; }
+        } else {
+            tiles[p] = T;
+        }
+    }
+    tiles = best_tiles;
+}
+
 static std::vector<uint64_t> unique_masks(const S12Plan &plan, int colbits)
 {
     std::vector<uint64_t> um;
@@ -439,6 +500,8 @@ static std::vector<uint64_t> unique_masks(const S12Plan &plan, int colbits)
 
 constexpr uint64_t RANDOM_COVER_SEED = 0x5EEDull;
 constexpr int RANDOM_COVER_TRIES = 2, RANDOM_COVER_ITERS = 30000;
+constexpr uint64_t LOCALITY_SEED = 0x10CA1ull;
+constexpr int LOCALITY_TRIES = 8, LOCALITY_ITERS = 40000;
 
 int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
 {
@@ -496,6 +559,17 @@ int s12_make_passes(S12Plan &plan, int colbits, int tile_bits, int low_bits)
                 break;
             best = found;
         }
+        // with the pass count fixed, give the tiles low memory bits (a few restarts)
+        std::vector<uint64_t> loc = best;
+        int loc_score = locality_score(best, colbits);
+        for (int t = 0; t < LOCALITY_TRIES && loc_score < 2 * (int)best.size(); ++t) {
+            std::vector<uint64_t> trial = best;
+            Lcg lr{LOCALITY_SEED + 0x9E3779B97F4A7C15ull * (uint64_t)(t + 1)};
+            refine_locality(um, nb, L, colbits, trial, lr, LOCALITY_ITERS);
+            const int sc = locality_score(trial, colbits);
+            if (sc > loc_score) { loc_score = sc; loc = trial; }
+        }
+        best = loc;
     }
     // Each term goes to the first tile that contains it; the diagonal goes to the pass with
     // the least term work (1 per real term, 2 per complex term: twice the FMAs), which then
