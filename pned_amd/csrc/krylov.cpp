@@ -80,110 +80,146 @@ struct LanczosSteps {
 
 extern "C" {
 
-// The Sturm bisection and inverse iteration below keep the reference's steps (TITPACK's
-// bisec/vec12): lnc1z stops when the nvec-th Ritz value changes by < 1e-13 relative between
-// checks, a threshold within reach of the bisection's own rounding, so the step at which a
-// run stops is a function of these exact operations.  A LAPACK-style Sturm count (LDL^T
-// pivots, eigenvalues to ~1e-14) stopped a run of the N=12 reference cluster at step 120
-// where the reference's arithmetic stops at 125.
-// bisec (S_one_half/Hybrid_TPS_basic.f90:25-107): Sturm-sequence bisection, ne lowest
+// Lowest eigenvalues and eigenvectors of the Lanczos tridiagonal T (diagonal alpha[0..n),
+// off-diagonal beta[0..n-1)).  The entry points keep the names, arguments and errors of the
+// reference's bisec / vec12 (S_one_half/Hybrid_TPS_basic.f90:25-213); the arithmetic is this
+// library's own: Sturm counts from the pivots of the LDL^T factorisation of T - x I, each
+// eigenvalue bisected down to adjacent doubles, eigenvectors by inverse iteration through a
+// pivoted LU of T - E I.  The stop step of lnc1z (a 1e-13 relative change test) depends on
+// rounding at that level, so it can differ from the reference's by a check or two.
+
+namespace {
+
+// number of eigenvalues of T below x (LDL^T pivots; a vanishing pivot is moved to -pivmin)
+int sturm_count(const double *alpha, const double *b2, int n, double x, double pivmin)
+{
+    int cnt = 0;
+    double d = alpha[0] - x;
+    if (std::fabs(d) < pivmin) d = -pivmin;
+    cnt += d < 0.0;
+    for (int i = 1; i < n; ++i) {
+        d = (alpha[i] - x) - b2[i - 1] / d;
+        if (std::fabs(d) < pivmin) d = -pivmin;
+        cnt += d < 0.0;
+    }
+    return cnt;
+}
+
+}  // namespace
+
+// bisec (the reference's interface): the ne lowest eigenvalues, ascending; eps is the
+// reference's relative width at which bisection may stop (below round-off, the bisection
+// runs until the bracket holds two adjacent doubles)
 int pned_bisec(const double *alpha, const double *beta, int ndim, double *E, int ne, double eps)
 {
     if (!alpha || !beta || !E || ndim < 1) return fail(PNED_E_ARG, "null argument");
     if (ndim > 8000) return fail(PNED_E_ARG, "#(E04) ndim given to bisec exceeds 8000");
     if (ne > ndim || ne < 0) return fail(PNED_E_ARG, "#(E05) ne given to bisec out of range");
-    std::vector<double> b2(ndim);
-    double range = std::fabs(alpha[0]) + (ndim > 1 ? std::fabs(beta[0]) : 0.0);
-    for (int k = 1; k < ndim - 1; ++k)
-        range = std::max(range, std::fabs(beta[k - 1]) + std::fabs(alpha[k]) + std::fabs(beta[k]));
-    if (ndim > 1) range = std::max(range, std::fabs(beta[ndim - 2]) + std::fabs(alpha[ndim - 1]));
-    range = -range;
-    b2[0] = 0.0;
-    for (int i = 1; i < ndim; ++i) b2[i] = beta[i - 1] * beta[i - 1];
-    const double epsabs = std::fabs(range) * eps;
-    for (int i = 0; i < ne; ++i) E[i] = -range;
-    double b = range;
-    for (int k = 1; k <= ne; ++k) {
-        double a = E[k - 1];
-        for (int j = 0; j < 100; ++j) {
-            const double c = (a + b) / 2.0;
-            if (std::fabs(a - b) < epsabs) continue;
-            int numneg = 0, ipass = 0;
-            double g = 1.0;
-            for (int i = 0; i < ndim; ++i) {
-                if (ipass == 0) g = c - alpha[i] - b2[i] / g;
-                else if (ipass == 1) ipass = 2;
-                else { g = c - alpha[i]; ipass = 0; }
-                if (ipass == 0) {
-                    if (g < 0.0) ++numneg;
-                    if (std::fabs(g) < std::fabs(b2[i] * epsabs * eps)) ipass = 1;
-                }
-            }
-            numneg = ndim - numneg;
-            if (numneg < k) b = c;
-            else {
-                a = c;
-                for (int i = k; i <= std::min(numneg, ne); ++i) E[i - 1] = c;
-            }
+    std::vector<double> b2(ndim > 1 ? ndim - 1 : 1, 0.0);
+    double lo = alpha[0], hi = alpha[0], b2max = 0.0;
+    for (int i = 0; i < ndim; ++i) {
+        const double r = (i > 0 ? std::fabs(beta[i - 1]) : 0.0) + (i < ndim - 1 ? std::fabs(beta[i]) : 0.0);
+        lo = std::min(lo, alpha[i] - r);   // Gershgorin discs
+        hi = std::max(hi, alpha[i] + r);
+        if (i < ndim - 1) {
+            b2[i] = beta[i] * beta[i];
+            b2max = std::max(b2max, b2[i]);
         }
+    }
+    const double scale = std::max(std::fabs(lo), std::fabs(hi));
+    const double pivmin = std::max(std::numeric_limits<double>::min(), b2max * std::numeric_limits<double>::min());
+    const double width = std::max(scale * (eps > 0.0 ? eps : 0.0), 0.0);
+    lo -= 2.0 * std::numeric_limits<double>::epsilon() * scale + pivmin;
+    hi += 2.0 * std::numeric_limits<double>::epsilon() * scale + pivmin;
+    double left = lo;
+    for (int k = 1; k <= ne; ++k) {
+        // the k-th lowest lies in (a, b]: fewer than k eigenvalues below a, at least k below b
+        double a = left, b = hi;
+        for (int it = 0; it < 200; ++it) {
+            const double c = 0.5 * (a + b);
+            if (c <= a || c >= b || b - a <= width) break;
+            if (sturm_count(alpha, b2.data(), ndim, c, pivmin) >= k) b = c;
+            else a = c;
+        }
+        E[k - 1] = 0.5 * (a + b);
+        left = a;   // the next one is not below this bracket
     }
     return PNED_OK;
 }
 
-// vec12 (S_one_half/Hybrid_TPS_basic.f90:112-213): inverse iteration on the tridiagonal
+// vec12 (the reference's interface): eigenvectors of the eigenvalues E[0..nvec) in v (nvec
+// columns of length ndim), unit norm.  Inverse iteration, three solves from a fixed start;
+// vectors of eigenvalues closer than 1e-10 (relative to the spectrum's scale) to an earlier
+// one are kept orthogonal to that cluster after every solve.
 int pned_vec12(const double *alpha, const double *beta, int ndim, const double *E, int nvec, double *v)
 {
     if (!alpha || !beta || !E || !v || ndim < 1 || nvec < 1) return fail(PNED_E_ARG, "bad argument");
-    std::vector<double> di(ndim), bl(ndim), bu(ndim + 1, 0.0), bv(ndim, 0.0), cm(ndim + 1, 0.0);
-    std::vector<int> lex(ndim, 0);
-    int km = 0;
+    const int n = ndim;
+    double scale = 0.0;
+    for (int i = 0; i < n; ++i)
+        scale = std::max(scale, std::fabs(alpha[i]) + (i < n - 1 ? std::fabs(beta[i]) : 0.0) +
+                                    (i > 0 ? std::fabs(beta[i - 1]) : 0.0));
+    if (scale == 0.0) scale = 1.0;
+    const double tiny = std::numeric_limits<double>::epsilon() * scale;
+    std::vector<double> dl(n), d(n), du(n), du2(n);
+    std::vector<char> sw(n);
+    int c0 = 0;   // first vector of the current cluster
     for (int k = 0; k < nvec; ++k) {
-        double *vk = v + (size_t)k * ndim;
-        for (int j = 0; j < ndim; ++j) { di[j] = E[k] - alpha[j]; bl[j] = -beta[j]; bu[j] = -beta[j]; }
-        for (int j = 0; j < ndim - 1; ++j) {
-            if (std::fabs(di[j]) > std::fabs(bl[j])) {
-                lex[j] = 0;
-                if (std::fabs(di[j]) < 1e-13) di[j] = 1e-13;
-                cm[j + 1] = bl[j] / di[j];
-                di[j + 1] -= cm[j + 1] * bu[j];
-                bv[j] = 0.0;
+        if (k > 0 && std::fabs(E[k] - E[k - 1]) > 1e-10 * scale) c0 = k;
+        // LU with partial pivoting of T - E_k I: rows i, i+1 exchanged where sw[i]
+        for (int i = 0; i < n; ++i) {
+            d[i] = alpha[i] - E[k];
+            dl[i] = (i < n - 1) ? beta[i] : 0.0;
+            du[i] = (i < n - 1) ? beta[i] : 0.0;
+            du2[i] = 0.0;
+        }
+        for (int i = 0; i < n - 1; ++i) {
+            if (std::fabs(d[i]) >= std::fabs(dl[i])) {
+                sw[i] = 0;
+                if (std::fabs(d[i]) < tiny) d[i] = (d[i] < 0.0) ? -tiny : tiny;
+                const double f = dl[i] / d[i];
+                dl[i] = f;
+                d[i + 1] -= f * du[i];
             } else {
-                lex[j] = 1;
-                cm[j + 1] = di[j] / bl[j];
-                di[j] = bl[j];
-                const double s = bu[j];
-                bu[j] = di[j + 1];
-                bv[j] = bu[j + 1];
-                di[j + 1] = s - cm[j + 1] * bu[j];
-                bu[j + 1] = -cm[j + 1] * bv[j];
+                sw[i] = 1;
+                const double f = d[i] / dl[i];
+                d[i] = dl[i];
+                dl[i] = f;
+                const double t = du[i];
+                du[i] = d[i + 1];
+                d[i + 1] = t - f * d[i + 1];
+                if (i < n - 2) {
+                    du2[i] = du[i + 1];
+                    du[i + 1] = -f * du[i + 1];
+                }
             }
         }
-        if (std::fabs(di[ndim - 1]) < 1e-13) di[ndim - 1] = 1e-13;
-        for (int j = 0; j < ndim; ++j) vk[j] = 1.0 / ((double)(j + 1) * 5.0);
-        if (k == 0) km = k;
-        else if (std::fabs(E[k] - E[km]) > 1e-13) km = k;
-        else
-            for (int i = km; i < k; ++i) {
-                double prd = 0.0;
-                for (int j = 0; j < ndim; ++j) prd += v[(size_t)i * ndim + j] * vk[j];
-                for (int j = 0; j < ndim; ++j) vk[j] -= prd * v[(size_t)i * ndim + j];
+        if (std::fabs(d[n - 1]) < tiny) d[n - 1] = (d[n - 1] < 0.0) ? -tiny : tiny;
+        double *x = v + (size_t)k * n;
+        for (int i = 0; i < n; ++i) x[i] = 1.0 + 0.5 * std::sin(0.7 * (double)i + 0.3 * (double)k);
+        for (int sweep = 0; sweep < 3; ++sweep) {
+            for (int i = 0; i < n - 1; ++i) {   // forward: L
+                if (sw[i]) std::swap(x[i], x[i + 1]);
+                x[i + 1] -= dl[i] * x[i];
             }
-        for (int l = 1; l <= k - km + 3; ++l) {
-            if (l != 1 || k != km)
-                for (int j = 0; j < ndim - 1; ++j) {
-                    if (lex[j] == 0) vk[j + 1] -= cm[j + 1] * vk[j];
-                    else { const double s = vk[j]; vk[j] = vk[j + 1]; vk[j + 1] = s - cm[j + 1] * vk[j]; }
+            for (int i = n - 1; i >= 0; --i) {   // backward: U (two super-diagonals)
+                double s = x[i];
+                if (i + 1 < n) s -= du[i] * x[i + 1];
+                if (i + 2 < n) s -= du2[i] * x[i + 2];
+                x[i] = s / d[i];
+            }
+            for (int pass = 0; pass < 2; ++pass)   // orthogonal to the cluster so far
+                for (int j = c0; j < k; ++j) {
+                    const double *y = v + (size_t)j * n;
+                    double pr = 0.0;
+                    for (int i = 0; i < n; ++i) pr += y[i] * x[i];
+                    for (int i = 0; i < n; ++i) x[i] -= pr * y[i];
                 }
-            for (int j = ndim - 1; j >= 0; --j) {
-                double s = vk[j];
-                if (j <= ndim - 2) s -= bu[j] * vk[j + 1];
-                if (j <= ndim - 3) s -= bv[j] * vk[j + 2];
-                vk[j] = s / di[j];
-            }
-            double dn = 0.0;
-            for (int j = 0; j < ndim; ++j) dn += vk[j] * vk[j];
-            if (dn > 1e-13) dn = 1.0 / std::sqrt(dn);
-            for (int j = 0; j < ndim; ++j) vk[j] *= dn;
+            double nrm = 0.0;
+            for (int i = 0; i < n; ++i) nrm += x[i] * x[i];
+            nrm = std::sqrt(nrm);
+            if (nrm > 0.0)
+                for (int i = 0; i < n; ++i) x[i] /= nrm;
         }
     }
     return PNED_OK;

@@ -500,7 +500,9 @@ def test_fortran_reference_call_sequence(tmp_path):
     itr, _, _, ene = oracle.lanczos(oop, g, max_itr=max_itr, nvec=nvec)
     e0 = float(vals["E0"])
     assert abs(e0 - ene[0]) <= 1e-10 * abs(ene[0])
-    assert int(vals["ITR"]) == itr
+    # stop step of the 1e-13 change test (checks every 5 steps): bounded, not matched, as
+    # rounding at 1e-15 in alpha/beta can move it by a check or two
+    assert abs(int(vals["ITR"]) - itr) <= 10, (vals["ITR"], itr)
     # inverse iteration from the same seeded start: basisme00000 holds E0 and x
     rec = _fortran_records(tmp_path / "basisme00000")
     assert np.frombuffer(rec[0], dtype=np.float64)[0] == e0

@@ -41,7 +41,10 @@ def test_lnc1_resume_is_bitwise(tmp_path, spec, q, stop):
     assert itr1 > stop
     if ctx.dim <= 1 << 16:   # oracle: the reference's loop on the same start vector
         itr_ref, a_ref, b_ref, e_ref = oracle.lanczos(oracle.OracleOp(op), x, max_itr=max_itr, nvec=nvec)
-        assert itr1 == itr_ref
+        # the stop step is where the 1e-13 relative change test between checks (every 5
+        # steps) first passes; rounding at 1e-15 in alpha/beta (another summation order) can
+        # move it by a check or two, so it is bounded, not matched
+        assert abs(itr1 - itr_ref) <= 10, (itr1, itr_ref)
         assert abs(ene1[0] - e_ref[0]) < 1e-10 * abs(e_ref[0])
         assert np.allclose(a1[:20], a_ref[:20], rtol=1e-12)
     # the same run stopped at the top of step `stop`, checkpointed, resumed in new vectors
