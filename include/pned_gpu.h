@@ -130,7 +130,9 @@ int pned_spin_plan_export(int n_sites, int q, int ibond, const int32_t *ipair, c
  * site_bit[site_bit[k]] == k.  A new single-operator S=1/2 context picks the layout that
  * puts the three sites into the lowest memory bits (the bits every tile pass keeps, for
  * contiguous 128-byte runs) that need the fewest tile passes; identity when no choice
- * saves a pass.  Every host boundary -- upload/download, vector files, the Gaussian start
+ * saves a pass.  From 30 sites on, the swaps left move sites into memory bits 3..7 when
+ * the tile passes then hold more of those bits (longer runs per 4 KB of memory), with the
+ * same pass count.  Every host boundary -- upload/download, vector files, the Gaussian start
  * vector (counter = basis index), S^z(q), S^+-(q), mTPQ observables, check1 -- maps
  * through it, so callers see the reference's basis order; only pned_vec_data exposes the
  * layout.  pned_ctx_sz_sector returns a context to the identity layout. */
