@@ -327,6 +327,13 @@ int pned_tps_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, pned_
  * data.  At most 8 structure-factor entries. */
 int pned_tps_fused(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, double lmax, int nsteps,
                    double *alpha, double *beta, double *coef);
+/* pned_tps_fused (the same outputs, bit for bit) with the steps replayed from a HIP graph
+ * captured once: graph_steps steps per replay (a multiple of 3, at most 3000; 0 = 30), the
+ * 1/beta scale of each step formed on the device, one host read-back per replay.  Same
+ * counterpart (parallelized_TPSz, Hybrid_TPS_TPS.f90:45-409) and requirements; the call
+ * allocates its step history (graph_steps x (2 + 26) x nvec doubles) and frees it on return. */
+int pned_tps_fused_graph(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, double lmax, int nsteps,
+                         int graph_steps, double *alpha, double *beta, double *coef);
 
 /* ---- DSF (Hybrid_TPS_DSF.f90) ------------------------------------------------------------ */
 /* intalv_zz (Hybrid_TPS_DSF.f90:18-87): v = normalised sum_j w(d_j) e^{i q.r_j} x with

@@ -352,6 +352,16 @@ int gaussian_fill(pned_ctx *ctx, pned_vec *v, uint64_t seed);
 int sz_q_apply(pned_ctx *ctx, const pned_vec *x, pned_vec *v, const double *q, const double *site_vec);
 int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, double *out);
 int tps_fused_reduce(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, const pned_vec *w, double *out);
+// fused mTPQ reduction outputs per column (10 + 2 x at most 8 structure-factor entries)
+constexpr int TPS_RED_WIDTH = 26;
+// the captured form of the fused mTPQ step: a device copy of the observable tables
+// (*dobs, freed by the caller with hipFree), then per step, enqueued only (no allocation, no
+// host round trip): su = 1/sqrt(red_prev[c][0]) (1 when red_prev is null), w = (L-H)(su u),
+// slot[0..2nc) = the H.v's per-column product (what pned_tps_fused reads back),
+// slot[2nc..) = the reduction of (up, u, w) ([column][TPS_RED_WIDTH])
+int tps_graph_setup(pned_ctx *ctx, const pned_vec *u, void **dobs, int *nsq);
+int tps_graph_step(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, pned_vec *w, const void *dobs, int nsq,
+                   const double *red_prev, double *su_dev, double *slot);
 int renbun_gpu(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,
                const double *salpha, const double *sbeta, double *w, double *dsf, int *conv);
 

@@ -762,6 +762,39 @@ int pned_tps_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, pned_vec *vg, pned_
     return PNED_OK;
 }
 
+// One column's alpha, beta and eval_phys_val row of fused mTPQ step i (1-based) from the
+// step's H.v product pr (2 doubles: <u|(L-H)(su u)>) and reduction o; su is the step's input
+// scale (1/beta of the step before, 1 at the first step) and sp the one before it.
+static void tps_fused_row(double lmax, int i, int nsq, double su, double sp, const double *pr, const double *o,
+                          double *alpha, double *beta, double *row)
+{
+    const double a1 = su * pr[0];                   // <v1|(L-H)|v1>
+    const double b1 = std::sqrt(o[0]);              // ||v0||
+    *alpha = lmax - a1;
+    *beta = b1;
+    const double gv = sp * su, vv = su * su;
+    row[0] = (double)(2 * i) / a1;
+    row[1] = lmax;
+    row[2] = b1;
+    row[3] = lmax - a1;
+    row[4] = lmax * gv * o[4] - sp * o[2];          // Re(l <vg|v1> - <vg|v0>)
+    row[5] = vv * o[6];
+    row[6] = gv * o[8];
+    for (int k = 0; k < nsq; ++k) {
+        const bool left_g = (k % 4) % 2 == 1;
+        row[7 + k] = (left_g ? gv : vv) * o[10 + 2 * k];
+    }
+}
+
+static int tps_fused_check(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb)
+{
+    if (ctx->site_vec.empty()) return fail(PNED_E_ARG, "call pned_tps_set_observables first");
+    if (ctx->q != 2) return fail(PNED_E_UNSUPPORTED, "fused mTPQ is S=1/2");
+    PNED_TRY(check_hv_pair(ctx, v1, va));
+    PNED_TRY(check_hv_pair(ctx, v1, vb));
+    return check_hv_pair(ctx, va, vb);
+}
+
 // The same recursion and coef rows as pned_tps with the vector work fused: vectors are
 // rotated instead of copied (vg <- v1 <- v0 by handle), the 1/beta normalisation rides
 // on the next H.v as a per-column input scale, and every reduction of a step is one pass
@@ -771,11 +804,7 @@ int pned_tps_fused(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, doub
                    double *beta, double *coef)
 {
     if (!ctx || !v1 || !va || !vb || !alpha || !beta || !coef) return fail(PNED_E_ARG, "null argument");
-    if (ctx->site_vec.empty()) return fail(PNED_E_ARG, "call pned_tps_set_observables first");
-    if (ctx->q != 2) return fail(PNED_E_UNSUPPORTED, "fused mTPQ is S=1/2");
-    PNED_TRY(check_hv_pair(ctx, v1, va));
-    PNED_TRY(check_hv_pair(ctx, v1, vb));
-    PNED_TRY(check_hv_pair(ctx, va, vb));
+    PNED_TRY(tps_fused_check(ctx, v1, va, vb));
     const int nc = v1->nvec, nsq = ctx->nsq, ncoef = 7 + nsq, width = 10 + 2 * nsq;
     pned_vec *u = v1, *up = va, *w = vb;
     PNED_TRY(pned_vec_zero(ctx, up));
@@ -786,26 +815,11 @@ int pned_tps_fused(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, doub
         PNED_TRY(launch_hv_s12(ctx, u, w, 0.0, true, pr.data(), dscale));   // w = (L-H)(su u) = v0
         PNED_TRY(tps_fused_reduce(ctx, up, u, w, red.data()));
         for (int c = 0; c < nc; ++c) {
-            const double *o = red.data() + (size_t)c * width;
-            const double a1 = su[c] * pr[2 * c];            // <v1|(L-H)|v1>
-            const double b1 = std::sqrt(o[0]);                // ||v0||
-            alpha[(size_t)(i - 1) * nc + c] = lmax - a1;
-            beta[(size_t)(i - 1) * nc + c] = b1;
-            double *row = coef + ((size_t)(i - 1) * nc + c) * ncoef;
-            const double gv = sp[c] * su[c], vv = su[c] * su[c];
-            row[0] = (double)(2 * i) / a1;
-            row[1] = lmax;
-            row[2] = b1;
-            row[3] = lmax - a1;
-            row[4] = lmax * gv * o[4] - sp[c] * o[2];        // Re(l <vg|v1> - <vg|v0>)
-            row[5] = vv * o[6];
-            row[6] = gv * o[8];
-            for (int k = 0; k < nsq; ++k) {
-                const bool left_g = (k % 4) % 2 == 1;
-                row[7 + k] = (left_g ? gv : vv) * o[10 + 2 * k];
-            }
+            const size_t at = (size_t)(i - 1) * nc + c;
+            tps_fused_row(lmax, i, nsq, su[c], sp[c], pr.data() + 2 * c, red.data() + (size_t)c * width, alpha + at,
+                          beta + at, coef + at * ncoef);
             sp[c] = su[c];
-            su[c] = 1.0 / b1;
+            su[c] = 1.0 / beta[at];
         }
         pned_vec *t = up;   // vg <- v1, v1 <- v0, v0 <- free
         up = u;
@@ -813,6 +827,114 @@ int pned_tps_fused(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, doub
         w = t;
     }
     PNED_HIP(hipStreamSynchronize(ctx->stream));
+    return PNED_OK;
+}
+
+namespace {
+// device and pinned buffers and the graph of one pned_tps_fused_graph call
+struct TpsGraphRun {
+    void *dobs = nullptr;
+    double *hist = nullptr, *su = nullptr, *host = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~TpsGraphRun()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (host) (void)hipHostFree(host);
+        if (hist) (void)hipFree(hist);
+        if (su) (void)hipFree(su);
+        if (dobs) (void)hipFree(dobs);
+    }
+};
+}  // namespace
+
+// pned_tps_fused with the step sequence captured once into a HIP graph of graph_steps
+// steps (a multiple of 3, so the rotated vector roles repeat; 0 = 30) and replayed: each
+// step's 1/beta scale is formed on the device from the step before, each step's H.v
+// product and reduction land in a per-step history slot, and the host reads the history
+// once per replay.  The kernels, their inputs and their order are those of pned_tps_fused,
+// so alpha, beta and coef are the same bit for bit.  Step 1 and the steps after the last
+// whole replay run un-captured through the same enqueue path.
+int pned_tps_fused_graph(pned_ctx *ctx, pned_vec *v1, pned_vec *va, pned_vec *vb, double lmax, int nsteps,
+                         int graph_steps, double *alpha, double *beta, double *coef)
+{
+    if (!ctx || !v1 || !va || !vb || !alpha || !beta || !coef) return fail(PNED_E_ARG, "null argument");
+    if (graph_steps == 0) graph_steps = 30;
+    if (graph_steps < 3 || graph_steps % 3 != 0 || graph_steps > 3000)
+        return fail(PNED_E_ARG, "graph_steps is a positive multiple of 3 (at most 3000), or 0 for 30");
+    if (nsteps < 1) return fail(PNED_E_ARG, "nsteps must be positive");
+    PNED_TRY(tps_fused_check(ctx, v1, va, vb));
+    PNED_HIP(hipSetDevice(ctx->device));
+    const int nc = v1->nvec, ncoef = 7 + ctx->nsq, G = graph_steps;
+    const size_t slot_d = 2 * (size_t)nc + (size_t)TPS_RED_WIDTH * nc;
+    TpsGraphRun run;
+    int nsq = 0;
+    PNED_TRY(tps_graph_setup(ctx, v1, &run.dobs, &nsq));
+    PNED_HIP(hipMalloc((void **)&run.hist, sizeof(double) * slot_d * G));
+    PNED_HIP(hipMalloc((void **)&run.su, sizeof(double) * nc));
+    PNED_HIP(hipHostMalloc((void **)&run.host, sizeof(double) * slot_d * G, hipHostMallocDefault));
+    pned_vec *u = v1, *up = va, *w = vb;
+    PNED_TRY(pned_vec_zero(ctx, up));
+    std::vector<double> su(nc, 1.0), sp(nc, 0.0);
+    // step i (1-based) writes slot (i - 2) mod G and scales by the slot before it
+    auto slot_of = [G](int i) { return ((i - 2) % G + G) % G; };
+    auto enqueue = [&](int i) -> int {
+        const int sl = slot_of(i);
+        const double *prev = i == 1 ? nullptr : run.hist + (size_t)slot_of(i - 1) * slot_d + 2 * (size_t)nc;
+        PNED_TRY(tps_graph_step(ctx, up, u, w, run.dobs, nsq, prev, run.su, run.hist + (size_t)sl * slot_d));
+        pned_vec *t = up;   // vg <- v1, v1 <- v0, v0 <- free
+        up = u;
+        u = w;
+        w = t;
+        return PNED_OK;
+    };
+    // rows of steps first..first+count-1 (their slots contiguous) from the device history
+    auto harvest = [&](int first, int count) -> int {
+        const int s0 = slot_of(first);
+        PNED_HIP(hipMemcpyAsync(run.host, run.hist + (size_t)s0 * slot_d, sizeof(double) * slot_d * count,
+                                hipMemcpyDeviceToHost, ctx->stream));
+        PNED_HIP(hipStreamSynchronize(ctx->stream));
+        for (int j = 0; j < count; ++j) {
+            const int i = first + j;
+            const double *h = run.host + (size_t)j * slot_d;
+            for (int c = 0; c < nc; ++c) {
+                const size_t at = (size_t)(i - 1) * nc + c;
+                tps_fused_row(lmax, i, ctx->nsq, su[c], sp[c], h + 2 * c, h + 2 * (size_t)nc + (size_t)c * TPS_RED_WIDTH,
+                              alpha + at, beta + at, coef + at * ncoef);
+                sp[c] = su[c];
+                su[c] = 1.0 / beta[at];
+            }
+        }
+        return PNED_OK;
+    };
+    // step 1 also builds the H.v plan, its tables and the partial buffers: nothing in the
+    // captured steps allocates or waits on the host
+    PNED_TRY(enqueue(1));
+    PNED_TRY(harvest(1, 1));
+    const int nrep = (nsteps - 1) / G;
+    int i = 2;
+    if (nrep > 0) {
+        // the buffers the captured kernels address must stay where step 1 left them
+        const void *keep[4] = {ctx->d_partial, ctx->d_result, ctx->pd_dev, ctx->d_tm};
+        PNED_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        int rc = PNED_OK;
+        for (int j = 0; j < G && rc == PNED_OK; ++j) rc = enqueue(i + j);
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &run.graph);
+        PNED_TRY(rc);
+        if (ec != hipSuccess) return fail(PNED_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        const void *now[4] = {ctx->d_partial, ctx->d_result, ctx->pd_dev, ctx->d_tm};
+        if (std::memcmp(keep, now, sizeof(keep)) != 0)
+            return fail(PNED_E_HIP, "mTPQ graph: a work buffer moved while the steps were captured");
+        PNED_HIP(hipGraphInstantiate(&run.exec, run.graph, nullptr, nullptr, 0));
+        for (int r = 0; r < nrep; ++r, i += G) {
+            PNED_HIP(hipGraphLaunch(run.exec, ctx->stream));
+            PNED_TRY(harvest(i, G));
+        }
+    }
+    const int rest = nsteps - i + 1;
+    for (int j = 0; j < rest; ++j) PNED_TRY(enqueue(i + j));
+    if (rest > 0) PNED_TRY(harvest(i, rest));
     return PNED_OK;
 }
 

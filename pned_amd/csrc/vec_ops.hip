@@ -1349,6 +1349,54 @@ int tps_observables(pned_ctx *ctx, const pned_vec *vg, const pned_vec *v1, doubl
 }
 
 // out per column: the 10 + 2*nsq values of k_tps_fused
+// launch shape of the fused mTPQ reduction for u's columns: chunk of cs states (about 16
+// elements per thread, at most TPS_CS_MAX states), grid, dynamic LDS
+struct TpsReduceShape {
+    int cs, grid;
+    size_t lds;
+};
+
+static int tps_reduce_shape(const pned_vec *u, int nsq, TpsReduceShape &s)
+{
+    if (10 + 2 * nsq > TPS_RED_WIDTH)
+        return fail(PNED_E_UNSUPPORTED, "fused mTPQ reduction: at most 8 structure-factor entries");
+    if (u->nvec > RED_TPB) return fail(PNED_E_UNSUPPORTED, "fused mTPQ reduction: at most 256 columns");
+    s.cs = std::max(1, std::min(TPS_CS_MAX, 16 * RED_TPB / u->nvec));
+    const size_t nchunk = ((size_t)u->dim + s.cs - 1) / s.cs;
+    s.grid = (int)std::min<size_t>(nchunk, (size_t)RED_GRID);
+    s.lds = sizeof(double) * ((size_t)TPS_RED_WIDTH * RED_TPB + (size_t)s.cs * (1 + 2 * nsq));
+    return PNED_OK;
+}
+
+// k_tps_fused over (up, u, w) into ctx->d_partial, then the final reduction into out_dev
+// ([column][TPS_RED_WIDTH]); dargs is a device copy of the context's ObsArgs.  Enqueued
+// only: no allocation, no host round trip (the partial buffer must already hold the grid)
+static int tps_fused_reduce_launch(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, const pned_vec *w,
+                                   const ObsArgs *dargs, int nsq, double *out_dev)
+{
+    TpsReduceShape s;
+    PNED_TRY(tps_reduce_shape(u, nsq, s));
+    const size_t nstates = (size_t)u->dim;
+    if (u->dtype == PNED_C128) {
+        auto k = k_tps_fused<double, TPS_RED_WIDTH>;
+        PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.lds));
+        hipLaunchKernelGGL(k, dim3(s.grid), dim3(RED_TPB), s.lds, ctx->stream, (const double2 *)up->data,
+                           (const double2 *)u->data, (const double2 *)w->data, dargs, nstates, u->nvec, s.cs,
+                           ctx->d_partial);
+    } else {
+        auto k = k_tps_fused<float, TPS_RED_WIDTH>;
+        PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s.lds));
+        hipLaunchKernelGGL(k, dim3(s.grid), dim3(RED_TPB), s.lds, ctx->stream, (const float2 *)up->data,
+                           (const float2 *)u->data, (const float2 *)w->data, dargs, nstates, u->nvec, s.cs,
+                           ctx->d_partial);
+    }
+    PNED_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce_rows, dim3(TPS_RED_WIDTH * u->nvec), dim3(256), 0, ctx->stream, ctx->d_partial, s.grid,
+                       TPS_RED_WIDTH * u->nvec, out_dev);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
 int tps_fused_reduce(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, const pned_vec *w, double *out)
 {
     if (ctx->sector_k >= 0) return fail(PNED_E_UNSUPPORTED, "observables need the full 2^N basis (sector basis active)");
@@ -1357,42 +1405,68 @@ int tps_fused_reduce(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, const
     PNED_TRY(check_pow2_cols(u->nvec));
     ObsArgs a;
     PNED_TRY(build_obs_args(ctx, a));
-    const int nsq = a.nsq;
-    constexpr int NOUT = 10 + 2 * 8;
-    if (10 + 2 * nsq > NOUT) return fail(PNED_E_UNSUPPORTED, "fused mTPQ reduction: at most 8 structure-factor entries");
-    const int width = 10 + 2 * nsq;
-    if (u->nvec > RED_TPB) return fail(PNED_E_UNSUPPORTED, "fused mTPQ reduction: at most 256 columns");
-    // chunk of cs states: about 16 elements per thread, at most TPS_CS_MAX states
-    const int cs = std::max(1, std::min(TPS_CS_MAX, 16 * RED_TPB / u->nvec));
-    const size_t nchunk = ((size_t)u->dim + cs - 1) / cs;
-    const int grid = (int)std::min<size_t>(nchunk, (size_t)RED_GRID);
-    PNED_TRY(ensure_partial(ctx, (size_t)grid * NOUT * u->nvec));
+    const int nsq = a.nsq, width = 10 + 2 * nsq;
+    TpsReduceShape s;
+    PNED_TRY(tps_reduce_shape(u, nsq, s));
+    PNED_TRY(ensure_partial(ctx, (size_t)s.grid * TPS_RED_WIDTH * u->nvec));
+    PNED_TRY(ensure_result(ctx, (size_t)TPS_RED_WIDTH * u->nvec));
     double *dargs;
     const int nd = (int)((sizeof(ObsArgs) + sizeof(double) - 1) / sizeof(double));
     std::vector<double> hbuf(nd);
     std::memcpy(hbuf.data(), &a, sizeof(ObsArgs));
     PNED_TRY(upload_scalars(ctx, hbuf.data(), nd, &dargs));
-    const size_t lds = sizeof(double) * ((size_t)NOUT * RED_TPB + (size_t)cs * (1 + 2 * nsq));
-    const size_t nstates = (size_t)u->dim;
-    if (u->dtype == PNED_C128) {
-        auto k = k_tps_fused<double, NOUT>;
-        PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k, dim3(grid), dim3(RED_TPB), lds, ctx->stream, (const double2 *)up->data,
-                           (const double2 *)u->data, (const double2 *)w->data, (const ObsArgs *)dargs, nstates,
-                           u->nvec, cs, ctx->d_partial);
-    } else {
-        auto k = k_tps_fused<float, NOUT>;
-        PNED_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k, dim3(grid), dim3(RED_TPB), lds, ctx->stream, (const float2 *)up->data,
-                           (const float2 *)u->data, (const float2 *)w->data, (const ObsArgs *)dargs, nstates,
-                           u->nvec, cs, ctx->d_partial);
-    }
-    PNED_HIP(hipGetLastError());
-    std::vector<double> all((size_t)NOUT * u->nvec);
-    PNED_TRY(finish_reduce(ctx, grid, NOUT * u->nvec, all.data()));
+    PNED_TRY(tps_fused_reduce_launch(ctx, up, u, w, (const ObsArgs *)dargs, nsq, ctx->d_result));
+    const size_t all = (size_t)TPS_RED_WIDTH * u->nvec;
+    PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double) * all, hipMemcpyDeviceToHost, ctx->stream));
+    PNED_HIP(hipStreamSynchronize(ctx->stream));
     for (int c = 0; c < u->nvec; ++c)
-        for (int k = 0; k < width; ++k) out[(size_t)c * width + k] = all[(size_t)c * NOUT + k];
+        for (int k = 0; k < width; ++k) out[(size_t)c * width + k] = ctx->h_result[(size_t)c * TPS_RED_WIDTH + k];
     return PNED_OK;
+}
+
+// su[c] = 1 / sqrt(red[c][0]) (1/||v0|| of the previous step), or 1 without a previous step
+__global__ void k_tps_scale(const double *__restrict__ red, int nc, double *__restrict__ su)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nc) su[c] = red ? 1.0 / sqrt(red[(size_t)c * TPS_RED_WIDTH]) : 1.0;
+}
+
+__global__ void k_copy_f64(const double *__restrict__ src, double *__restrict__ dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+int tps_graph_setup(pned_ctx *ctx, const pned_vec *u, void **dobs, int *nsq)
+{
+    if (ctx->sector_k >= 0) return fail(PNED_E_UNSUPPORTED, "observables need the full 2^N basis (sector basis active)");
+    PNED_TRY(check_pow2_cols(u->nvec));
+    ObsArgs a;
+    PNED_TRY(build_obs_args(ctx, a));
+    TpsReduceShape s;
+    PNED_TRY(tps_reduce_shape(u, a.nsq, s));
+    PNED_TRY(ensure_partial(ctx, (size_t)s.grid * TPS_RED_WIDTH * u->nvec));
+    PNED_HIP(hipMalloc(dobs, sizeof(ObsArgs)));
+    const hipError_t e = hipMemcpy(*dobs, &a, sizeof(ObsArgs), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(*dobs);
+        *dobs = nullptr;
+        return fail(PNED_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    }
+    *nsq = a.nsq;
+    return PNED_OK;
+}
+
+int tps_graph_step(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, pned_vec *w, const void *dobs, int nsq,
+                   const double *red_prev, double *su_dev, double *slot)
+{
+    const int nc = u->nvec;
+    hipLaunchKernelGGL(k_tps_scale, dim3((nc + 63) / 64), dim3(64), 0, ctx->stream, red_prev, nc, su_dev);
+    PNED_HIP(hipGetLastError());
+    PNED_TRY(launch_hv_s12(ctx, u, w, 0.0, true, nullptr, su_dev));   // w = (L-H)(su u)
+    hipLaunchKernelGGL(k_copy_f64, dim3((2 * nc + 63) / 64), dim3(64), 0, ctx->stream, ctx->d_result, slot, 2 * nc);
+    PNED_HIP(hipGetLastError());
+    return tps_fused_reduce_launch(ctx, up, u, w, (const ObsArgs *)dobs, nsq, slot + 2 * nc);
 }
 
 int renbun_gpu(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,

@@ -115,6 +115,7 @@ def lib():
     L.pned_tps_set_observables.argtypes = [P, _dp, i32, _dp]
     L.pned_tps.argtypes = [P, P, P, P, P, f64, i32, _dp, _dp, _dp]
     L.pned_tps_fused.argtypes = [P, P, P, P, f64, i32, _dp, _dp, _dp]
+    L.pned_tps_fused_graph.argtypes = [P, P, P, P, f64, i32, i32, _dp, _dp, _dp]
     L.pned_intalv_zz.argtypes = [P, P, P, _dp, _dp, C.POINTER(f64)]
     L.pned_intalv_xy.argtypes = [P, P, P, _dp, _dp, i32, C.POINTER(f64)]
     L.pned_dsflnc1z.argtypes = [P, P, P, f64, i32, _dp, _dp]
@@ -515,13 +516,19 @@ class Context:
         check(lib().pned_tps(self.h, v1.h, v0.h, vg.h, vt.h, lmax, nsteps, alpha, beta, coef))
         return alpha.reshape(nsteps, nc), beta.reshape(nsteps, nc), coef.reshape(nsteps, nc, ncoef)
 
-    def tps_fused(self, v1: Vector, va: Vector, vb: Vector, lmax: float, nsteps: int):
+    def tps_fused(self, v1: Vector, va: Vector, vb: Vector, lmax: float, nsteps: int, graph_steps=None):
+        """Fused mTPQ; graph_steps (a multiple of 3, or 0 for 30) replays the steps from a
+        captured HIP graph (pned_tps_fused_graph), None launches them one by one."""
         nc = v1.nvec
         ncoef = 7 + getattr(self, "_nsq", 0)
         alpha = np.zeros(nsteps * nc)
         beta = np.zeros(nsteps * nc)
         coef = np.zeros(nsteps * nc * ncoef)
-        check(lib().pned_tps_fused(self.h, v1.h, va.h, vb.h, lmax, nsteps, alpha, beta, coef))
+        if graph_steps is None:
+            check(lib().pned_tps_fused(self.h, v1.h, va.h, vb.h, lmax, nsteps, alpha, beta, coef))
+        else:
+            check(lib().pned_tps_fused_graph(self.h, v1.h, va.h, vb.h, lmax, nsteps, int(graph_steps), alpha, beta,
+                                             coef))
         return alpha.reshape(nsteps, nc), beta.reshape(nsteps, nc), coef.reshape(nsteps, nc, ncoef)
 
     def intalv_zz(self, x: Vector, v: Vector, q, site_vec) -> float:

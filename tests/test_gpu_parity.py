@@ -586,6 +586,29 @@ def test_tps_fused_matches_oracle(dtype, tol):
         assert np.allclose(co[:, c, :], c_ref, rtol=max(tol, 1e-9), atol=1e-9 if dtype == gpu.C128 else 1e-4), c
 
 
+@pytest.mark.parametrize("dtype", [gpu.C128, gpu.C64])
+@pytest.mark.parametrize("steps,graph_steps", [(20, 6), (4, 0), (32, 3)])
+def test_tps_fused_graph_matches_launches(dtype, steps, graph_steps):
+    """The fused mTPQ run replayed from a captured HIP graph gives the one-launch-at-a-time
+    run's alpha, beta and coef rows bit for bit: step 1 un-captured, then whole replays
+    (each read back and checked) and the remaining steps un-captured; (4, 0) never replays."""
+    _say("tps fused graph", dtype, steps, graph_steps)
+    cl = lattice.reference_cluster(12)
+    op = model.build_operator(cl, 20, 90.0, tps=True)
+    ctx = gpu.Context(op)
+    ctx.tps_set_observables(cl.site_vec, cl.sq_momenta())
+    nvec = 8
+    v1, va, vb = (ctx.vector(nvec, dtype) for _ in range(3))
+    runs = []
+    for g in (None, graph_steps, None):   # launches, graph, launches again (context still usable)
+        ctx.set_initial_vec(v1, 21)
+        runs.append(ctx.tps_fused(v1, va, vb, op.shift_L, steps, graph_steps=g))
+    for k in range(3):
+        assert np.array_equal(runs[1][k], runs[0][k]), k
+        assert np.array_equal(runs[2][k], runs[0][k]), k
+    assert np.all(np.isfinite(runs[1][2]))
+
+
 def test_tps_fused_c64_n24_r64():
     """The batched form at its configured size: 3x4 torus (N=24), R=64 samples, mTPQ steps of
     (l - H/N) with the bond-norm shift.  The fused complex64 run against the fused complex128
@@ -873,6 +896,9 @@ def test_argument_errors_leave_the_context_usable():
         (lambda: ctx.lanczos_batch_step(v1, foreign, one, np.zeros(1), np.zeros(1)), gpu.E_ARG),
         (lambda: ctx.tps_fused(v1, v0, v0, 1.0, 2), gpu.E_ARG),
         (lambda: ctx.tps_fused(v1, foreign, v0, 1.0, 2), gpu.E_ARG),
+        (lambda: ctx.tps_fused(v1, v0, v0, 1.0, 2, graph_steps=6), gpu.E_ARG),
+        (lambda: ctx.tps_fused(v1, wide, v0, 1.0, 2, graph_steps=6), gpu.E_ARG),
+        (lambda: ctx.tps_fused(v1, small, v0, 1.0, 8, graph_steps=4), gpu.E_ARG),   # not a multiple of 3
         (lambda: ctx.autotune(v1, v1), gpu.E_ARG),
         (lambda: ctx.autotune(foreign, v0), gpu.E_ARG),
         (lambda: ctx.p_mltply_v3(v1, wide), gpu.E_ARG),                     # column counts differ
