@@ -85,6 +85,11 @@ int pned_ctx_autotune(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, int reps,
                       int *best_low_bits, int *best_threads, double *best_ms);
 /* spin-S kernel choice: 1 = tiled passes (default, q = 3..5), 0 = per-state gather */
 int pned_ctx_spin_kernel(pned_ctx *ctx, int tiled);
+/* HIP graph replay in the Krylov loops (no reference counterpart): 1 (default) = pned_lnc1
+ * and pned_dsflnc1z on one-column S=1/2 full-space contexts replay their steps between the
+ * host's convergence checks from captured graphs of 5 steps; 0 = every step launched
+ * and read back one by one.  Both give the same alpha/beta bit for bit. */
+int pned_ctx_graphs(pned_ctx *ctx, int enable);
 /* S^z sector basis (parity_type = 0, mk_list12: S_one_half/Hybrid_TPS_parallel.f90:452-578):
  * restrict the context to the states with n_down set bits, indexed as the reference's
  * list1/list3 tables do; n_down < 0 returns to the full 2^N space.  The context's dim

@@ -1272,6 +1272,13 @@ int pned_ctx_spin_kernel(pned_ctx *ctx, int tiled)
     return PNED_OK;
 }
 
+int pned_ctx_graphs(pned_ctx *ctx, int enable)
+{
+    if (!ctx) return fail(PNED_E_ARG, "null context");
+    ctx->use_graphs = enable ? 1 : 0;
+    return PNED_OK;
+}
+
 int pned_ctx_s12_kernel(pned_ctx *ctx, int kind)
 {
     if (!ctx) return fail(PNED_E_ARG, "null context");

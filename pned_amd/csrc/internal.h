@@ -244,6 +244,7 @@ struct pned_ctx {
     // spin-S
     pned::GpuTables spin;
     int spin_tiled = 1;              // tiled kernel (q = 3..5); 0 = per-state gather kernel
+    int use_graphs = 1;              // Krylov loops replay captured HIP graphs of their steps
     void *spin_pd = nullptr;         // tiled-pass tables
     int spin_K = -1, spin_np = 0;
     std::vector<int> spin_nb, spin_tsz, spin_nlo, spin_ncls, spin_cplx;
@@ -284,6 +285,13 @@ int cocg_update(pned_ctx *ctx, const pned_vec *p, const pned_vec *y, pned_vec *x
                 double *out4);
 // after such steps: v1 <- sx x, v0 <- y + a sx x, (x, y) = (v1, v0) or, swapped, (v0, v1)
 int lanczos_lazy_restore(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, bool swapped, double sx, double a);
+// lanczos_lazy_step's enqueue-only form (one column, S=1/2 full space), which Lanczos
+// loops capture into graphs: st = [sx, sy, prd re, prd im, b2 of the step before] on the
+// device; the step sets sx = 1/sqrt(b2), sy = -sqrt(b2) sx (fresh: 1, 0), runs the step and
+// writes [prd re, prd im, b2] to slot and b2 to st[4].  No allocation or host wait once
+// lanczos_lazy_setup and one un-captured step of the same shape have run.
+int lanczos_lazy_setup(pned_ctx *ctx, const pned_vec *x);
+int lanczos_lazy_enqueue(pned_ctx *ctx, pned_vec *x, pned_vec *y, bool fresh, double *st, double *slot);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
 // returns the context to the identity layout (re-decomposes its operator); only while no
 // vector of the context holds data in the old layout matters to the caller

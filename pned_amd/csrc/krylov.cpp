@@ -24,12 +24,23 @@ double cabs2(const double *z) { return std::sqrt(z[0] * z[0] + z[1] * z[1]); }
 // reference's update as written.
 // finish() leaves v1 and v0 as the reference's loop leaves them (the last Lanczos vector
 // and H v1 - beta v_prev), so a dump for restart holds the same vectors.
+// With graphs on (one S=1/2 full-space column), the steps are enqueued in their device form:
+// each step forms its sx, sy from the previous step's beta^2 on the device and leaves
+// [prd, beta^2] in a history slot, so advance() replays 5 steps at a time from a captured
+// graph (one per buffer role) and reads the 5 slots back once.  The host repeats the same
+// sx, sy arithmetic to form alpha = sx prd and beta = sqrt(beta^2), so both forms give the
+// same alpha/beta bit for bit.
 struct LanczosSteps {
+    static constexpr int GRAPH_STEPS = 5;
     pned_ctx *ctx;
     pned_vec *v1, *v0;
-    bool lazy;
+    bool lazy, dev;
     pned_vec *X, *Y;
     double sx = 1.0, sy = 0.0;
+    double *st = nullptr;     // device: [sx, sy, prd re, prd im, beta^2], then GRAPH_STEPS slots of 3
+    double *host = nullptr;   // pinned read-back of the slots
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     // explicit: the reference's update on the true vectors at every step (no deferred
     // normalisation), so the loop state can be checkpointed and resumed bit for bit
     LanczosSteps(pned_ctx *c, pned_vec *a, pned_vec *b, bool explicit_update = false)
@@ -37,8 +48,20 @@ struct LanczosSteps {
           lazy(!explicit_update && a->nvec == 1 &&
                ((c->q == 2 && c->sector_k < 0) ||
                 (c->q >= 3 && c->q <= 5 && c->spin_tiled && a->dtype == PNED_C128))),
+          dev(lazy && c->q == 2 && c->use_graphs),
           X(a), Y(b)
     {
+    }
+    LanczosSteps(const LanczosSteps &) = delete;
+    LanczosSteps &operator=(const LanczosSteps &) = delete;
+    ~LanczosSteps()
+    {
+        for (int k = 0; k < 2; ++k) {
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+        }
+        if (host) (void)hipHostFree(host);
+        if (st) (void)hipFree(st);
     }
     int lazy_ab(bool fresh, double &a1, double &b1)
     {
@@ -48,9 +71,38 @@ struct LanczosSteps {
         b1 = std::sqrt(b2);
         return PNED_OK;
     }
+    // the host's copy of the device's role swap and scales for the next step
+    void roll(double b1)
+    {
+        const double nsx = 1.0 / b1, nsy = -b1 * sx;   // v1 is Y / b1, v0 is X times -b1 sx
+        std::swap(X, Y);
+        sx = nsx;
+        sy = nsy;
+    }
+    // alpha, beta of the n steps whose slots were just filled (roll() before each but the
+    // first step of a run); b1 carries the beta before them
+    int harvest(int n, bool fresh, double &b1, double *a_out, double *b_out)
+    {
+        PNED_HIP(hipMemcpyAsync(host, st + 5, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+        PNED_HIP(hipStreamSynchronize(ctx->stream));
+        for (int j = 0; j < n; ++j) {
+            if (!(fresh && j == 0)) roll(b1);
+            a_out[j] = sx * host[3 * j];
+            b1 = std::sqrt(host[3 * j + 2]);
+            b_out[j] = b1;
+        }
+        return PNED_OK;
+    }
     // v0 = H v1 from v1 (normalised) and alpha, beta
     int first(double &a1, double &b1)
     {
+        if (dev) {
+            PNED_TRY(lanczos_lazy_setup(ctx, X));
+            PNED_HIP(hipMalloc((void **)&st, sizeof(double) * (5 + 3 * GRAPH_STEPS)));
+            PNED_HIP(hipHostMalloc((void **)&host, sizeof(double) * 3 * GRAPH_STEPS, hipHostMallocDefault));
+            PNED_TRY(lanczos_lazy_enqueue(ctx, X, Y, true, st, st + 5));
+            return harvest(1, true, b1, &a1, &b1);
+        }
         if (lazy) return lazy_ab(true, a1, b1);
         PNED_TRY(pned_vec_zero(ctx, v0));
         return pned_calc_alpha_beta(ctx, v1, v0, &a1, &b1);
@@ -58,15 +110,64 @@ struct LanczosSteps {
     // the update with the previous (a1, b1), then the next alpha, beta into them
     int next(double &a1, double &b1)
     {
+        if (dev) {
+            // enqueued with the roles after the swap that harvest's roll() mirrors
+            PNED_TRY(lanczos_lazy_enqueue(ctx, Y, X, false, st, st + 5));
+            return harvest(1, false, b1, &a1, &b1);
+        }
         if (lazy) {
-            const double nsx = 1.0 / b1, nsy = -b1 * sx;   // v1 is Y / b1, v0 is X times -b1 sx
-            std::swap(X, Y);
-            sx = nsx;
-            sy = nsy;
+            roll(b1);
             return lazy_ab(false, a1, b1);
         }
         PNED_TRY(pned_lanczos_update(ctx, v1, v0, &a1, &b1));
         return pned_calc_alpha_beta(ctx, v1, v0, &a1, &b1);
+    }
+    int capture(int role)
+    {
+        // the buffers the captured kernels address must stay where the first step left them
+        const void *keep[4] = {ctx->d_partial, ctx->d_result, ctx->pd_dev, ctx->d_tm};
+        PNED_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        pned_vec *x = X, *y = Y;
+        int rc = PNED_OK;
+        for (int j = 0; j < GRAPH_STEPS && rc == PNED_OK; ++j) {
+            std::swap(x, y);
+            rc = lanczos_lazy_enqueue(ctx, x, y, false, st, st + 5 + 3 * j);
+        }
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph[role]);
+        PNED_TRY(rc);
+        if (ec != hipSuccess) return fail(PNED_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        const void *now[4] = {ctx->d_partial, ctx->d_result, ctx->pd_dev, ctx->d_tm};
+        if (std::memcmp(keep, now, sizeof(keep)) != 0)
+            return fail(PNED_E_HIP, "Lanczos graph: a work buffer moved while the steps were captured");
+        PNED_HIP(hipGraphInstantiate(&exec[role], graph[role], nullptr, nullptr, 0));
+        return PNED_OK;
+    }
+    // up to n more steps (next() n times): alpha, beta of each into a_out, b_out; a1, b1
+    // in/out as in next(); *ran = the steps taken.  Whole groups of GRAPH_STEPS replay a captured
+    // graph in the device form; one by one, with stop_at_breakdown a step with beta <
+    // 0.5e-30 ends the run early.
+    int advance(int n, double &a1, double &b1, double *a_out, double *b_out, int *ran, bool stop_at_breakdown)
+    {
+        int k = 0;
+        *ran = 0;
+        while (k < n) {
+            if (stop_at_breakdown && k > 0 && b_out[k - 1] < 0.5e-30) break;
+            if (dev && n - k >= GRAPH_STEPS) {
+                const int role = X != v1;
+                if (!exec[role]) PNED_TRY(capture(role));
+                PNED_HIP(hipGraphLaunch(exec[role], ctx->stream));
+                PNED_TRY(harvest(GRAPH_STEPS, false, b1, a_out + k, b_out + k));
+                k += GRAPH_STEPS;
+                a1 = a_out[k - 1];
+            } else {
+                PNED_TRY(next(a1, b1));
+                a_out[k] = a1;
+                b_out[k] = b1;
+                ++k;
+            }
+            *ran = k;
+        }
+        return PNED_OK;
     }
     // a1: the last alpha
     int finish(double a1)
@@ -240,26 +341,38 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
     beta[0] = b1;
     double ebefor = 0.0, eps = 1e-25;
     const int ne = std::max(nvec, 1);
-    for (int i = 2; i <= last; ++i) {
-        PNED_TRY(run.next(a1, b1));
-        alpha[i - 1] = a1;
-        beta[i - 1] = b1;
-        if (b1 < 0.5e-30) {
-            *itr = i;
-            PNED_TRY(run.finish(a1));
-            return fail(PNED_E_BREAKDOWN, "#(E07) Tridiagonalization unsuccessful in lnc1: beta too small at i=" +
-                                              std::to_string(i));
+    std::vector<double> ab, bb;
+    for (int i = 2; i <= last;) {
+        // the steps up to the next one after which the host acts: the 4*nvec-th (first
+        // bisection), then every 5th (the convergence test), or the last
+        int e = last;
+        if (fixed_steps <= 0) e = std::min(e, i <= 4 * ne ? 4 * ne : (i + 4) / 5 * 5);
+        const int n = e - i + 1;
+        ab.resize(n);
+        bb.resize(n);
+        int ran = 0;
+        PNED_TRY(run.advance(n, a1, b1, ab.data(), bb.data(), &ran, true));
+        for (int j = 0; j < ran; ++j) {
+            alpha[i + j - 1] = ab[j];
+            beta[i + j - 1] = bb[j];
+            if (bb[j] < 0.5e-30) {
+                *itr = i + j;
+                PNED_TRY(run.finish(a1));
+                return fail(PNED_E_BREAKDOWN, "#(E07) Tridiagonalization unsuccessful in lnc1: beta too small at i=" +
+                                                  std::to_string(i + j));
+            }
         }
+        i = e + 1;
         if (fixed_steps > 0) continue;
-        if (i > 4 * ne && i % 5 == 0) {
-            PNED_TRY(pned_bisec(alpha, beta, i, ene, ne, eps));
+        if (e > 4 * ne && e % 5 == 0) {
+            PNED_TRY(pned_bisec(alpha, beta, e, ene, ne, eps));
             if (std::fabs((ebefor - ene[ne - 1]) / ene[ne - 1]) < 1e-13) {
-                *itr = i;
+                *itr = e;
                 return run.finish(a1);
             }
             ebefor = ene[ne - 1];
         }
-        if (i == 4 * ne) {
+        if (e == 4 * ne) {
             eps = 1e-25;
             PNED_TRY(pned_bisec(alpha, beta, 4 * ne, ene, ne, eps));
             ebefor = ene[ne - 1];
@@ -984,10 +1097,16 @@ int pned_dsflnc1z(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, double cnorm, int s
     PNED_TRY(run.first(a1, b1));
     salpha[0] = a1;
     sbeta[1] = b1;
-    for (int i = 2; i <= s_maxitr - 1; ++i) {
-        PNED_TRY(run.next(a1, b1));
-        salpha[i - 1] = a1;
-        sbeta[i] = b1;
+    // steps 2 .. s_maxitr-1, with no test inside the loop (as the reference's)
+    const int n = s_maxitr - 2;
+    if (n > 0) {
+        std::vector<double> ab(n), bb(n);
+        int ran = 0;
+        PNED_TRY(run.advance(n, a1, b1, ab.data(), bb.data(), &ran, false));
+        for (int j = 0; j < n; ++j) {
+            salpha[j + 1] = ab[j];
+            sbeta[j + 2] = bb[j];
+        }
     }
     return run.finish(a1);
 }

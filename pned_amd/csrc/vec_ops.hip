@@ -929,6 +929,76 @@ int lanczos_lazy_step(pned_ctx *ctx, pned_vec *x, pned_vec *y, const double *sx,
     return PNED_OK;
 }
 
+__global__ void k_copy_f64(const double *__restrict__ src, double *__restrict__ dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// the captured form's scalars: st = [sx, sy, prd re, prd im, b2 of the step before]
+__global__ void k_lz_scale(double *__restrict__ st, int fresh)
+{
+    if (threadIdx.x != 0) return;
+    if (fresh) {
+        st[0] = 1.0;
+        st[1] = 0.0;
+    } else {
+        const double b = sqrt(st[4]), sxo = st[0];   // v1 is y / b, v0 is x times -b sx
+        st[0] = 1.0 / b;
+        st[1] = -b * sxo;
+    }
+}
+
+// slot = [prd re, prd im, b2], st[4] = b2; fused: all three from the H.v's result
+// ([prd re, prd im, b2]), else prd from st[2..4) and b2 from the beta kernel's result
+__global__ void k_lz_store(const double *__restrict__ result, double *__restrict__ st, double *__restrict__ slot,
+                           int fused)
+{
+    if (threadIdx.x != 0) return;
+    const double pr = fused ? result[0] : st[2], pi = fused ? result[1] : st[3];
+    const double b2 = fused ? result[2] : result[0];
+    slot[0] = pr;
+    slot[1] = pi;
+    slot[2] = b2;
+    st[4] = b2;
+}
+
+int lanczos_lazy_setup(pned_ctx *ctx, const pned_vec *x)
+{
+    PNED_TRY(ensure_result(ctx, 3 * (size_t)x->nvec + 8));
+    return ensure_partial(ctx, (size_t)RED_GRID * x->nvec);
+}
+
+int lanczos_lazy_enqueue(pned_ctx *ctx, pned_vec *x, pned_vec *y, bool fresh, double *st, double *slot)
+{
+    if (x->nvec != 1 || ctx->q != 2 || ctx->sector_k >= 0)
+        return fail(PNED_E_UNSUPPORTED, "captured Lanczos step: one column of an S=1/2 full-space context");
+    hipLaunchKernelGGL(k_lz_scale, dim3(1), dim3(64), 0, ctx->stream, st, fresh ? 1 : 0);
+    PNED_HIP(hipGetLastError());
+    bool fused = false;
+    PNED_TRY(launch_hv_s12(ctx, x, y, 0.0, fresh, nullptr, st, fresh ? nullptr : st + 1, &fused));
+    if (!fused) {
+        const size_t nelem = (size_t)x->dim;
+        hipLaunchKernelGGL(k_copy_f64, dim3(1), dim3(64), 0, ctx->stream, ctx->d_result, st + 2, 2);
+        PNED_HIP(hipGetLastError());
+        if (x->dtype == PNED_C128)
+            hipLaunchKernelGGL(k_lanczos_lazy<double>, dim3(RED_GRID), dim3(RED_TPB), 0, ctx->stream,
+                               (const double2 *)x->data, (double2 *)y->data, (const double *)st,
+                               (const double *)(st + 2), nelem, 1, ctx->d_partial);
+        else
+            hipLaunchKernelGGL(k_lanczos_lazy<float>, dim3(RED_GRID), dim3(RED_TPB), 0, ctx->stream,
+                               (const float2 *)x->data, (float2 *)y->data, (const double *)st,
+                               (const double *)(st + 2), nelem, 1, ctx->d_partial);
+        PNED_HIP(hipGetLastError());
+        hipLaunchKernelGGL(reduce_rows, dim3(1), dim3(256), 0, ctx->stream, ctx->d_partial, RED_GRID, 1,
+                           ctx->d_result);
+        PNED_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_lz_store, dim3(1), dim3(64), 0, ctx->stream, ctx->d_result, st, slot, fused ? 1 : 0);
+    PNED_HIP(hipGetLastError());
+    return PNED_OK;
+}
+
 int cocg_update(pned_ctx *ctx, const pned_vec *p, const pned_vec *y, pned_vec *x, pned_vec *r, const double *a,
                 double *out4)
 {
@@ -1429,12 +1499,6 @@ __global__ void k_tps_scale(const double *__restrict__ red, int nc, double *__re
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < nc) su[c] = red ? 1.0 / sqrt(red[(size_t)c * TPS_RED_WIDTH]) : 1.0;
-}
-
-__global__ void k_copy_f64(const double *__restrict__ src, double *__restrict__ dst, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
 }
 
 int tps_graph_setup(pned_ctx *ctx, const pned_vec *u, void **dobs, int *nsq)

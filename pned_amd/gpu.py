@@ -52,6 +52,7 @@ def lib():
     L.pned_ctx_tune.argtypes = [P, i32, i32, i32]
     L.pned_ctx_plan_info.argtypes = [P, C.POINTER(i32), _ip, i32]
     L.pned_ctx_spin_kernel.argtypes = [P, i32]
+    L.pned_ctx_graphs.argtypes = [P, i32]
     L.pned_ctx_s12_kernel.argtypes = [P, i32]
     L.pned_ctx_sz_sector.argtypes = [P, i32]
     L.pned_io_subrecord_limit.argtypes = [C.c_uint64]
@@ -277,6 +278,10 @@ class Context:
         k, l, t, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         check(lib().pned_ctx_autotune(self.h, v1.h, v0.h, reps, C.byref(k), C.byref(l), C.byref(t), C.byref(ms)))
         return {"tile_bits": k.value, "low_bits": l.value, "threads": t.value, "ms": ms.value}
+
+    def graphs(self, enable: bool):
+        """HIP graph replay of the one-column S=1/2 Lanczos steps (lnc1, dsflnc1z); on by default."""
+        check(lib().pned_ctx_graphs(self.h, 1 if enable else 0))
 
     def spin_kernel(self, tiled: bool):
         check(lib().pned_ctx_spin_kernel(self.h, 1 if tiled else 0))

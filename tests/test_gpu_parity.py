@@ -217,6 +217,40 @@ def test_lanczos_parity_spin1(spec, phi):
     assert np.allclose(beta[:20], b2[:20], rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("dtype", [gpu.C128, gpu.C64])
+def test_lanczos_graphs_match_launches(dtype):
+    """lnc1 (converging, and fixed steps) and dsflnc1z with their steps replayed from captured
+    5-step graphs give the launch-by-launch run's alpha/beta, energies, stop step and final
+    vectors bit for bit; complex64 takes the step whose beta kernel is a separate launch."""
+    _say("lanczos graphs", dtype)
+    op = model.build_operator(lattice.reference_cluster(16), 20, 126.0, field=0.1)
+    ctx = gpu.Context(op)
+    v1, v0 = ctx.vector(1, dtype), ctx.vector(1, dtype)
+    runs = []
+    for on in (True, False, True):   # graphs, launches, graphs again
+        ctx.graphs(on)
+        r = {}
+        ctx.set_initial_vec(v1, 20261202)
+        r["conv"] = ctx.lnc1(v1, v0, max_itr=1024, nvec=8)
+        ctx.set_initial_vec(v1, 11)
+        r["fixed"] = ctx.lnc1(v1, v0, fixed_steps=23, nvec=1)
+        r["fixed_vecs"] = (v1.download(), v0.download())
+        ctx.set_initial_vec(v1, 12)
+        r["dsf"] = ctx.dsflnc1z(v1, v0, 0.5, 38)
+        r["dsf_vecs"] = (v1.download(), v0.download())
+        runs.append(r)
+    ref = runs[1]
+    for got in (runs[0], runs[2]):
+        for key in ("conv", "fixed"):
+            assert got[key][0] == ref[key][0], key
+            for j in (1, 2, 3):
+                assert np.array_equal(got[key][j], ref[key][j]), (key, j)
+        for key in ("fixed_vecs", "dsf", "dsf_vecs"):
+            for a, b in zip(got[key], ref[key]):
+                assert np.array_equal(a, b), key
+    assert ref["conv"][0] % 5 == 0 and ref["conv"][0] > 32
+
+
 @pytest.mark.parametrize("q", [2, 3])
 def test_one_column_batch_step_equals_lnc1(q):
     """A one-column lanczos_batch_step sequence (the serving path for large sizes) gives
