@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=1 << 27)
     ap.add_argument("--cfg", default="N32",
-                    help="N32 (default), N24, TPS, SERVE, LANCZOS, LANCZOS32, SPIN1, SPINHV")
+                    help="N32 (default), N24, TPS, SERVE, LANCZOS, LANCZOS32, SPIN1, SPINHV, GRAPHS")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune", default="", help="fixed plan K,L,threads (skips the autotuner)")
     ap.add_argument("--requests", type=int, default=120, help="SERVE: number of requests")
@@ -117,6 +117,8 @@ def main():
         return bench_spin1(args)
     if args.cfg == "SERVE":
         return bench_serve(args)
+    if args.cfg == "GRAPHS":
+        return bench_graphs(args)
 
     if args.cfg == "N32":
         cl = lattice.torus(4, 4)
@@ -727,6 +729,90 @@ def bench_serve(args):
                       "capacity_ratio": round(cap_b / cap_s, 3), "arrival_rate": round(rate, 3),
                       "batched": batched, "sequential": plain}}
     print(json.dumps(out), flush=True)
+
+
+def bench_graphs(args):
+    """Captured HIP graphs against launch-by-launch steps, where launches and host round
+    trips weigh most: fused mTPQ (pned_tps_fused vs pned_tps_fused_graph, 30-step graphs) and
+    fixed-step lnc1 (pned_ctx_graphs on/off, 5-step graphs), ms per step over 301 / 300 steps,
+    the better of two timed runs each after a warm-up run; both forms must give the same
+    alpha/beta (and mTPQ rows) bit for bit."""
+    import numpy as np
+    import torch  # noqa: F401
+
+    from pned_amd import gpu, lattice, model
+
+    def timed(run, check):
+        best, ref = None, None
+        for rep in range(3):
+            t0 = time.perf_counter()
+            got = run()
+            dt = time.perf_counter() - t0
+            if ref is None:
+                ref = got
+            else:
+                check(got, ref)
+                best = dt if best is None else min(best, dt)
+        return best, ref
+
+    def same(a, b):
+        for x, y in zip(a, b):
+            if not np.array_equal(np.asarray(x), np.asarray(y)):
+                raise SystemExit("graph and launch-by-launch runs differ")
+
+    out = {"tps_fused": {}, "lnc1": {}}
+    for a, b, R, dt in ((2, 2, 8, gpu.C128), (2, 4, 8, gpu.C128), (3, 3, 8, gpu.C128), (2, 4, 64, gpu.C64)):
+        cl = lattice.torus(a, b)
+        op = model.build_operator(cl, 20, 90.0, tps=True)
+        ctx = gpu.Context(op)
+        ctx.tps_set_observables(cl.site_vec, cl.sq_momenta())
+        v1, va, vb = (ctx.vector(R, dt) for _ in range(3))
+        steps = 301
+        res = {}
+        refs = []
+        for g in (None, 30):
+            def run(g=g):
+                ctx.set_initial_vec(v1, 5)
+                ctx.sync()
+                return ctx.tps_fused(v1, va, vb, op.shift_L, steps, graph_steps=g)
+            t, r = timed(run, same)
+            refs.append(r)
+            res["graph" if g else "launch"] = round(t / steps * 1e3, 4)
+        same(refs[0], refs[1])
+        res["speedup"] = round(res["launch"] / res["graph"], 3)
+        key = f"N={cl.n} R={R} {'c128' if dt == gpu.C128 else 'c64'}"
+        out["tps_fused"][key] = res
+        _progress(f"fused mTPQ {key}: {res}")
+        for v in (v1, va, vb):
+            v.free()
+        ctx.close()
+    for name, cl in (("N=16", lattice.reference_cluster(16)), ("N=20", lattice.torus(2, 5)),
+                     ("N=24", lattice.reference_cluster(24))):
+        op = model.build_operator(cl, 20, 54.0)
+        ctx = gpu.Context(op)
+        v1, v0 = ctx.vector(), ctx.vector()
+        steps = 300
+        res = {}
+        refs = []
+        for on in (False, True):
+            def run(on=on):
+                ctx.graphs(on)
+                ctx.set_initial_vec(v1, 5)
+                ctx.sync()
+                r = ctx.lnc1(v1, v0, fixed_steps=steps, nvec=1)
+                return r[1], r[2]
+            t, r = timed(run, same)
+            refs.append(r)
+            res["graph" if on else "launch"] = round(t / steps * 1e3, 4)
+        same(refs[0], refs[1])
+        res["speedup"] = round(res["launch"] / res["graph"], 3)
+        out["lnc1"][name] = res
+        _progress(f"lnc1 {name}: {res}")
+        v1.free()
+        v0.free()
+        ctx.close()
+    print(json.dumps({"metric": "ms per Krylov step, captured HIP graphs vs launch-by-launch",
+                      "unit": "ms/step", "higher_is_better": False, "config": out}), flush=True)
 
 
 if __name__ == "__main__":
