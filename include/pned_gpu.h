@@ -281,6 +281,14 @@ int pned_set_initial_column(pned_ctx *ctx, pned_vec *v, int col, uint64_t seed);
 int pned_bisec(const double *alpha, const double *beta, int ndim, double *ene, int ne, double eps);
 int pned_vec12(const double *alpha, const double *beta, int ndim, const double *ene, int nvec,
                double *coef);
+/* The same two solves (same arguments, errors and results, bit for bit: one source for both
+ * sides) in FP64 on the context's GPU: a wavefront per eigenvalue evaluating six levels of
+ * the bisection tree per round, a lane per cluster of eigenvectors; host arrays in and out.  pned_lnc1 and its span/reorth forms take their
+ * convergence checks' eigenvalues from pned_ctx_bisec. */
+int pned_ctx_bisec(pned_ctx *ctx, const double *alpha, const double *beta, int ndim, double *ene, int ne,
+                   double eps);
+int pned_ctx_vec12(pned_ctx *ctx, const double *alpha, const double *beta, int ndim, const double *ene, int nvec,
+                   double *coef);
 
 /* Ritz vector: x = sum_k coef[k] v_k regenerated by re-running the Lanczos recursion
  * from the start vector in v1 (3 resident vectors: v1, v0, x).  Uses the alpha/beta of the

@@ -373,5 +373,17 @@ int tps_graph_step(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, pned_ve
 int renbun_gpu(pned_ctx *ctx, int i_range, int s_maxitr, double split, double ene, double eta,
                const double *salpha, const double *sbeta, double *w, double *dsf, int *conv);
 
+// tridiagonal T (alpha, beta of length ndim, ndim - 1): b2 = beta^2 and the bisection's
+// start bracket (Gershgorin bounds widened by round-off), stop width and pivot floor
+struct TridiagBounds {
+    double lo, hi, width, pivmin;
+};
+int tridiag_bounds(const double *alpha, const double *beta, int ndim, double eps, std::vector<double> &b2,
+                   TridiagBounds &tb);
+// inverse iteration's cluster starts (c0[k]: first eigenvalue of k's cluster, eigenvalues
+// closer than 1e-10 of the spectrum's scale) and pivot floor
+int tridiag_clusters(const double *alpha, const double *beta, int n, const double *E, int nvec,
+                     std::vector<int> &c0, double &tiny);
+
 inline size_t elem_bytes(int dtype) { return dtype == PNED_C128 ? 16 : 8; }
 }  // namespace pned

@@ -8,6 +8,7 @@
 #include <cstring>
 
 #include "internal.h"
+#include "tridiag.h"
 
 using namespace pned;
 
@@ -179,44 +180,14 @@ struct LanczosSteps {
 
 }  // namespace
 
-extern "C" {
+namespace pned {
 
-// Lowest eigenvalues and eigenvectors of the Lanczos tridiagonal T (diagonal alpha[0..n),
-// off-diagonal beta[0..n-1)).  The entry points keep the names, arguments and errors of the
-// reference's bisec / vec12 (S_one_half/Hybrid_TPS_basic.f90:25-213); the arithmetic is this
-// library's own: Sturm counts from the pivots of the LDL^T factorisation of T - x I, each
-// eigenvalue bisected down to adjacent doubles, eigenvectors by inverse iteration through a
-// pivoted LU of T - E I.  The stop step of lnc1z (a 1e-13 relative change test) depends on
-// rounding at that level, so it can differ from the reference's by a check or two.
-
-namespace {
-
-// number of eigenvalues of T below x (LDL^T pivots; a vanishing pivot is moved to -pivmin)
-int sturm_count(const double *alpha, const double *b2, int n, double x, double pivmin)
+int tridiag_bounds(const double *alpha, const double *beta, int ndim, double eps, std::vector<double> &b2,
+                   TridiagBounds &tb)
 {
-    int cnt = 0;
-    double d = alpha[0] - x;
-    if (std::fabs(d) < pivmin) d = -pivmin;
-    cnt += d < 0.0;
-    for (int i = 1; i < n; ++i) {
-        d = (alpha[i] - x) - b2[i - 1] / d;
-        if (std::fabs(d) < pivmin) d = -pivmin;
-        cnt += d < 0.0;
-    }
-    return cnt;
-}
-
-}  // namespace
-
-// bisec (the reference's interface): the ne lowest eigenvalues, ascending; eps is the
-// reference's relative width at which bisection may stop (below round-off, the bisection
-// runs until the bracket holds two adjacent doubles)
-int pned_bisec(const double *alpha, const double *beta, int ndim, double *E, int ne, double eps)
-{
-    if (!alpha || !beta || !E || ndim < 1) return fail(PNED_E_ARG, "null argument");
+    if (!alpha || !beta || ndim < 1) return fail(PNED_E_ARG, "null argument");
     if (ndim > 8000) return fail(PNED_E_ARG, "#(E04) ndim given to bisec exceeds 8000");
-    if (ne > ndim || ne < 0) return fail(PNED_E_ARG, "#(E05) ne given to bisec out of range");
-    std::vector<double> b2(ndim > 1 ? ndim - 1 : 1, 0.0);
+    b2.assign(ndim > 1 ? ndim - 1 : 1, 0.0);
     double lo = alpha[0], hi = alpha[0], b2max = 0.0;
     for (int i = 0; i < ndim; ++i) {
         const double r = (i > 0 ? std::fabs(beta[i - 1]) : 0.0) + (i < ndim - 1 ? std::fabs(beta[i]) : 0.0);
@@ -228,23 +199,53 @@ int pned_bisec(const double *alpha, const double *beta, int ndim, double *E, int
         }
     }
     const double scale = std::max(std::fabs(lo), std::fabs(hi));
-    const double pivmin = std::max(std::numeric_limits<double>::min(), b2max * std::numeric_limits<double>::min());
-    const double width = std::max(scale * (eps > 0.0 ? eps : 0.0), 0.0);
-    lo -= 2.0 * std::numeric_limits<double>::epsilon() * scale + pivmin;
-    hi += 2.0 * std::numeric_limits<double>::epsilon() * scale + pivmin;
-    double left = lo;
-    for (int k = 1; k <= ne; ++k) {
-        // the k-th lowest lies in (a, b]: fewer than k eigenvalues below a, at least k below b
-        double a = left, b = hi;
-        for (int it = 0; it < 200; ++it) {
-            const double c = 0.5 * (a + b);
-            if (c <= a || c >= b || b - a <= width) break;
-            if (sturm_count(alpha, b2.data(), ndim, c, pivmin) >= k) b = c;
-            else a = c;
-        }
-        E[k - 1] = 0.5 * (a + b);
-        left = a;   // the next one is not below this bracket
-    }
+    tb.pivmin = std::max(std::numeric_limits<double>::min(), b2max * std::numeric_limits<double>::min());
+    tb.width = std::max(scale * (eps > 0.0 ? eps : 0.0), 0.0);
+    tb.lo = lo - (2.0 * std::numeric_limits<double>::epsilon() * scale + tb.pivmin);
+    tb.hi = hi + (2.0 * std::numeric_limits<double>::epsilon() * scale + tb.pivmin);
+    return PNED_OK;
+}
+
+int tridiag_clusters(const double *alpha, const double *beta, int n, const double *E, int nvec,
+                     std::vector<int> &c0, double &tiny)
+{
+    double scale = 0.0;
+    for (int i = 0; i < n; ++i)
+        scale = std::max(scale, std::fabs(alpha[i]) + (i < n - 1 ? std::fabs(beta[i]) : 0.0) +
+                                    (i > 0 ? std::fabs(beta[i - 1]) : 0.0));
+    if (scale == 0.0) scale = 1.0;
+    tiny = std::numeric_limits<double>::epsilon() * scale;
+    c0.assign(nvec, 0);
+    for (int k = 1; k < nvec; ++k) c0[k] = (std::fabs(E[k] - E[k - 1]) > 1e-10 * scale) ? k : c0[k - 1];
+    return PNED_OK;
+}
+
+}  // namespace pned
+
+extern "C" {
+
+// Lowest eigenvalues and eigenvectors of the Lanczos tridiagonal T (diagonal alpha[0..n),
+// off-diagonal beta[0..n-1)).  The entry points keep the names, arguments and errors of the
+// reference's bisec / vec12 (S_one_half/Hybrid_TPS_basic.f90:25-213); the arithmetic is this
+// library's own: Sturm counts from the pivots of the LDL^T factorisation of T - x I, each
+// eigenvalue bisected down to adjacent doubles, eigenvectors by inverse iteration through a
+// pivoted LU of T - E I.  The stop step of lnc1z (a 1e-13 relative change test) depends on
+// rounding at that level, so it can differ from the reference's by a check or two.
+
+// bisec (the reference's interface): the ne lowest eigenvalues, ascending; eps is the
+// reference's relative width at which bisection may stop (below round-off, the bisection
+// runs until the bracket holds two adjacent doubles).  Each eigenvalue is bisected from the
+// Gershgorin bounds on its own, so the eigenvalues are independent lanes on the device; a
+// bisection started elsewhere ends in the same adjacent-double bracket, or, where the stop
+// width eps*scale is reached first (eigenvalues near zero), within that width of it
+int pned_bisec(const double *alpha, const double *beta, int ndim, double *E, int ne, double eps)
+{
+    if (!alpha || !beta || !E || ndim < 1) return fail(PNED_E_ARG, "null argument");
+    if (ne > ndim || ne < 0) return fail(PNED_E_ARG, "#(E05) ne given to bisec out of range");
+    std::vector<double> b2;
+    TridiagBounds tb;
+    PNED_TRY(tridiag_bounds(alpha, beta, ndim, eps, b2, tb));
+    for (int k = 1; k <= ne; ++k) E[k - 1] = bisect_kth(alpha, b2.data(), ndim, k, tb.lo, tb.hi, tb.width, tb.pivmin);
     return PNED_OK;
 }
 
@@ -256,73 +257,14 @@ int pned_vec12(const double *alpha, const double *beta, int ndim, const double *
 {
     if (!alpha || !beta || !E || !v || ndim < 1 || nvec < 1) return fail(PNED_E_ARG, "bad argument");
     const int n = ndim;
-    double scale = 0.0;
-    for (int i = 0; i < n; ++i)
-        scale = std::max(scale, std::fabs(alpha[i]) + (i < n - 1 ? std::fabs(beta[i]) : 0.0) +
-                                    (i > 0 ? std::fabs(beta[i - 1]) : 0.0));
-    if (scale == 0.0) scale = 1.0;
-    const double tiny = std::numeric_limits<double>::epsilon() * scale;
+    std::vector<int> c0;
+    double tiny;
+    PNED_TRY(tridiag_clusters(alpha, beta, n, E, nvec, c0, tiny));
     std::vector<double> dl(n), d(n), du(n), du2(n);
     std::vector<char> sw(n);
-    int c0 = 0;   // first vector of the current cluster
-    for (int k = 0; k < nvec; ++k) {
-        if (k > 0 && std::fabs(E[k] - E[k - 1]) > 1e-10 * scale) c0 = k;
-        // LU with partial pivoting of T - E_k I: rows i, i+1 exchanged where sw[i]
-        for (int i = 0; i < n; ++i) {
-            d[i] = alpha[i] - E[k];
-            dl[i] = (i < n - 1) ? beta[i] : 0.0;
-            du[i] = (i < n - 1) ? beta[i] : 0.0;
-            du2[i] = 0.0;
-        }
-        for (int i = 0; i < n - 1; ++i) {
-            if (std::fabs(d[i]) >= std::fabs(dl[i])) {
-                sw[i] = 0;
-                if (std::fabs(d[i]) < tiny) d[i] = (d[i] < 0.0) ? -tiny : tiny;
-                const double f = dl[i] / d[i];
-                dl[i] = f;
-                d[i + 1] -= f * du[i];
-            } else {
-                sw[i] = 1;
-                const double f = d[i] / dl[i];
-                d[i] = dl[i];
-                dl[i] = f;
-                const double t = du[i];
-                du[i] = d[i + 1];
-                d[i + 1] = t - f * d[i + 1];
-                if (i < n - 2) {
-                    du2[i] = du[i + 1];
-                    du[i + 1] = -f * du[i + 1];
-                }
-            }
-        }
-        if (std::fabs(d[n - 1]) < tiny) d[n - 1] = (d[n - 1] < 0.0) ? -tiny : tiny;
-        double *x = v + (size_t)k * n;
-        for (int i = 0; i < n; ++i) x[i] = 1.0 + 0.5 * std::sin(0.7 * (double)i + 0.3 * (double)k);
-        for (int sweep = 0; sweep < 3; ++sweep) {
-            for (int i = 0; i < n - 1; ++i) {   // forward: L
-                if (sw[i]) std::swap(x[i], x[i + 1]);
-                x[i + 1] -= dl[i] * x[i];
-            }
-            for (int i = n - 1; i >= 0; --i) {   // backward: U (two super-diagonals)
-                double s = x[i];
-                if (i + 1 < n) s -= du[i] * x[i + 1];
-                if (i + 2 < n) s -= du2[i] * x[i + 2];
-                x[i] = s / d[i];
-            }
-            for (int pass = 0; pass < 2; ++pass)   // orthogonal to the cluster so far
-                for (int j = c0; j < k; ++j) {
-                    const double *y = v + (size_t)j * n;
-                    double pr = 0.0;
-                    for (int i = 0; i < n; ++i) pr += y[i] * x[i];
-                    for (int i = 0; i < n; ++i) x[i] -= pr * y[i];
-                }
-            double nrm = 0.0;
-            for (int i = 0; i < n; ++i) nrm += x[i] * x[i];
-            nrm = std::sqrt(nrm);
-            if (nrm > 0.0)
-                for (int i = 0; i < n; ++i) x[i] /= nrm;
-        }
-    }
+    for (int k = 0; k < nvec; ++k)
+        inverse_iteration(alpha, beta, n, E[k], k, c0[k], tiny, v, dl.data(), d.data(), du.data(), du2.data(),
+                          sw.data());
     return PNED_OK;
 }
 
@@ -365,7 +307,7 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
         i = e + 1;
         if (fixed_steps > 0) continue;
         if (e > 4 * ne && e % 5 == 0) {
-            PNED_TRY(pned_bisec(alpha, beta, e, ene, ne, eps));
+            PNED_TRY(pned_ctx_bisec(ctx, alpha, beta, e, ene, ne, eps));
             if (std::fabs((ebefor - ene[ne - 1]) / ene[ne - 1]) < 1e-13) {
                 *itr = e;
                 return run.finish(a1);
@@ -374,13 +316,13 @@ int pned_lnc1(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int max_itr, int nvec, 
         }
         if (e == 4 * ne) {
             eps = 1e-25;
-            PNED_TRY(pned_bisec(alpha, beta, 4 * ne, ene, ne, eps));
+            PNED_TRY(pned_ctx_bisec(ctx, alpha, beta, 4 * ne, ene, ne, eps));
             ebefor = ene[ne - 1];
         }
     }
     PNED_TRY(run.finish(a1));
     if (fixed_steps > 0) {
-        PNED_TRY(pned_bisec(alpha, beta, last, ene, std::min(ne, last), 1e-25));
+        PNED_TRY(pned_ctx_bisec(ctx, alpha, beta, last, ene, std::min(ne, last), 1e-25));
         *itr = last;
         return PNED_OK;
     }
@@ -428,7 +370,7 @@ int pned_lnc1_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int first, int sto
         for (int i = start - 1; i >= 2 && prev < 0; --i)
             if ((i > 4 * ne && i % 5 == 0) || i == 4 * ne) prev = i;
         if (prev > 0) {
-            PNED_TRY(pned_bisec(alpha, beta, prev, ene, ne, eps));
+            PNED_TRY(pned_ctx_bisec(ctx, alpha, beta, prev, ene, ne, eps));
             ebefor = ene[ne - 1];
         }
     }
@@ -447,7 +389,7 @@ int pned_lnc1_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int first, int sto
         }
         if (fixed_steps > 0) continue;
         if (i > 4 * ne && i % 5 == 0) {
-            PNED_TRY(pned_bisec(alpha, beta, i, ene, ne, eps));
+            PNED_TRY(pned_ctx_bisec(ctx, alpha, beta, i, ene, ne, eps));
             if (std::fabs((ebefor - ene[ne - 1]) / ene[ne - 1]) < 1e-13) {
                 *itr = i;
                 return PNED_OK;
@@ -455,12 +397,12 @@ int pned_lnc1_span(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, int first, int sto
             ebefor = ene[ne - 1];
         }
         if (i == 4 * ne) {
-            PNED_TRY(pned_bisec(alpha, beta, 4 * ne, ene, ne, eps));
+            PNED_TRY(pned_ctx_bisec(ctx, alpha, beta, 4 * ne, ene, ne, eps));
             ebefor = ene[ne - 1];
         }
     }
     *itr = last;
-    if (fixed_steps > 0) return pned_bisec(alpha, beta, last, ene, std::min(ne, last), eps);
+    if (fixed_steps > 0) return pned_ctx_bisec(ctx, alpha, beta, last, ene, std::min(ne, last), eps);
     return fail(PNED_E_NOCONV, "#(W07) lnc1 did not converge within " + std::to_string(last) + " steps");
 }
 
@@ -522,7 +464,7 @@ static int lanczos_reorth(pned_ctx *ctx, pned_vec *const *basis, int nbasis, pne
         PNED_TRY(pned_scale(ctx, basis[i], sc));
         if (fixed > 0) continue;
         if (i >= 4 * ne && i % 5 == 0) {
-            PNED_TRY(pned_bisec(alpha, beta, i, ene, ne, 1e-25));
+            PNED_TRY(pned_ctx_bisec(ctx, alpha, beta, i, ene, ne, 1e-25));
             if (i > 4 * ne && std::fabs((ebefor - ene[ne - 1]) / ene[ne - 1]) < 1e-13) {
                 *itr = i;
                 return PNED_OK;
@@ -531,7 +473,7 @@ static int lanczos_reorth(pned_ctx *ctx, pned_vec *const *basis, int nbasis, pne
         }
     }
     *itr = last;
-    if (fixed > 0) return pned_bisec(alpha, beta, last, ene, std::min(ne, last), 1e-25);
+    if (fixed > 0) return pned_ctx_bisec(ctx, alpha, beta, last, ene, std::min(ne, last), 1e-25);
     return fail(PNED_E_NOCONV, "reorthogonalised Lanczos did not converge within " + std::to_string(last) + " steps");
 }
 

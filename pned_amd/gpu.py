@@ -108,6 +108,8 @@ def lib():
     L.pned_dsflnc1z_span.argtypes = [P, P, P, f64, i32, i32, i32, _dp, _dp, C.POINTER(i32)]
     L.pned_bisec.argtypes = [_dp, _dp, i32, _dp, i32, f64]
     L.pned_vec12.argtypes = [_dp, _dp, i32, _dp, i32, _dp]
+    L.pned_ctx_bisec.argtypes = [P, _dp, _dp, i32, _dp, i32, f64]
+    L.pned_ctx_vec12.argtypes = [P, _dp, _dp, i32, _dp, i32, _dp]
     L.pned_ritz_vector.argtypes = [P, P, P, P, i32, _dp, _dp, _dp]
     L.pned_inv1.argtypes = [P, f64, P, P, P, P, P, i32, i32, i32, C.POINTER(i32)]
     L.pned_cocg.argtypes = [P, f64, f64, P, P, P, P, P, i32, f64, C.POINTER(i32), C.POINTER(f64)]
@@ -278,6 +280,22 @@ class Context:
         k, l, t, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         check(lib().pned_ctx_autotune(self.h, v1.h, v0.h, reps, C.byref(k), C.byref(l), C.byref(t), C.byref(ms)))
         return {"tile_bits": k.value, "low_bits": l.value, "threads": t.value, "ms": ms.value}
+
+    def bisec(self, alpha, beta, ne, eps=1e-25):
+        """pned_bisec's eigenvalues, computed on this context's GPU (the same doubles)."""
+        a = np.ascontiguousarray(alpha, dtype=float)
+        b = np.ascontiguousarray(beta, dtype=float)
+        E = np.zeros(max(ne, 1))
+        check(lib().pned_ctx_bisec(self.h, a, b, len(a), E, ne, eps))
+        return E[:ne]
+
+    def vec12(self, alpha, beta, E, nvec):
+        """pned_vec12's eigenvectors, computed on this context's GPU (the same doubles)."""
+        a = np.ascontiguousarray(alpha, dtype=float)
+        b = np.ascontiguousarray(beta, dtype=float)
+        v = np.zeros(len(a) * nvec)
+        check(lib().pned_ctx_vec12(self.h, a, b, len(a), np.ascontiguousarray(E, dtype=float), nvec, v))
+        return v.reshape(nvec, len(a))
 
     def graphs(self, enable: bool):
         """HIP graph replay of the one-column S=1/2 Lanczos steps (lnc1, dsflnc1z); on by default."""
