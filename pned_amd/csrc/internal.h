@@ -289,8 +289,9 @@ int lanczos_lazy_restore(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, bool swapped
 // loops capture into graphs: st = [sx, sy, prd re, prd im, b2 of the step before] on the
 // device; the step sets sx = 1/sqrt(b2), sy = -sqrt(b2) sx (fresh: 1, 0), runs the step and
 // writes [prd re, prd im, b2] to slot and b2 to st[4].  No allocation or host wait once
-// lanczos_lazy_setup and one un-captured step of the same shape have run.
-int lanczos_lazy_setup(pned_ctx *ctx, const pned_vec *x);
+// lanczos_lazy_setup (which also checks the pair) and one un-captured step of the same
+// shape have run.
+int lanczos_lazy_setup(pned_ctx *ctx, const pned_vec *x, const pned_vec *y);
 int lanczos_lazy_enqueue(pned_ctx *ctx, pned_vec *x, pned_vec *y, bool fresh, double *st, double *slot);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
 // returns the context to the identity layout (re-decomposes its operator); only while no

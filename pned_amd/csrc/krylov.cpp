@@ -98,7 +98,7 @@ struct LanczosSteps {
     int first(double &a1, double &b1)
     {
         if (dev) {
-            PNED_TRY(lanczos_lazy_setup(ctx, X));
+            PNED_TRY(lanczos_lazy_setup(ctx, X, Y));
             PNED_HIP(hipMalloc((void **)&st, sizeof(double) * (5 + 3 * GRAPH_STEPS)));
             PNED_HIP(hipHostMalloc((void **)&host, sizeof(double) * 3 * GRAPH_STEPS, hipHostMallocDefault));
             PNED_TRY(lanczos_lazy_enqueue(ctx, X, Y, true, st, st + 5));

@@ -963,14 +963,16 @@ __global__ void k_lz_store(const double *__restrict__ result, double *__restrict
     st[4] = b2;
 }
 
-int lanczos_lazy_setup(pned_ctx *ctx, const pned_vec *x)
+int lanczos_lazy_setup(pned_ctx *ctx, const pned_vec *x, const pned_vec *y)
 {
+    PNED_TRY(check_hv_pair(ctx, x, y));
     PNED_TRY(ensure_result(ctx, 3 * (size_t)x->nvec + 8));
     return ensure_partial(ctx, (size_t)RED_GRID * x->nvec);
 }
 
 int lanczos_lazy_enqueue(pned_ctx *ctx, pned_vec *x, pned_vec *y, bool fresh, double *st, double *slot)
 {
+    PNED_TRY(check_hv_pair(ctx, x, y));
     if (x->nvec != 1 || ctx->q != 2 || ctx->sector_k >= 0)
         return fail(PNED_E_UNSUPPORTED, "captured Lanczos step: one column of an S=1/2 full-space context");
     hipLaunchKernelGGL(k_lz_scale, dim3(1), dim3(64), 0, ctx->stream, st, fresh ? 1 : 0);
@@ -1524,6 +1526,8 @@ int tps_graph_setup(pned_ctx *ctx, const pned_vec *u, void **dobs, int *nsq)
 int tps_graph_step(pned_ctx *ctx, const pned_vec *up, const pned_vec *u, pned_vec *w, const void *dobs, int nsq,
                    const double *red_prev, double *su_dev, double *slot)
 {
+    PNED_TRY(check_hv_pair(ctx, u, w));
+    PNED_TRY(check_hv_pair(ctx, up, u));
     const int nc = u->nvec;
     hipLaunchKernelGGL(k_tps_scale, dim3((nc + 63) / 64), dim3(64), 0, ctx->stream, red_prev, nc, su_dev);
     PNED_HIP(hipGetLastError());
