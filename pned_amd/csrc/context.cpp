@@ -950,6 +950,54 @@ static void build_diag_program(DiagDev &dh)
     dh.nprog = e;
 }
 
+bool split_diag_slots(const DiagDev &dh, uint32_t slot_bits, DiagDev &out)
+{
+    if (dh.nprog < 1) return false;
+    int ncls = 0;
+    for (int e = 0; e < dh.nprog; ++e) ncls = std::max(ncls, (int)((dh.prog_a[e] >> 8) & 7u) + 1);
+    if (ncls > 4) return false;
+    std::vector<uint32_t> sa, sb, ra, rb;
+    for (int c = 0; c < ncls; ++c) {
+        const size_t s0 = sa.size(), r0 = ra.size();
+        for (int e = 0; e < dh.nprog; ++e) {
+            const uint32_t pa = dh.prog_a[e] & ~128u;
+            if ((int)((pa >> 8) & 7u) != c) continue;
+            const uint32_t sh = pa & 63u, m = dh.prog_b[e];
+            // positions b of the mask whose bit b, or bit b + sh of a pair, is a register bit
+            const uint32_t touch = slot_bits | ((pa & 64u) ? (uint32_t)((uint64_t)slot_bits >> sh) : 0u);
+            if (m & touch) {
+                sa.push_back(pa);
+                sb.push_back(m & touch);
+            }
+            if (m & ~touch) {
+                ra.push_back(pa);
+                rb.push_back(m & ~touch);
+            }
+        }
+        if (sa.size() == s0) {   // the class still ends in the slot program
+            sa.push_back((uint32_t)c << 8);
+            sb.push_back(0u);
+        }
+        sa.back() |= 128u;
+        if (ra.size() > r0) ra.back() |= 128u;
+    }
+    if (sa.size() + ra.size() > 48) return false;
+    out = dh;
+    std::memset(out.prog_a, 0, sizeof(uint32_t) * 48);
+    std::memset(out.prog_b, 0, sizeof(uint32_t) * 48);
+    for (size_t i = 0; i < sa.size(); ++i) {
+        out.prog_a[i] = sa[i];
+        out.prog_b[i] = sb[i];
+    }
+    for (size_t i = 0; i < ra.size(); ++i) {
+        out.prog_a[sa.size() + i] = ra[i];
+        out.prog_b[sa.size() + i] = rb[i];
+    }
+    out.nprog = (int)sa.size();
+    out.nrest = (int)ra.size();
+    return true;
+}
+
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
 {
     if (ctx->plan_colbits == colbits && ctx->plan_dtype == dtype) return PNED_OK;
@@ -1168,6 +1216,7 @@ int pned_ctx_destroy(pned_ctx *ctx)
     for (pned_vec *v : ctx->vecs) v->ctx = nullptr;   // they stay valid for pned_vec_destroy
     if (ctx->d_terms) (void)hipFree(ctx->d_terms);
     if (ctx->d_diag) (void)hipFree(ctx->d_diag);
+    if (ctx->d_diag_tm) (void)hipFree(ctx->d_diag_tm);
     if (ctx->pd_dev) (void)hipFree(ctx->pd_dev);
     if (ctx->d_tm) (void)hipFree(ctx->d_tm);
     free_sector(ctx);

@@ -363,6 +363,7 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
     // the diagonal as a lane program (see DiagDev): group e in lane e, class weights in
     // lanes 48..63; one operator only (batch contexts weight per column)
     const int dprog = (a.diag && a.nop <= 1) ? diagp->nprog : 0;
+    const int drest = dprog ? diagp->nrest : 0;   // split program: lanes dprog.. counted per tile
     const uint32_t dpa = dprog ? diagp->prog_a[tid & 63] : 0u;
     const uint32_t dpb = dprog ? diagp->prog_b[tid & 63] : 0u;
 
@@ -405,6 +406,22 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
         if (rd0) fetch_y(tb0, 0, ycr, yci);
         const uint64_t nx = tix + 1;
         if (nx < tend) fetch_x(outer_base(nx) | offLo);
+        // the diagonal's per-class counts over the bit positions that no register bit takes
+        // part in: the same for every slot of this thread and tile (16 bits per class)
+        uint64_t rcnt = 0;
+        if (drest) {
+            const uint32_t s0 = (uint32_t)(tb0 >> a.colbits);
+            uint32_t acc = 0;
+            for (int e = dprog; e < dprog + drest; ++e) {
+                const uint32_t pa = (uint32_t)__builtin_amdgcn_readlane((int)dpa, e);
+                const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)dpb, e);
+                acc += (uint32_t)__popc(((pa & 64u) ? (s0 ^ (s0 >> (pa & 63u))) : s0) & m);
+                if (pa & 128u) {
+                    rcnt |= (uint64_t)acc << (16u * ((pa >> 8) & 7u));
+                    acc = 0u;
+                }
+            }
+        }
 
         static_for<NS>([&](auto HI) {
             constexpr int r0 = HI * S;
@@ -435,6 +452,7 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
                     }
                     if (pa & 128u) {
                         const int c = (int)((pa >> 8) & 7u);
+                        const uint32_t rc = (uint32_t)(rcnt >> (16 * c)) & 0xFFFFu;
                         const uint64_t wrb = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)dpa, 48 + c) |
                                              ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)dpa, 56 + c) << 32);
                         const uint64_t wib = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)dpb, 48 + c) |
@@ -442,8 +460,8 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
                         const double wr = __longlong_as_double((long long)wrb);
                         const double wi = __longlong_as_double((long long)wib);
                         static_for<S>([&](auto I) {
-                            dr[I] = fma((double)cnt[I], wr, dr[I]);
-                            di[I] = fma((double)cnt[I], wi, di[I]);
+                            dr[I] = fma((double)(cnt[I] + rc), wr, dr[I]);
+                            di[I] = fma((double)(cnt[I] + rc), wi, di[I]);
                             cnt[I] = 0u;
                         });
                     }
@@ -911,6 +929,20 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
         PNED_HIP(hipMemcpy(ctx->pd_dev, host.data(), sizeof(PassDev) * npass, hipMemcpyHostToDevice));
         PNED_HIP(hipMalloc((void **)&ctx->d_tm, sizeof(TmTerm) * tmh.size()));
         PNED_HIP(hipMemcpy(ctx->d_tm, tmh.data(), sizeof(TmTerm) * tmh.size(), hipMemcpyHostToDevice));
+        // the diagonal pass's lane program split by that pass's register bits
+        if (ctx->d_diag_tm) PNED_HIP(hipFree(ctx->d_diag_tm));
+        ctx->d_diag_tm = nullptr;
+        for (int p = 0; p < npass && pl.nop <= 1 && ctx->diag_host.nprog > 0; ++p) {
+            const PassPlan &pp = pl.passes[p];
+            if (!pp.diag) continue;
+            uint32_t slot_bits = 0;
+            for (int k = __builtin_ctz(tpb[p]); k < pp.K; ++k) slot_bits |= 1u << (pp.tb[k] - colbits);
+            DiagDev dt;
+            if (split_diag_slots(ctx->diag_host, slot_bits, dt)) {
+                PNED_HIP(hipMalloc((void **)&ctx->d_diag_tm, sizeof(DiagDev)));
+                PNED_HIP(hipMemcpy(ctx->d_diag_tm, &dt, sizeof(DiagDev), hipMemcpyHostToDevice));
+            }
+        }
         ctx->pd_threads = TPBpref;
     }
     int total_rows = 0;
@@ -944,12 +976,13 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
         const size_t tile_bytes = (c128 ? 16 : 8) * (size_t)TPB * R;
         const bool tm = ctx->s12_kernel == 1 && (c128 ? has_tm<double>(TPB, R) : has_tm<float>(TPB, R)) &&
                         tile_bytes + tm_coef_bytes(a) <= LDS_LIMIT;
+        const DiagDev *dgp = (pp.diag && ctx->d_diag_tm) ? ctx->d_diag_tm : ctx->d_diag;
         if (!noalloc && lz_fused && p == npass - 1 && p > 0 && tm && c128 && nvec == 1 && pl.nop <= 1) {
             // deferred-normalisation Lanczos step: this pass's <v1|H v1> share first (no v0
             // traffic), alpha from all passes on the device, then the pass itself forms
             // v0 - Re(alpha) s v1 and the partials of beta^2
             a.lz_mode = 1;
-            PNED_TRY(dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid));
+            PNED_TRY(dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, dgp, pdp, a, part, grid));
             total_rows += grid;
             hipLaunchKernelGGL(reduce_partials, dim3(2 * nvec), dim3(256), 0, ctx->stream, ctx->d_partial, total_rows,
                                2 * nvec, ctx->d_result, 0);
@@ -957,7 +990,7 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
             a.lz_mode = 2;
             a.lz_prd = ctx->d_result;
             part = ctx->d_partial + (size_t)total_rows * 2 * nvec;
-            PNED_TRY(dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid));
+            PNED_TRY(dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, dgp, pdp, a, part, grid));
             hipLaunchKernelGGL(reduce_partials, dim3(2 * nvec), dim3(256), 0, ctx->stream, part, grid, 2 * nvec,
                                ctx->d_result + 2 * nvec, 0);
             PNED_HIP(hipGetLastError());
@@ -967,9 +1000,9 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
         if (tm) {
             // the term table is in pass order, so the kernel indexes it by first_term as well
             if (c128)
-                rc = dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid);
+                rc = dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, dgp, pdp, a, part, grid);
             else
-                rc = dispatch_tm<float>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, ctx->d_diag, pdp, a, part, grid);
+                rc = dispatch_tm<float>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, dgp, pdp, a, part, grid);
         } else if (c128) {
             rc = dispatch_pass<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_diag, pdp, a, part, grid);
         } else {

@@ -174,6 +174,12 @@ struct DiagDev {             // flattened diagonal for the kernel (constant per 
     // nprog = 0: not available (more than 48 groups or 8 weight classes).
     int nprog;
     uint32_t prog_a[64], prog_b[64];
+    // nrest > 0 (the term-major kernel's diagonal pass): the program split by the pass's
+    // register bits -- lanes 0..nprog-1 keep, per class, the bit positions that involve a
+    // register bit (every class ends there, with an empty mask where it has none), lanes
+    // nprog..nprog+nrest-1 the positions that do not, which are the same for all register
+    // slots of a thread and are counted once per tile (at most 4 classes: 16-bit counts)
+    int nrest;
 };
 
 // ------------------------------------------------------------------------------------
@@ -219,6 +225,7 @@ struct pned_ctx {
     pned::DevTerm *d_terms = nullptr;
     size_t d_terms_cap = 0;
     pned::DiagDev *d_diag = nullptr;
+    pned::DiagDev *d_diag_tm = nullptr;     // diagonal program split by the diag pass's register bits
     void *pd_dev = nullptr;          // per-pass address tables of the current plan
     pned::TmTerm *d_tm = nullptr;    // per-pass tile-coordinate terms (term-major kernel)
     std::vector<int> tm_ncat;        // [pass][4] terms per term-major category
@@ -294,6 +301,10 @@ int lanczos_lazy_restore(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, bool swapped
 int lanczos_lazy_setup(pned_ctx *ctx, const pned_vec *x, const pned_vec *y);
 int lanczos_lazy_enqueue(pned_ctx *ctx, pned_vec *x, pned_vec *y, bool fresh, double *st, double *slot);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
+// dh's lane program split by register bits (slot_bits: state bits held in register slots,
+// in the diagonal's state coordinates); false when it does not fit (more than 4 classes or
+// 48 lanes), then out is unchanged
+bool split_diag_slots(const DiagDev &dh, uint32_t slot_bits, DiagDev &out);
 // returns the context to the identity layout (re-decomposes its operator); only while no
 // vector of the context holds data in the old layout matters to the caller
 int reset_layout(pned_ctx *ctx);
