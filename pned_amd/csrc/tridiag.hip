@@ -87,10 +87,10 @@ extern "C" int pned_ctx_bisec(pned_ctx *ctx, const double *alpha, const double *
 {
     if (!ctx || !alpha || !beta || !E || ndim < 1) return fail(PNED_E_ARG, "null argument");
     if (ne > ndim || ne < 0) return fail(PNED_E_ARG, "#(E05) ne given to bisec out of range");
-    if (ne == 0) return PNED_OK;
     std::vector<double> b2;
     TridiagBounds tb;
     PNED_TRY(tridiag_bounds(alpha, beta, ndim, eps, b2, tb));
+    if (ne == 0) return PNED_OK;
     PNED_HIP(hipSetDevice(ctx->device));
     // device: alpha (ndim), b2 (ndim - 1, at least 1), E (ne)
     const size_t nb2 = b2.size();
@@ -110,7 +110,6 @@ extern "C" int pned_ctx_vec12(pned_ctx *ctx, const double *alpha, const double *
                               int nvec, double *v)
 {
     if (!ctx || !alpha || !beta || !E || !v || ndim < 1 || nvec < 1) return fail(PNED_E_ARG, "bad argument");
-    if (ndim > 8000) return fail(PNED_E_ARG, "ndim exceeds 8000");
     const int n = ndim;
     std::vector<int> c0;
     double tiny;
@@ -126,7 +125,7 @@ extern "C" int pned_ctx_vec12(pned_ctx *ctx, const double *alpha, const double *
     const size_t nbeta = (size_t)n, nwork = (size_t)ncl * 4 * n;
     const size_t nint = ((size_t)(ncl + 1) * sizeof(int) + 7) / 8, nsw = ((size_t)ncl * n + 7) / 8;
     const size_t total = (size_t)n + nbeta + nvec + (size_t)nvec * n + nwork + nint + nsw;
-    if (total > (size_t)1 << 30) return fail(PNED_E_ARG, "vec12: too many vectors for the work buffer");
+    if (total > (size_t)1 << 30) return fail(PNED_E_NOMEM, "vec12: more than 8 GiB of device work space");
     double *d;
     PNED_TRY(ensure_scalars(ctx, (int)total, &d));
     double *dal = d, *dbe = dal + n, *dE = dbe + nbeta, *dv = dE + nvec, *dw = dv + (size_t)nvec * n;

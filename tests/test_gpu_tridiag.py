@@ -83,9 +83,15 @@ def test_lanczos_checks_use_device_bisection(ctx):
 def test_device_tridiagonal_argument_errors(ctx):
     _say("tridiagonal errors")
     a, b = np.ones(5), np.ones(5)
-    for call in (lambda: ctx.bisec(a, b, 6), lambda: ctx.bisec(np.ones(8001), np.ones(8001), 1),
-                 lambda: ctx.vec12(np.ones(8001), np.ones(8001), np.zeros(1), 1)):
+    big = np.ones(8001)
+    for call in (lambda: ctx.bisec(a, b, 6), lambda: ctx.bisec(big, big, 1), lambda: ctx.bisec(big, big, 0),
+                 lambda: gpu.bisec(big, big, 0), lambda: ctx.vec12(a, b, np.zeros(0), 0)):
         with pytest.raises(gpu.PnedError) as err:
             call()
         assert err.value.code == gpu.E_ARG
     assert np.array_equal(ctx.bisec(a, b, 2), gpu.bisec(a, b, 2))
+    # inverse iteration takes any size, as the host's does
+    rng = np.random.default_rng(11)
+    a2, b2 = rng.standard_normal(9000), rng.random(9000) + 0.05
+    e2 = gpu.bisec(a2[:8000], b2[:8000], 2)
+    assert np.array_equal(ctx.vec12(a2, b2, e2, 2), gpu.vec12(a2, b2, e2, 2))
