@@ -432,11 +432,15 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
             if (dprog) {
                 // all S slots at once, group-outer: per group two readlanes, per slot a shift,
                 // an xor and an and with a popcount; per weight class one convert and FMA pair
-                uint32_t st[S], cnt[S];
+                // a slot's state is sb | hs[I] (disjoint bits), and x -> x ^ (x >> sh) is linear
+                // over xor, so a group's bits of slot I are its bits of sb (per thread, formed
+                // once per group) xor those of hs[I] (uniform, on the scalar unit)
+                uint32_t cnt[S], hs[S];
                 double dr[S], di[S];
                 const double d0r = diagp->c0r + a.shift, d0i = diagp->c0i;
+                const uint32_t sb = (uint32_t)(tb0 >> a.colbits);
                 static_for<S>([&](auto I) {
-                    st[I] = (uint32_t)((tb0 | pd->hi[r0 + I]) >> a.colbits);
+                    hs[I] = (uint32_t)(pd->hi[r0 + I] >> a.colbits);
                     cnt[I] = 0u;
                     dr[I] = d0r;
                     di[I] = d0i;
@@ -446,9 +450,13 @@ __global__ __launch_bounds__(TPB) void hv_s12_tm(const typename C2<Real>::T *__r
                     const uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)dpb, e);
                     const uint32_t sh = pa & 63u;
                     if (pa & 64u) {
-                        static_for<S>([&](auto I) { cnt[I] += (uint32_t)__popc((st[I] ^ (st[I] >> sh)) & m); });
+                        const uint32_t f = (sb ^ (sb >> sh)) & m;
+                        static_for<S>([&](auto I) {
+                            cnt[I] += (uint32_t)__popc(f ^ ((hs[I] ^ (hs[I] >> sh)) & m));
+                        });
                     } else {
-                        static_for<S>([&](auto I) { cnt[I] += (uint32_t)__popc(st[I] & m); });
+                        const uint32_t f = sb & m;
+                        static_for<S>([&](auto I) { cnt[I] += (uint32_t)__popc(f ^ (hs[I] & m)); });
                     }
                     if (pa & 128u) {
                         const int c = (int)((pa >> 8) & 7u);
