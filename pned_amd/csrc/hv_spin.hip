@@ -890,9 +890,15 @@ extern "C" int pned_spin_plan_export(int n_sites, int q, int ibond, const int32_
     return PNED_OK;
 }
 
+// Launches one pass on a grid of resident blocks (compute units x the blocks of this
+// instance that fit on one at once, at most grid_cap and the tile count), so every block
+// lives for the whole pass and walks its tiles with the next tile's loads in flight; the
+// XCD-local tile order needs the grid to be a multiple of 8.  Spin-1 N=18: 256 blocks (one
+// per CU) 19.38-19.40 ms per H.v against 20.27 with 2048, 20.06 with 1024, 19.87-20.03 with
+// 512 and 24.4 with 384 (a second, partial round of blocks).  Returns the grid used.
 template <int Q, int TPB, int R>
-static void launch_spin_pass(pned_ctx *ctx, int grid, size_t lds, bool cplx, const pned_vec *v1, pned_vec *v0,
-                             const SpinPassDev *pd, const SpinPassArgs &a, double *part)
+static int launch_spin_pass(pned_ctx *ctx, int grid_cap, size_t lds, bool cplx, const pned_vec *v1, pned_vec *v0,
+                            const SpinPassDev *pd, SpinPassArgs a, double *part)
 {
     // slots per sweep: only the last sweep may be partial (the instances' tiles fill all but
     // the last slot), and 7 slots of 1024 threads keep 2 (real) or 1 (complex coefficients)
@@ -901,8 +907,19 @@ static void launch_spin_pass(pned_ctx *ctx, int grid, size_t lds, bool cplx, con
     constexpr int SC = (R == 7) ? 1 : S;
     auto k = cplx ? hv_spin_pass<Q, TPB, R, SC, true> : hv_spin_pass<Q, TPB, R, S, false>;
     (void)hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (ctx->num_cu <= 0 &&
+        hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess)
+        ctx->num_cu = 0;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k, TPB, lds) != hipSuccess) per_cu = 0;
+    uint64_t want = (ctx->num_cu > 0 && per_cu > 0) ? (uint64_t)ctx->num_cu * per_cu : (uint64_t)grid_cap;
+    want = std::min<uint64_t>(std::min<uint64_t>(want, (uint64_t)grid_cap), a.tile_count);
+    if (want >= 8) want &= ~(uint64_t)7;
+    const int grid = (int)std::max<uint64_t>(want, 1);
+    a.xcd = (grid % 8 == 0) ? 1 : 0;
     hipLaunchKernelGGL(k, dim3(grid), dim3(TPB), lds, ctx->stream, (const double2 *)v1->data, (double2 *)v0->data,
                        pd, a, part);
+    return grid;
 }
 
 int launch_hv_spin_tiled(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double eperbd, bool fresh,
@@ -956,8 +973,7 @@ int launch_hv_spin_tiled_shift(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, 
         a.zr = zr;
         a.zi = zi;
         a.dotu = dotu ? 1 : 0;
-        const int grid = (int)std::min<uint64_t>(a.tile_count, (uint64_t)grid_cap);
-        a.xcd = (grid % 8 == 0) ? 1 : 0;
+        int grid = 0;
         const SpinPassDev *pd = reinterpret_cast<const SpinPassDev *>(ctx->spin_pd) + p;
         double *part = ctx->d_partial + 2 * (size_t)rows;
         // 1024 threads for tiles above 2048 states, else 256; R = ceil(tsz / threads) slots
@@ -967,7 +983,7 @@ int launch_hv_spin_tiled_shift(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, 
         const size_t lds = spin_lds_bytes(a.tsz, a.ncls, qq, cplx, tpb);
 #define PNED_SPIN_CASE(QV, TV, RV)                                                                                   \
         if (q == QV && tpb == TV && R == RV) {                                                                      \
-            launch_spin_pass<QV, TV, RV>(ctx, grid, lds, cplx, v1, v0, pd, a, part);                                \
+            grid = launch_spin_pass<QV, TV, RV>(ctx, grid_cap, lds, cplx, v1, v0, pd, a, part);                     \
         } else
         // every tile size q^K (q^K <= 8192) maps to one of these
         PNED_SPIN_CASE(3, 1024, 7) PNED_SPIN_CASE(3, 1024, 3) PNED_SPIN_CASE(3, 256, 3) PNED_SPIN_CASE(3, 256, 1)

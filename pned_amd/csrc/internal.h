@@ -251,6 +251,7 @@ struct pned_ctx {
     // spin-S
     pned::GpuTables spin;
     int spin_tiled = 1;              // tiled kernel (q = 3..5); 0 = per-state gather kernel
+    int num_cu = 0;                  // compute units of the device (read on first use)
     int use_graphs = 1;              // Krylov loops replay captured HIP graphs of their steps
     void *spin_pd = nullptr;         // tiled-pass tables
     int spin_K = -1, spin_np = 0;
