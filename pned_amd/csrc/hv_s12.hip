@@ -700,6 +700,9 @@ static size_t tm_coef_bytes(const PassArgs &a)
     return sizeof(double) * ((size_t)4 * a.nterm * a.nop + (a.diag ? (size_t)2 * (a.ngroups + 1) * a.nop : 0));
 }
 constexpr size_t LDS_LIMIT = 160 * 1024;
+// a batched tile of a single-operator context holds the columns of one 256-byte run (32
+// complex64 or 16 complex128 values per state); more columns run in groups
+constexpr size_t COL_GROUP_BYTES = 256;
 
 template <typename Real, int TPB, int R, int S>
 static int launch_tm(pned_ctx *ctx, const void *v1, void *v0, const TmTerm *tm, const DiagDev *ddiag,
@@ -753,8 +756,10 @@ static int dispatch_tm(pned_ctx *ctx, int TPB, int R, const void *v1, void *v0, 
 // Tile-coordinate term table of one pass (see TmTerm), sorted into the term-major
 // kernel's categories: 0 real, 1 real with register selector bits, 2 complex, 3 complex
 // with register selector bits (batch contexts: every term complex).  ncat gets the counts.
-static void build_tm_terms(const S12Plan &pl, const PassPlan &pp, int TPB, int colbits, std::vector<TmTerm> &out,
-                           int ncat[4])
+// colbits: the plan's column bits; outer selector bits at or above them sit gap bits higher
+// in memory (column groups)
+static void build_tm_terms(const S12Plan &pl, const PassPlan &pp, int TPB, int colbits, int gap,
+                           std::vector<TmTerm> &out, int ncat[4])
 {
     const int LT = __builtin_ctz(TPB);
     uint64_t tilemask = 0;
@@ -771,7 +776,8 @@ static void build_tm_terms(const S12Plan &pl, const PassPlan &pp, int TPB, int c
             if (selm >> pp.tb[k] & 1) stile |= 1u << k;
         t.slo = stile & (uint32_t)(TPB - 1);
         t.shi = stile >> LT;
-        t.sout = selm & ~tilemask;
+        const uint64_t so = selm & ~tilemask;
+        t.sout = (so & ((1ull << colbits) - 1)) | ((so >> colbits) << (colbits + gap));
         const bool cx = !d.is_real || pl.nop > 1;
         t.flags = (cx ? 1u : 0u) | ((t.shi != 0) ? 2u : 0u);
         t.term = (uint32_t)(pp.first_term + i);
@@ -848,12 +854,21 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
     while ((1 << colbits) < nvec) ++colbits;
     if ((1 << colbits) != nvec)
         return fail(PNED_E_UNSUPPORTED, "batched H.v needs a power-of-two column count");
+    // A single-operator batch wider than one 256-byte run of columns runs in column groups: the
+    // plan's tiles hold that run's column bits, the higher column bits are fixed per launch
+    // (one launch per group and pass, base pointers moved by the group), so a tile spends its
+    // other bits on sites and a pass covers more terms (N=24 with 64 columns: 4 -> 3 passes).
+    // Plan positions at or above cg move up by gap bits in memory (expand).
+    const int run_bits = __builtin_ctzll(COL_GROUP_BYTES / (v1->dtype == PNED_C128 ? 16 : 8));
+    const int cg = (ctx->s12.nop <= 1 && colbits > run_bits) ? run_bits : colbits;
+    const int gap = colbits - cg, ngrp = 1 << gap, ncol_t = 1 << cg;
+    auto expand = [cg, gap](int b) { return b < cg ? b : b + gap; };
     if (noalloc) {
-        if (ctx->plan_colbits != colbits || ctx->plan_dtype != v1->dtype || ctx->pd_dev == nullptr)
+        if (ctx->plan_colbits != cg || ctx->pd_gap != gap || ctx->plan_dtype != v1->dtype || ctx->pd_dev == nullptr)
             return fail(PNED_E_ARG, "no H.v plan for this vector shape yet: call pned_p_mltply_v3 once with "
                                     "vectors of this shape first (that call allocates the plan's tables)");
     } else {
-        PNED_TRY(ensure_s12_plan(ctx, colbits, v1->dtype));
+        PNED_TRY(ensure_s12_plan(ctx, cg, v1->dtype));
     }
     const S12Plan &pl = ctx->s12;
     if (pl.nop > 1 && pl.nop != nvec)
@@ -891,7 +906,7 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
     if (noalloc && ctx->pd_threads != TPBpref)
         return fail(PNED_E_ARG, "no H.v plan for this thread count yet: call pned_p_mltply_v3 once after "
                                 "changing the tuning (that call rebuilds the plan's tables)");
-    if (ctx->pd_threads != TPBpref || ctx->pd_dev == nullptr) {
+    if (ctx->pd_threads != TPBpref || ctx->pd_gap != gap || ctx->pd_dev == nullptr) {
         std::vector<PassDev> host(npass);
         for (int p = 0; p < npass; ++p) {
             const PassPlan &pp = pl.passes[p];
@@ -901,32 +916,33 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
             for (int t = 0; t < tpb[p]; ++t) {
                 uint64_t o = 0;
                 for (int k = 0; k < LT; ++k)
-                    if ((t >> k) & 1) o |= 1ull << pp.tb[k];
+                    if ((t >> k) & 1) o |= 1ull << expand(pp.tb[k]);
                 d.lo[t] = o;
             }
             for (int r = 0; r < rr[p]; ++r) {
                 uint64_t o = 0;
                 for (int k = 0; LT + k < pp.K; ++k)
-                    if ((r >> k) & 1) o |= 1ull << pp.tb[LT + k];
+                    if ((r >> k) & 1) o |= 1ull << expand(pp.tb[LT + k]);
                 d.hi[r] = o;
             }
             const int no = (int)pp.ob.size(), nlo = no < 11 ? no : 11, nhi = no - nlo;
             for (int x = 0; x < (1 << nlo); ++x) {
                 uint64_t o = 0;
                 for (int k = 0; k < nlo; ++k)
-                    if ((x >> k) & 1) o |= 1ull << pp.ob[k];
+                    if ((x >> k) & 1) o |= 1ull << expand(pp.ob[k]);
                 d.blo[x] = o;
             }
             for (int x = 0; x < (1 << nhi); ++x) {
                 uint64_t o = 0;
                 for (int k = 0; k < nhi; ++k)
-                    if ((x >> k) & 1) o |= 1ull << pp.ob[nlo + k];
+                    if ((x >> k) & 1) o |= 1ull << expand(pp.ob[nlo + k]);
                 d.bhi[x] = o;
             }
         }
         std::vector<TmTerm> tmh;
         ctx->tm_ncat.assign(4 * (size_t)npass, 0);
-        for (int p = 0; p < npass; ++p) build_tm_terms(pl, pl.passes[p], tpb[p], colbits, tmh, &ctx->tm_ncat[4 * p]);
+        for (int p = 0; p < npass; ++p)
+            build_tm_terms(pl, pl.passes[p], tpb[p], cg, gap, tmh, &ctx->tm_ncat[4 * p]);
         if (tmh.empty()) tmh.push_back(TmTerm{});
         PNED_HIP(hipStreamSynchronize(ctx->stream));
         if (ctx->pd_dev) PNED_HIP(hipFree(ctx->pd_dev));
@@ -944,7 +960,7 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
             const PassPlan &pp = pl.passes[p];
             if (!pp.diag) continue;
             uint32_t slot_bits = 0;
-            for (int k = __builtin_ctz(tpb[p]); k < pp.K; ++k) slot_bits |= 1u << (pp.tb[k] - colbits);
+            for (int k = __builtin_ctz(tpb[p]); k < pp.K; ++k) slot_bits |= 1u << (pp.tb[k] - cg);
             DiagDev dt;
             if (split_diag_slots(ctx->diag_host, slot_bits, dt)) {
                 PNED_HIP(hipMalloc((void **)&ctx->d_diag_tm, sizeof(DiagDev)));
@@ -952,7 +968,22 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
             }
         }
         ctx->pd_threads = TPBpref;
+        ctx->pd_gap = gap;
     }
+    // blocks per pass: at least 8 contiguous tiles each (so a block's next-tile loads overlap
+    // its terms for most of its tiles) while that leaves 256 or more, at most grid_cap.  At
+    // N=24 (2048 tiles of 2^13) 256 blocks ran 0.524 ms per H.v against 0.532 with 1024 and
+    // 0.554 with 2048; at N=32 4096 blocks of 128 tiles beat 1024 and 256 by about 0.5 ms
+    auto pass_grid = [&](int p) {
+        const uint64_t tc = 1ull << pl.passes[p].ob.size();
+        const uint64_t want = std::max<uint64_t>(256, tc / 8);
+        return (int)std::min<uint64_t>(std::min<uint64_t>(tc, want), (uint64_t)grid_cap);
+    };
+    // partial rows: per column group [pass][block] of 2 ncol_t values, the groups one after another
+    int rows_grp = 0;
+    for (int p = 0; p < npass; ++p) rows_grp += pass_grid(p);
+    const bool c128 = v1->dtype == PNED_C128;
+    const size_t esz = c128 ? 16 : 8;
     int total_rows = 0;
     for (int p = 0; p < npass; ++p) {
         const PassPlan &pp = pl.passes[p];
@@ -966,7 +997,7 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
         a.nterm = pp.nterm;
         a.diag = pp.diag ? 1 : 0;
         a.fresh = fresh ? 1 : 0;
-        a.ncol = nvec;
+        a.ncol = ncol_t;
         a.shift = -(double)ctx->ibond * eperbd;
         a.nop = pl.nop;
         a.colcoef = ctx->d_colcoef;
@@ -976,17 +1007,9 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
         for (int c = 0; c < 4; ++c) a.ncat[c] = ctx->tm_ncat[4 * p + c];
         a.ngroups = ctx->diag_host.ngroups;
         const int TPB = tpb[p], R = rr[p];
-        // blocks: at least 8 contiguous tiles each (so a block's next-tile loads overlap its
-        // terms for most of its tiles) while that leaves 256 or more, at most grid_cap.  At N=24
-        // (2048 tiles of 2^13) 256 blocks ran 0.524 ms per H.v against 0.532 with 1024 and 0.554
-        // with 2048; at N=32 4096 blocks of 128 tiles beat 1024 and 256 by about 0.5 ms
-        const uint64_t want = std::max<uint64_t>(256, a.tile_count / 8);
-        int grid = (int)std::min<uint64_t>(std::min<uint64_t>(a.tile_count, want), (uint64_t)grid_cap);
-        double *part = part_base + (size_t)total_rows * 2 * nvec;
+        const int grid = pass_grid(p);
         const PassDev *pdp = reinterpret_cast<const PassDev *>(ctx->pd_dev) + p;
-        int rc;
-        const bool c128 = v1->dtype == PNED_C128;
-        const size_t tile_bytes = (c128 ? 16 : 8) * (size_t)TPB * R;
+        const size_t tile_bytes = esz * (size_t)TPB * R;
         const bool tm = ctx->s12_kernel == 1 && (c128 ? has_tm<double>(TPB, R) : has_tm<float>(TPB, R)) &&
                         tile_bytes + tm_coef_bytes(a) <= LDS_LIMIT;
         const DiagDev *dgp = (pp.diag && ctx->d_diag_tm) ? ctx->d_diag_tm : ctx->d_diag;
@@ -994,6 +1017,7 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
             // deferred-normalisation Lanczos step: this pass's <v1|H v1> share first (no v0
             // traffic), alpha from all passes on the device, then the pass itself forms
             // v0 - Re(alpha) s v1 and the partials of beta^2
+            double *part = part_base + (size_t)total_rows * 2 * nvec;
             a.lz_mode = 1;
             PNED_TRY(dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, dgp, pdp, a, part, grid));
             total_rows += grid;
@@ -1010,24 +1034,38 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
             *lz_fused = true;
             return PNED_OK;
         }
-        if (tm) {
-            // the term table is in pass order, so the kernel indexes it by first_term as well
-            if (c128)
-                rc = dispatch_tm<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, dgp, pdp, a, part, grid);
-            else
-                rc = dispatch_tm<float>(ctx, TPB, R, v1->data, v0->data, ctx->d_tm, dgp, pdp, a, part, grid);
-        } else if (c128) {
-            rc = dispatch_pass<double>(ctx, TPB, R, v1->data, v0->data, ctx->d_diag, pdp, a, part, grid);
-        } else {
-            rc = dispatch_pass<float>(ctx, TPB, R, v1->data, v0->data, ctx->d_diag, pdp, a, part, grid);
+        for (int g = 0; g < ngrp; ++g) {
+            // column group g: columns g*ncol_t .. (g+1)*ncol_t-1 of every state
+            const size_t eo = (size_t)g * ncol_t * esz;
+            const void *x = static_cast<const char *>(v1->data) + eo;
+            void *y = static_cast<char *>(v0->data) + eo;
+            a.colscale = d_colscale ? d_colscale + (size_t)g * ncol_t : nullptr;
+            a.yscale = d_yscale ? d_yscale + (size_t)g * ncol_t : nullptr;
+            double *part = part_base + ((size_t)g * rows_grp + total_rows) * 2 * ncol_t;
+            int rc;
+            if (tm) {
+                // the term table is in pass order, so the kernel indexes it by first_term as well
+                if (c128)
+                    rc = dispatch_tm<double>(ctx, TPB, R, x, y, ctx->d_tm, dgp, pdp, a, part, grid);
+                else
+                    rc = dispatch_tm<float>(ctx, TPB, R, x, y, ctx->d_tm, dgp, pdp, a, part, grid);
+            } else if (c128) {
+                rc = dispatch_pass<double>(ctx, TPB, R, x, y, ctx->d_diag, pdp, a, part, grid);
+            } else {
+                rc = dispatch_pass<float>(ctx, TPB, R, x, y, ctx->d_diag, pdp, a, part, grid);
+            }
+            if (rc != PNED_OK) return rc;
         }
-        if (rc != PNED_OK) return rc;
         total_rows += grid;
     }
     const int ncol2 = 2 * nvec;
-    hipLaunchKernelGGL(reduce_partials, dim3(ncol2), dim3(256), 0, ctx->stream, part_base, total_rows, ncol2,
-                       noalloc ? result_ext : ctx->d_result, 0);
-    PNED_HIP(hipGetLastError());
+    double *const out = noalloc ? result_ext : ctx->d_result;
+    for (int g = 0; g < ngrp; ++g) {
+        hipLaunchKernelGGL(reduce_partials, dim3(2 * ncol_t), dim3(256), 0, ctx->stream,
+                           part_base + (size_t)g * rows_grp * 2 * ncol_t, rows_grp, 2 * ncol_t,
+                           out + (size_t)g * 2 * ncol_t, 0);
+        PNED_HIP(hipGetLastError());
+    }
     if (noalloc) return PNED_OK;
     if (prdct_host) {
         PNED_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_result, sizeof(double) * ncol2,

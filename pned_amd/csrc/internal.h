@@ -248,6 +248,7 @@ struct pned_ctx {
     double *d_colcoef = nullptr;     // batch contexts: per-column term coefficients
     double *d_coldiag = nullptr;     // batch contexts: per-column diagonal weights
     int pd_threads = -1;
+    int pd_gap = -1;                 // column-group bits the address tables were built for
     // spin-S
     pned::GpuTables spin;
     int spin_tiled = 1;              // tiled kernel (q = 3..5); 0 = per-state gather kernel
