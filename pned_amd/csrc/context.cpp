@@ -1001,6 +1001,7 @@ bool split_diag_slots(const DiagDev &dh, uint32_t slot_bits, DiagDev &out)
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
 {
     if (ctx->plan_colbits == colbits && ctx->plan_dtype == dtype) return PNED_OK;
+    ctx->plan_req_colbits = -1;   // set again by ensure_s12_plan_cols once it has chosen
     // the whole tile sits in LDS (160 KB on gfx950): 2^13 complex128 or 2^14 complex64.
     // Untuned default (the autotuner's usual choice): vectors of up to 2^25 elements two
     // bits below that limit (small tiles, 256 threads, more blocks in flight), larger
@@ -1049,6 +1050,24 @@ int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype)
     ctx->plan_dtype = dtype;
     ctx->pd_threads = -1;    // address tables follow the new plan
     return upload_batch_coefficients(ctx);
+}
+
+int ensure_s12_plan_cols(pned_ctx *ctx, int colbits, int run_bits, int dtype, int *cg)
+{
+    if (ctx->plan_req_colbits == colbits && ctx->plan_colbits >= 0 && ctx->plan_dtype == dtype) {
+        *cg = ctx->plan_colbits;
+        return PNED_OK;
+    }
+    PNED_TRY(ensure_s12_plan(ctx, colbits, dtype));
+    *cg = colbits;
+    if (ctx->s12.nop <= 1 && colbits > run_bits) {
+        const size_t whole = ctx->s12.passes.size();
+        PNED_TRY(ensure_s12_plan(ctx, run_bits, dtype));
+        if (ctx->s12.passes.size() < whole) *cg = run_bits;
+        else PNED_TRY(ensure_s12_plan(ctx, colbits, dtype));   // the same plan again (deterministic)
+    }
+    ctx->plan_req_colbits = colbits;
+    return PNED_OK;
 }
 
 int upload_batch_coefficients(pned_ctx *ctx)

@@ -859,17 +859,20 @@ static int hv_s12_run(pned_ctx *ctx, const pned_vec *v1, pned_vec *v0, double ep
     // (one launch per group and pass, base pointers moved by the group), so a tile spends its
     // other bits on sites and a pass covers more terms (N=24 with 64 columns: 4 -> 3 passes).
     // Plan positions at or above cg move up by gap bits in memory (expand).
+    // Groups are taken only where they save a pass (ensure_s12_plan_cols).
     const int run_bits = __builtin_ctzll(COL_GROUP_BYTES / (v1->dtype == PNED_C128 ? 16 : 8));
-    const int cg = (ctx->s12.nop <= 1 && colbits > run_bits) ? run_bits : colbits;
-    const int gap = colbits - cg, ngrp = 1 << gap, ncol_t = 1 << cg;
-    auto expand = [cg, gap](int b) { return b < cg ? b : b + gap; };
+    int cg = colbits;
     if (noalloc) {
-        if (ctx->plan_colbits != cg || ctx->pd_gap != gap || ctx->plan_dtype != v1->dtype || ctx->pd_dev == nullptr)
+        if (ctx->plan_req_colbits != colbits || ctx->plan_colbits < 0 || ctx->plan_dtype != v1->dtype ||
+            ctx->pd_dev == nullptr || ctx->pd_gap != colbits - ctx->plan_colbits)
             return fail(PNED_E_ARG, "no H.v plan for this vector shape yet: call pned_p_mltply_v3 once with "
                                     "vectors of this shape first (that call allocates the plan's tables)");
+        cg = ctx->plan_colbits;
     } else {
-        PNED_TRY(ensure_s12_plan(ctx, cg, v1->dtype));
+        PNED_TRY(ensure_s12_plan_cols(ctx, colbits, run_bits, v1->dtype, &cg));
     }
+    const int gap = colbits - cg, ngrp = 1 << gap, ncol_t = 1 << cg;
+    auto expand = [cg, gap](int b) { return b < cg ? b : b + gap; };
     const S12Plan &pl = ctx->s12;
     if (pl.nop > 1 && pl.nop != nvec)
         return fail(PNED_E_ARG, "a batch context's vectors need one column per operator");

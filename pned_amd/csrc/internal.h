@@ -249,6 +249,7 @@ struct pned_ctx {
     double *d_coldiag = nullptr;     // batch contexts: per-column diagonal weights
     int pd_threads = -1;
     int pd_gap = -1;                 // column-group bits the address tables were built for
+    int plan_req_colbits = -1;       // column bits of the vectors the current plan was chosen for
     // spin-S
     pned::GpuTables spin;
     int spin_tiled = 1;              // tiled kernel (q = 3..5); 0 = per-state gather kernel
@@ -303,6 +304,11 @@ int lanczos_lazy_restore(pned_ctx *ctx, pned_vec *v1, pned_vec *v0, bool swapped
 int lanczos_lazy_setup(pned_ctx *ctx, const pned_vec *x, const pned_vec *y);
 int lanczos_lazy_enqueue(pned_ctx *ctx, pned_vec *x, pned_vec *y, bool fresh, double *st, double *slot);
 int ensure_s12_plan(pned_ctx *ctx, int colbits, int dtype);
+// the plan for 2^colbits interleaved columns: with column groups (tiles holding run_bits
+// column bits, the higher ones fixed per launch) when that takes fewer passes than all
+// columns in the tile (single-operator contexts); *cg = the plan's column bits.  The choice
+// is kept until the plan is rebuilt.
+int ensure_s12_plan_cols(pned_ctx *ctx, int colbits, int run_bits, int dtype, int *cg);
 // dh's lane program split by register bits (slot_bits: state bits held in register slots,
 // in the diagonal's state coordinates); false when it does not fit (more than 4 classes or
 // 48 lanes), then out is unchanged
