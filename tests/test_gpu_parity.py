@@ -648,7 +648,9 @@ def test_tps_fused_c64_n24_r64():
     (l - H/N) with the bond-norm shift.  The fused complex64 run against the fused complex128
     run on the same 64 start vectors (every column's E/N at every step within 1e-6), and the
     first steps of two columns of each against the oracle's mTPQ restatement of the same start
-    vector (the oracle takes about 5 s per step at this size)."""
+    vector (the oracle takes about 5 s per step at this size).  Both precisions run the H.V in
+    column groups (one 256-byte run of columns per tile), which takes 3 passes here where all
+    64 columns in the tile take 4."""
     _say("tps fused c64 N=24 R=64")
     cl = lattice.torus(3, 4)
     lb = model.bond_norm_shift(cl, 20, 90.0)
@@ -663,6 +665,7 @@ def test_tps_fused_c64_n24_r64():
         for c in range(R):
             ctx.set_initial_column(v1, c, seeds[c])
         runs[dtype] = ctx.tps_fused(v1, va, vb, op.shift_L, k)
+        assert ctx.plan_info()[0] == 3, dtype
         for v in (v1, va, vb):
             v.free()
     a64, a128 = runs[gpu.C64][0], runs[gpu.C128][0]
